@@ -1,0 +1,1043 @@
+// CPU oracle -- test infrastructure only.  Never linked into the product.
+//
+// Literal CPU restatement of PeriodicDMKTreeT<Real,3> construction and evaluate():
+//   * Impl ctor / evaluate ........ cpp/src/periodic_dmk.cpp:41-186
+//   * lattice_setup / compute_grid  cpp/include/dmk/pbc.hpp:23-117
+//   * images with source index .... cpp/include/pdmk/image_layout.hpp:14-88
+//   * rect grid + bin sort ........ cpp/include/dmk/pbc.hpp:265-391
+//   * PBCTree ctor (refine, 2:1) .. cpp/src/dmk/pbc_tree.cpp:138-740
+//   * metadata (flags, lists) ..... cpp/src/dmk/pbc_tree.cpp:746-1011
+//   * upward pass ................. cpp/src/dmk/pbc_tree.cpp:1139-1206, cpp/src/dmk/proxy.cpp:153-212,
+//                                   cpp/src/dmk/tensorprod.cpp:39-75
+//   * outgoing PW ................. cpp/src/dmk/pbc_tree.cpp:1208-1249, cpp/src/dmk/proxy.cpp:42-90
+//   * form_eval ................... cpp/src/dmk/pbc_tree.cpp:1251-1440, cpp/include/dmk/tree.hpp:287-316,
+//                                   cpp/src/dmk/planewave.cpp:49-105, cpp/src/dmk/proxy.cpp:307-448
+//   * direct interactions ......... cpp/src/dmk/pbc_tree.cpp:1442-1681, cpp/include/dmk/tree.hpp:111-220,
+//                                   cpp/include/dmk/vector_kernels.hpp:354-421, 495-533
+//   * self correction / desort .... cpp/src/dmk/pbc_tree.cpp:1759-1821
+//   * far field ................... cpp/src/dmk/pbc.cpp:378-541 (3-D, non-QP)
+//
+// Deliberate differences (all below the n_digits tolerance):
+//   * tables are computed in long double / double and cast to Real;
+//   * the residual kernel uses an exact 1/sqrt instead of SCTL's approx_rsqrt;
+//   * the far field is the exact truncated Fourier sum evaluated as a direct DFT
+//     in FP64 (DUCC's NUFFT approximates the same sum to eps/4).
+#pragma once
+
+#include "otables.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle {
+
+template <typename Real>
+void lattice_setup(const Real *avec, Real *umat, Real *qmat) {
+    constexpr int DIM = 3;
+    std::array<Real, 3> anorm;
+    std::array<int, 3> iord;
+    for (int i = 0; i < DIM; i++) {
+        Real s = 0;
+        for (int k = 0; k < DIM; k++) s += avec[k + i * DIM] * avec[k + i * DIM];
+        anorm[i] = std::sqrt(s);
+        iord[i] = i;
+    }
+    for (int i = 0; i < DIM - 1; i++)
+        for (int j = i + 1; j < DIM; j++)
+            if (anorm[j] > anorm[i]) { std::swap(anorm[i], anorm[j]); std::swap(iord[i], iord[j]); }
+    Real asort[9];
+    for (int i = 0; i < DIM; i++)
+        for (int j = 0; j < DIM; j++) asort[j + i * DIM] = avec[j + iord[i] * DIM];
+    Real r11 = anorm[0];
+    for (int i = 0; i < DIM; i++) qmat[i] = asort[i] / r11;
+    Real r12 = 0;
+    for (int i = 0; i < DIM; i++) r12 += qmat[i] * asort[i + DIM];
+    for (int i = 0; i < DIM; i++) qmat[i + DIM] = asort[i + DIM] - r12 * qmat[i];
+    Real r22 = 0;
+    for (int i = 0; i < DIM; i++) r22 += qmat[i + DIM] * qmat[i + DIM];
+    r22 = std::sqrt(r22);
+    for (int i = 0; i < DIM; i++) qmat[i + DIM] /= r22;
+    Real r13 = 0, r23 = 0, r33 = 0;
+    for (int i = 0; i < 3; i++) { r13 += qmat[i] * asort[i + 2 * DIM]; r23 += qmat[i + DIM] * asort[i + 2 * DIM]; }
+    for (int i = 0; i < 3; i++) qmat[i + 2 * DIM] = asort[i + 2 * DIM] - r13 * qmat[i] - r23 * qmat[i + DIM];
+    for (int i = 0; i < 3; i++) r33 += qmat[i + 2 * DIM] * qmat[i + 2 * DIM];
+    r33 = std::sqrt(r33);
+    for (int i = 0; i < 3; i++) qmat[i + 2 * DIM] /= r33;
+    for (int i = 0; i < 9; i++) umat[i] = 0;
+    umat[0] = r11; umat[3] = r12; umat[4] = r22; umat[6] = r13; umat[7] = r23; umat[8] = r33;
+}
+
+template <typename Real>
+void compute_grid(const Real *umat, Real rc, int *mgrid, Real *xgrid0) {
+    for (int i = 2; i >= 0; i--) {
+        Real xmin = 0, xmax = umat[i + i * 3];
+        for (int j = i + 1; j < 3; j++) {
+            Real v = umat[i + j * 3];
+            xmin += std::min(Real(0), v);
+            xmax += std::max(Real(0), v);
+        }
+        mgrid[i] = std::max(1, (int)std::ceil((xmax - xmin) / rc));
+        xgrid0[i] = xmin;
+    }
+}
+
+template <typename Real>
+void images_with_src_idx(const Real *umat, Real rc, const int *mgrid, const Real *xgrid0, int ns,
+                         const Real *src, std::vector<Real> &img, std::vector<int> &idx) {
+    Real xextlo[3], xexthi[3], xlo[3], xhi[3];
+    for (int i = 0; i < 3; ++i) {
+        xextlo[i] = xgrid0[i] - rc;
+        xexthi[i] = xgrid0[i] + (mgrid[i] + 1) * rc;
+        xlo[i] = xgrid0[i];
+        Real w = umat[i + i * 3];
+        for (int j = i + 1; j < 3; ++j) w += std::abs(umat[i + j * 3]);
+        xhi[i] = xgrid0[i] + w;
+    }
+    img.clear();
+    idx.clear();
+    int n3max = (int)((mgrid[2] + 1) * rc / umat[8]) + 1;
+    for (int n3 = -n3max; n3 <= n3max; ++n3) {
+        Real s3 = umat[8] * n3;
+        if (xlo[2] + s3 > xexthi[2] || xhi[2] + s3 < xextlo[2]) continue;
+        int n2max = (int)(((mgrid[1] + 1) * rc + std::abs(umat[7] * n3)) / umat[4]) + 1;
+        for (int n2 = -n2max; n2 <= n2max; ++n2) {
+            Real s2 = umat[4] * n2 + umat[7] * n3;
+            if (xlo[1] + s2 > xexthi[1] || xhi[1] + s2 < xextlo[1]) continue;
+            Real off12 = umat[3] * n2 + umat[6] * n3;
+            int n1max = (int)(((mgrid[0] + 1) * rc + std::abs(off12)) / umat[0]) + 1;
+            for (int n1 = -n1max; n1 <= n1max; ++n1) {
+                if (n1 == 0 && n2 == 0 && n3 == 0) continue;
+                Real s1 = umat[0] * n1 + off12;
+                if (xlo[0] + s1 > xexthi[0] || xhi[0] + s1 < xextlo[0]) continue;
+                for (int j = 0; j < ns; ++j) {
+                    Real x0 = src[j * 3] + s1, x1 = src[j * 3 + 1] + s2, x2 = src[j * 3 + 2] + s3;
+                    if (x0 < xextlo[0] || x0 > xexthi[0] || x1 < xextlo[1] || x1 > xexthi[1] || x2 < xextlo[2] ||
+                        x2 > xexthi[2])
+                        continue;
+                    img.push_back(x0); img.push_back(x1); img.push_back(x2);
+                    idx.push_back(j);
+                }
+            }
+        }
+    }
+}
+
+template <typename Real>
+struct Tree {
+    static constexpr int mc = 8, ncoll = 27;
+    // parameters
+    double eps = 0;
+    int n_digits = 0;   // round(log10(1/eps) - 0.1)
+    int n_per_leaf = 0;
+    // geometry (Real, as the reference)
+    Real avec[9], umat[9], qmat[9], xgrid0[3], rc, beta;
+    int mgrid[3];
+    int n_src = 0, nsall = 0;
+    std::vector<Real> r_src;       // rotated originals [3*n_src]
+    std::vector<int> img_src_idx;  // per image, originating source
+    // topology
+    int nboxes = 0;
+    std::vector<int> depth, parent, nchild;
+    std::vector<std::array<int, 8>> children;
+    std::vector<std::array<int, 27>> coll;
+    std::vector<std::vector<int>> level_indices;
+    std::vector<Real> boxsize, centers;
+    std::vector<int> src_cnt, trg_cnt, src_start, trg_start;
+    std::vector<Real> r_src_sorted, r_trg_sorted, charge_sorted, pot_trg_sorted;
+    std::vector<int> src_perm, trg_perm;
+    // metadata
+    std::vector<char> ifpwexp, iftensprodeval, is_leaf, has_proxy_from_children;
+    std::vector<std::vector<int>> list1;
+    struct Work { int src_box, center_box, level; };
+    std::vector<Work> c2p_work;
+    std::vector<int> direct_work;
+    std::vector<long> up_off, down_off, pw_off;
+    std::vector<Real> proxy_up, proxy_down;
+    std::vector<std::complex<Real>> pw_out;
+    std::vector<char> down_zeroed;
+    // tables
+    Tables tab;
+    int p = 0, n_pw = 0, n_modes = 0;
+    std::vector<Real> c2pR, p2cR, w0R, coeffs;
+    std::vector<std::vector<std::complex<Real>>> poly2pwR, pw2polyR, wpwshiftR;
+    // far field
+    int ms = 0, mt = 0, mu = 0;
+    std::vector<double> ff_weight;   // [mu][mt][ms]
+
+    int n_levels() const { return (int)level_indices.size(); }
+    const Real *center(int b) const { return &centers[3 * b]; }
+
+    Tree(const double *cell_colmajor, const double *pos, int N, int n_digits_in, int npl, int nz) {
+        n_per_leaf = npl;
+        eps = std::pow(10.0, -std::max(1, n_digits_in));
+        n_digits = std::max(1, (int)std::round(std::log10(1.0 / eps) - 0.1));
+        n_src = N;
+        for (int i = 0; i < 9; ++i) avec[i] = (Real)cell_colmajor[i];
+        lattice_setup<Real>(avec, umat, qmat);
+        if (nz <= 0) throw std::runtime_error("oracle: nz must be > 0");
+        Real min_axis = std::min(umat[0], std::min(umat[4], umat[8]));
+        rc = min_axis / nz;
+        beta = (Real)beta_for_eps(eps);
+        compute_grid<Real>(umat, rc, mgrid, xgrid0);
+        r_src.resize(3 * (size_t)N);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < 3; ++j) {
+                Real s = 0;
+                for (int k = 0; k < 3; ++k) s += qmat[k + j * 3] * (Real)pos[3 * i + k];
+                r_src[3 * (size_t)i + j] = s;
+            }
+        std::vector<Real> img;
+        images_with_src_idx<Real>(umat, rc, mgrid, xgrid0, N, r_src.data(), img, img_src_idx);
+        nsall = N + (int)img_src_idx.size();
+        std::vector<Real> srcall(3 * (size_t)nsall);
+        std::copy(r_src.begin(), r_src.end(), srcall.begin());
+        std::copy(img.begin(), img.end(), srcall.begin() + 3 * (size_t)N);
+        build_tree(srcall);
+        metadata();
+        far_field_setup();
+    }
+
+    // ---------------------------------------------------------------- tree build
+    void sort_children(int b, int n_pts, int start, std::vector<Real> &pos, std::vector<int> &perm,
+                       std::vector<int> &se, const std::vector<Real> &cen) {
+        if (n_pts <= 0) return;
+        const Real *pc = &cen[3 * b];
+        std::vector<int> ch(n_pts);
+        int cnt[8] = {};
+        for (int i = 0; i < n_pts; ++i) {
+            int ci = 0;
+            for (int d = 0; d < 3; ++d)
+                if (pos[3 * (size_t)(start + i) + d] >= pc[d]) ci |= 1 << d;
+            ch[i] = ci;
+            cnt[ci]++;
+        }
+        int off[8];
+        off[0] = start;
+        for (int c = 1; c < 8; ++c) off[c] = off[c - 1] + cnt[c - 1];
+        for (int c = 0; c < 8; ++c) {
+            int cb = children[b][c];
+            se[2 * cb] = off[c];
+            se[2 * cb + 1] = off[c] + cnt[c] - 1;
+        }
+        std::vector<Real> tp(pos.begin() + 3 * (size_t)start, pos.begin() + 3 * (size_t)(start + n_pts));
+        std::vector<int> tperm(perm.begin() + start, perm.begin() + start + n_pts);
+        int run[8];
+        std::memcpy(run, off, sizeof(off));
+        for (int i = 0; i < n_pts; ++i) {
+            int g = run[ch[i]]++;
+            for (int d = 0; d < 3; ++d) pos[3 * (size_t)g + d] = tp[3 * (size_t)i + d];
+            perm[g] = tperm[i];
+        }
+    }
+
+    void bin_sort(const int *mle, const Real *xelo, int nb0, int ns, const Real *pts, int *perm, int *se) {
+        const Real rcinv = Real(1) / rc;
+        std::vector<int> cnt(nb0, 0), boxof(ns);
+        for (int i = 0; i < ns; ++i) {
+            int ii[3];
+            for (int k = 0; k < 3; ++k) {
+                ii[k] = (int)((pts[3 * (size_t)i + k] - xelo[k]) * rcinv);
+                if (ii[k] < 0) ii[k] = 0;
+                if (ii[k] >= mle[k]) ii[k] = mle[k] - 1;
+            }
+            int b = ii[2] * mle[0] * mle[1] + ii[1] * mle[0] + ii[0];
+            boxof[i] = b;
+            cnt[b]++;
+        }
+        std::vector<int> ip(nb0 + 1, 0);
+        for (int b = 0; b < nb0; ++b) {
+            ip[b + 1] = ip[b] + cnt[b];
+            se[2 * b] = ip[b];
+            se[2 * b + 1] = ip[b + 1] - 1;
+        }
+        for (int i = 0; i < ns; ++i) perm[ip[boxof[i]]++] = i;
+    }
+
+    int add_box(int par, int lev) {
+        depth.push_back(lev);
+        parent.push_back(par);
+        nchild.push_back(0);
+        std::array<int, 8> c;
+        c.fill(-1);
+        children.push_back(c);
+        std::array<int, 27> q;
+        q.fill(-1);
+        coll.push_back(q);
+        return nboxes++;
+    }
+
+    void child_colleagues(int nb, Real child_bs, const std::vector<Real> &cen) {
+        coll[nb].fill(-1);
+        const Real dp1 = 1.05 * child_bs;
+        int dad = parent[nb];
+        for (int ic = 0; ic < 27; ++ic) {
+            int jb = coll[dad][ic];
+            if (jb < 0 || nchild[jb] == 0) continue;
+            for (int jc = 0; jc < 8; ++jc) {
+                int kb = children[jb][jc];
+                if (kb < 0) continue;
+                bool adj = true;
+                for (int d = 0; d < 3; ++d)
+                    if (std::abs(cen[3 * nb + d] - cen[3 * kb + d]) > dp1) { adj = false; break; }
+                if (!adj) continue;
+                int s = 0, stride = 1;
+                for (int d = 0; d < 3; ++d) {
+                    Real diff = cen[3 * kb + d] - cen[3 * nb + d];
+                    int di = (int)std::round(diff / child_bs);
+                    s += (di + 1) * stride;
+                    stride *= 3;
+                }
+                if (s >= 0 && s < 27) coll[nb][s] = kb;
+            }
+        }
+    }
+
+    void refine_box(int b, int L, std::vector<Real> &cen, std::vector<int> &src_se, std::vector<int> &trg_se,
+                    std::vector<Real> &ssort, std::vector<Real> &tsort) {
+        const Real bsh = rc / (Real)(1 << (L + 2));
+        nchild[b] = 8;
+        for (int c = 0; c < 8; ++c) {
+            int id = add_box(b, L + 1);
+            children[b][c] = id;
+            cen.resize(3 * (size_t)nboxes);
+            for (int d = 0; d < 3; ++d) cen[3 * id + d] = cen[3 * b + d] + (((c >> d) & 1) ? 1 : -1) * bsh;
+            src_se.resize(2 * (size_t)nboxes);
+            trg_se.resize(2 * (size_t)nboxes);
+            src_se[2 * id] = 0; src_se[2 * id + 1] = -1;
+            trg_se[2 * id] = 0; trg_se[2 * id + 1] = -1;
+        }
+        int npts = src_se[2 * b + 1] - src_se[2 * b] + 1;
+        if (npts > 0) sort_children(b, npts, src_se[2 * b], ssort, src_perm, src_se, cen);
+        int tn = trg_se[2 * b + 1] - trg_se[2 * b] + 1;
+        if (tn > 0) sort_children(b, tn, trg_se[2 * b], tsort, trg_perm, trg_se, cen);
+    }
+
+    void build_tree(const std::vector<Real> &srcall) {
+        int mle[3];
+        Real xelo[3];
+        for (int i = 0; i < 3; ++i) { mle[i] = mgrid[i] + 2; xelo[i] = xgrid0[i] - rc; }
+        const int nb0 = mle[0] * mle[1] * mle[2];
+        std::vector<Real> cen(3 * (size_t)nb0);
+        for (int iz = 0, b = 0; iz < mle[2]; ++iz)
+            for (int iy = 0; iy < mle[1]; ++iy)
+                for (int ix = 0; ix < mle[0]; ++ix, ++b) {
+                    add_box(-1, 0);
+                    cen[3 * b + 0] = xelo[0] + (ix + Real(0.5)) * rc;
+                    cen[3 * b + 1] = xelo[1] + (iy + Real(0.5)) * rc;
+                    cen[3 * b + 2] = xelo[2] + (iz + Real(0.5)) * rc;
+                    int id = 0;
+                    for (int kz = -1; kz <= 1; ++kz)
+                        for (int ky = -1; ky <= 1; ++ky)
+                            for (int kx = -1; kx <= 1; ++kx, ++id) {
+                                int jx = ix + kx, jy = iy + ky, jz = iz + kz;
+                                coll[b][id] = (jx < 0 || jx >= mle[0] || jy < 0 || jy >= mle[1] || jz < 0 ||
+                                               jz >= mle[2])
+                                                  ? -1
+                                                  : jz * mle[0] * mle[1] + jy * mle[0] + jx;
+                            }
+                }
+        src_perm.resize(nsall);
+        std::vector<int> src_se(2 * (size_t)nb0), trg_se(2 * (size_t)nb0);
+        bin_sort(mle, xelo, nb0, nsall, srcall.data(), src_perm.data(), src_se.data());
+        trg_perm.resize(n_src);
+        bin_sort(mle, xelo, nb0, n_src, r_src.data(), trg_perm.data(), trg_se.data());
+        std::vector<Real> ssort(3 * (size_t)nsall), tsort(3 * (size_t)n_src);
+        for (int i = 0; i < nsall; ++i)
+            for (int d = 0; d < 3; ++d) ssort[3 * (size_t)i + d] = srcall[3 * (size_t)src_perm[i] + d];
+        for (int i = 0; i < n_src; ++i)
+            for (int d = 0; d < 3; ++d) tsort[3 * (size_t)i + d] = r_src[3 * (size_t)trg_perm[i] + d];
+
+        const int ndiv_eff = (int)std::ceil(n_per_leaf * 1.5);
+        int max_level = 0;
+        for (int level = 0; level < 50; ++level) {
+            bool any = false;
+            std::vector<int> at;
+            for (int b = 0; b < nboxes; ++b)
+                if (depth[b] == level && nchild[b] == 0) at.push_back(b);
+            for (int b : at) {
+                int npts = src_se[2 * b + 1] - src_se[2 * b] + 1;
+                if (npts <= ndiv_eff) continue;
+                any = true;
+                refine_box(b, level, cen, src_se, trg_se, ssort, tsort);
+            }
+            if (!any) break;
+            max_level = level + 1;
+            const Real child_bs = rc / (Real)(1 << (level + 1));
+            for (int b = 0; b < nboxes; ++b)
+                if (depth[b] == level + 1) child_colleagues(b, child_bs, cen);
+        }
+
+        // 2:1 balance
+        bool changed = true;
+        int iter = 0;
+        while (changed && iter++ < 50) {
+            changed = false;
+            for (int L = 0; L <= max_level; ++L) {
+                const Real lvl_bs = rc / (Real)(1 << L);
+                const Real child_bs_L = rc / (Real)(1 << (L + 1));
+                const Real adj_tol = 1.05 * 0.5 * (lvl_bs + child_bs_L);
+                std::vector<int> leaves;
+                for (int b = 0; b < nboxes; ++b)
+                    if (depth[b] == L && nchild[b] == 0 && (src_se[2 * b + 1] - src_se[2 * b] + 1) > 0)
+                        leaves.push_back(b);
+                for (int b : leaves) {
+                    if (nchild[b] != 0) continue;
+                    bool violate = false;
+                    for (int ic = 0; ic < 27 && !violate; ++ic) {
+                        int cc = coll[b][ic];
+                        if (cc < 0 || cc == b || nchild[cc] == 0) continue;
+                        for (int jc = 0; jc < 8; ++jc) {
+                            int gc = children[cc][jc];
+                            if (gc < 0 || nchild[gc] == 0) continue;
+                            bool adj = true;
+                            for (int d = 0; d < 3; ++d)
+                                if (std::abs(cen[3 * b + d] - cen[3 * gc + d]) > adj_tol) { adj = false; break; }
+                            if (adj) { violate = true; break; }
+                        }
+                    }
+                    if (!violate) continue;
+                    refine_box(b, L, cen, src_se, trg_se, ssort, tsort);
+                    for (int c = 0; c < 8; ++c) child_colleagues(children[b][c], child_bs_L, cen);
+                    for (int c = 0; c < 8; ++c) {
+                        int nb = children[b][c];
+                        for (int ic = 0; ic < 27; ++ic) {
+                            int kb = coll[nb][ic];
+                            if (kb < 0 || kb == nb) continue;
+                            int s = 0, stride = 1;
+                            for (int d = 0; d < 3; ++d) {
+                                Real diff = cen[3 * nb + d] - cen[3 * kb + d];
+                                s += ((int)std::round(diff / child_bs_L) + 1) * stride;
+                                stride *= 3;
+                            }
+                            if (s >= 0 && s < 27 && coll[kb][s] != nb) coll[kb][s] = nb;
+                        }
+                    }
+                    changed = true;
+                    if (L + 1 > max_level) max_level = L + 1;
+                }
+            }
+        }
+
+        centers = cen;
+        centers.resize(3 * (size_t)nboxes);
+        const int nl = max_level + 1;
+        level_indices.assign(nl, {});
+        for (int b = 0; b < nboxes; ++b) level_indices[depth[b]].push_back(b);
+        boxsize.resize(nl + 1);
+        boxsize[0] = rc;
+        for (int l = 1; l <= nl; ++l) boxsize[l] = boxsize[l - 1] / Real(2);
+        is_leaf.resize(nboxes);
+        for (int b = 0; b < nboxes; ++b) is_leaf[b] = (nchild[b] == 0);
+        src_cnt.assign(nboxes, 0);
+        trg_cnt.assign(nboxes, 0);
+        src_start.resize(nboxes);
+        trg_start.resize(nboxes);
+        for (int b = 0; b < nboxes; ++b) {
+            if (is_leaf[b]) {
+                src_cnt[b] = std::max(0, src_se[2 * b + 1] - src_se[2 * b] + 1);
+                trg_cnt[b] = std::max(0, trg_se[2 * b + 1] - trg_se[2 * b] + 1);
+            }
+            src_start[b] = src_se[2 * b];
+            trg_start[b] = trg_se[2 * b];
+        }
+        for (int l = nl - 1; l >= 0; --l)
+            for (int b : level_indices[l])
+                if (parent[b] >= 0) { src_cnt[parent[b]] += src_cnt[b]; trg_cnt[parent[b]] += trg_cnt[b]; }
+        r_src_sorted = ssort;
+        r_trg_sorted = tsort;
+        charge_sorted.assign(nsall, 0);
+        pot_trg_sorted.assign(n_src, 0);
+    }
+
+    // ---------------------------------------------------------------- metadata
+    void metadata() {
+        const int nl = n_levels();
+        ifpwexp.assign(nboxes, 0);
+        for (int b : level_indices[0]) ifpwexp[b] = 1;
+        for (int l = 1; l < nl; ++l)
+            for (int b : level_indices[l]) {
+                if (nchild[b] > 0) { ifpwexp[b] = 1; continue; }
+                for (int ic = 0; ic < 27; ++ic) {
+                    int nb = coll[b][ic];
+                    if (nb >= 0 && nchild[nb] > 0) { ifpwexp[b] = 1; break; }
+                }
+            }
+        for (int b = 0; b < nboxes; ++b)
+            if (src_cnt[b] == 0 && trg_cnt[b] == 0) ifpwexp[b] = 0;
+        iftensprodeval.assign(nboxes, 0);
+        for (int b = 0; b < nboxes; ++b)
+            if (is_leaf[b] && src_cnt[b] + trg_cnt[b] > 0) iftensprodeval[b] = 1;
+
+        list1.assign(nboxes, {});
+        for (int l = 0; l < nl; ++l)
+            for (int box : level_indices[l]) {
+                if (!is_leaf[box]) continue;
+                auto &L1 = list1[box];
+                for (int ic = 0; ic < 27; ++ic) {
+                    int nb = coll[box][ic];
+                    if (nb < 0) continue;
+                    if (is_leaf[nb] && src_cnt[nb] > 0) {
+                        L1.push_back(nb);
+                    } else if (nchild[nb] > 0 && l + 1 < nl) {
+                        const Real cut = 1.05 * 0.75 * boxsize[l];
+                        for (int jc = 0; jc < 8; ++jc) {
+                            int kb = children[nb][jc];
+                            if (kb < 0 || src_cnt[kb] == 0 || !is_leaf[kb]) continue;
+                            bool in = true;
+                            for (int d = 0; d < 3; ++d)
+                                if (std::abs(center(box)[d] - center(kb)[d]) > cut) { in = false; break; }
+                            if (in) L1.push_back(kb);
+                        }
+                    }
+                }
+                if (l == 0) continue;
+                int dad = parent[box];
+                if (dad < 0) continue;
+                const Real cut2 = 1.5 * 1.05 * boxsize[l];
+                for (int ic = 0; ic < 27; ++ic) {
+                    int nb = coll[dad][ic];
+                    if (nb < 0 || !is_leaf[nb] || src_cnt[nb] == 0) continue;
+                    bool in = true;
+                    for (int d = 0; d < 3; ++d)
+                        if (std::abs(center(box)[d] - center(nb)[d]) > cut2) { in = false; break; }
+                    if (in) L1.push_back(nb);
+                }
+            }
+
+        has_proxy_from_children.assign(nboxes, 0);
+        c2p_work.clear();
+        for (int l = nl - 1; l >= 0; --l)
+            for (int b : level_indices[l]) {
+                if (!ifpwexp[b] || src_cnt[b] == 0) continue;
+                bool chp = false;
+                for (int c = 0; c < 8; ++c) {
+                    int ch = children[b][c];
+                    if (ch >= 0 && ifpwexp[ch] && src_cnt[ch] > 0) { chp = true; break; }
+                }
+                if (chp) {
+                    has_proxy_from_children[b] = 1;
+                    for (int c = 0; c < 8; ++c) {
+                        int ch = children[b][c];
+                        if (ch >= 0 && src_cnt[ch] > 0 && !ifpwexp[ch]) c2p_work.push_back({ch, b, l});
+                    }
+                } else {
+                    c2p_work.push_back({b, b, l});
+                }
+            }
+        direct_work.clear();
+        for (int b = 0; b < nboxes; ++b) {
+            if (!is_leaf[b] || list1[b].empty()) continue;
+            if (src_cnt[b] == 0 && trg_cnt[b] == 0) continue;
+            direct_work.push_back(b);
+        }
+
+        // tables
+        tab.build(eps, (double)beta, (double)rc, nl);
+        p = tab.n_order;
+        n_pw = tab.n_pw;
+        n_modes = tab.n_modes;
+        const int ncoef = p * p * p;
+        up_off.assign(nboxes, -1);
+        down_off.assign(nboxes, -1);
+        pw_off.assign(nboxes, -1);
+        long nu = 0, nd = 0, np = 0;
+        for (int b = 0; b < nboxes; ++b) {
+            if (ifpwexp[b] && src_cnt[b] > 0) { up_off[b] = nu; nu += ncoef; pw_off[b] = np; np += n_modes; }
+            if ((ifpwexp[b] || iftensprodeval[b]) && src_cnt[b] + trg_cnt[b] > 0) { down_off[b] = nd; nd += ncoef; }
+        }
+        proxy_up.assign(nu, 0);
+        proxy_down.assign(nd, 0);
+        pw_out.assign(np, 0);
+        down_zeroed.assign(nboxes, 0);
+
+        auto castv = [](const std::vector<double> &v) { return std::vector<Real>(v.begin(), v.end()); };
+        auto castc = [](const std::vector<std::complex<double>> &v) {
+            std::vector<std::complex<Real>> o(v.size());
+            for (size_t i = 0; i < v.size(); ++i) o[i] = {(Real)v[i].real(), (Real)v[i].imag()};
+            return o;
+        };
+        c2pR = castv(tab.c2p);
+        p2cR = castv(tab.p2c);
+        w0R = castv(tab.w0);
+        coeffs = castv(residual_coeffs(n_digits));
+        poly2pwR.clear(); pw2polyR.clear(); wpwshiftR.clear();
+        for (int l = 0; l < nl; ++l) {
+            poly2pwR.push_back(castc(tab.poly2pw[l]));
+            pw2polyR.push_back(castc(tab.pw2poly[l]));
+            wpwshiftR.push_back(castc(tab.wpwshift[l]));
+        }
+    }
+
+    // ---------------------------------------------------------------- transforms
+    static void cheb(int p, Real x, Real *T) {
+        T[0] = 1;
+        if (p > 1) T[1] = x;
+        for (int i = 2; i < p; ++i) T[i] = Real(2) * x * T[i - 1] - T[i - 2];
+    }
+
+    // C(a,b,c) += sum_s Tx_a Ty_b Tz_c q_s  (proxy.cpp:153-212)
+    void charge2proxy(int src_box, int center_box, int level, Real *C) {
+        const Real sc = Real(2.0) / boxsize[level];
+        const Real *cn = center(center_box);
+        std::vector<Real> tx(p), ty(p), tz(p), yz(p * p);
+        for (int s = 0; s < src_cnt[src_box]; ++s) {
+            const size_t i = src_start[src_box] + s;
+            const Real *r = &r_src_sorted[3 * i];
+            cheb(p, sc * (r[0] - cn[0]), tx.data());
+            cheb(p, sc * (r[1] - cn[1]), ty.data());
+            cheb(p, sc * (r[2] - cn[2]), tz.data());
+            const Real q = charge_sorted[i];
+            for (int c = 0; c < p; ++c)
+                for (int b = 0; b < p; ++b) yz[b + p * c] = ty[b] * (tz[c] * q);
+            for (int bc = 0; bc < p * p; ++bc)
+                for (int a = 0; a < p; ++a) C[a + p * bc] += tx[a] * yz[bc];
+        }
+    }
+
+    // fout(a',b',c') (+)= sum U0(a',a) U1(b',b) U2(c',c) fin(a,b,c)  (tensorprod.cpp:39-75)
+    void tensor3(bool add, const Real *fin, const Real *U, Real *fout) {
+        const int n = p;
+        const Real *U0 = U, *U1 = U + n * n, *U2 = U + 2 * n * n;
+        std::vector<Real> t1(n * n * n, 0), t2(n * n * n, 0);
+        for (int c2 = 0; c2 < n; ++c2)
+            for (int c = 0; c < n; ++c) {
+                const Real u = U2[c2 + n * c];
+                for (int ab = 0; ab < n * n; ++ab) t1[ab + n * n * c2] += u * fin[ab + n * n * c];
+            }
+        for (int c2 = 0; c2 < n; ++c2)
+            for (int b2 = 0; b2 < n; ++b2)
+                for (int b = 0; b < n; ++b) {
+                    const Real u = U1[b2 + n * b];
+                    for (int a = 0; a < n; ++a) t2[a + n * b2 + n * n * c2] += u * t1[a + n * b + n * n * c2];
+                }
+        for (int bc = 0; bc < n * n; ++bc)
+            for (int a2 = 0; a2 < n; ++a2) {
+                Real s = 0;
+                for (int a = 0; a < n; ++a) s += U0[a2 + n * a] * t2[a + n * bc];
+                fout[a2 + n * bc] = add ? fout[a2 + n * bc] + s : s;
+            }
+    }
+
+    // pw(k1,k2,k3) = sum P(k1,m1) P(k2,m2) P(k3,m3) C(m1,m2,m3), k3 < h  (proxy.cpp:42-90)
+    void proxy2pw(const Real *C, const std::complex<Real> *P, std::complex<Real> *pw) {
+        const int h = (n_pw + 1) / 2;
+        std::vector<std::complex<Real>> t1(p * p * h), t2(n_pw * p * h);
+        for (int k3 = 0; k3 < h; ++k3)
+            for (int m12 = 0; m12 < p * p; ++m12) {
+                std::complex<Real> s = 0;
+                for (int m3 = 0; m3 < p; ++m3) s += P[k3 + n_pw * m3] * C[m12 + p * p * m3];
+                t1[m12 + p * p * k3] = s;
+            }
+        for (int k3 = 0; k3 < h; ++k3)
+            for (int k2 = 0; k2 < n_pw; ++k2)
+                for (int m1 = 0; m1 < p; ++m1) {
+                    std::complex<Real> s = 0;
+                    for (int m2 = 0; m2 < p; ++m2) s += P[k2 + n_pw * m2] * t1[m1 + p * m2 + p * p * k3];
+                    t2[m1 + p * k2 + p * n_pw * k3] = s;
+                }
+        for (int k3 = 0; k3 < h; ++k3)
+            for (int k2 = 0; k2 < n_pw; ++k2)
+                for (int k1 = 0; k1 < n_pw; ++k1) {
+                    std::complex<Real> s = 0;
+                    for (int m1 = 0; m1 < p; ++m1) s += P[k1 + n_pw * m1] * t2[m1 + p * k2 + p * n_pw * k3];
+                    pw[k1 + n_pw * k2 + n_pw * n_pw * k3] = s;
+                }
+    }
+
+    // proxy(k1,k2,k3) += 2 Re sum W(m1,k1) W(m2,k2) hw(m3) W(m3,k3) pw(m1,m2,m3)  (planewave.cpp:49-105)
+    void pw2proxy(const std::complex<Real> *pw, const std::complex<Real> *W, Real *C) {
+        const int h = (n_pw + 1) / 2, half = n_pw / 2;
+        std::vector<std::complex<Real>> t1(p * n_pw * h), t2(p * p * h);
+        for (int m3 = 0; m3 < h; ++m3)
+            for (int m2 = 0; m2 < n_pw; ++m2)
+                for (int k1 = 0; k1 < p; ++k1) {
+                    std::complex<Real> s = 0;
+                    for (int m1 = 0; m1 < n_pw; ++m1) s += W[m1 + n_pw * k1] * pw[m1 + n_pw * m2 + n_pw * n_pw * m3];
+                    t1[k1 + p * m2 + p * n_pw * m3] = s;
+                }
+        for (int m3 = 0; m3 < h; ++m3)
+            for (int k2 = 0; k2 < p; ++k2)
+                for (int k1 = 0; k1 < p; ++k1) {
+                    std::complex<Real> s = 0;
+                    for (int m2 = 0; m2 < n_pw; ++m2) s += W[m2 + n_pw * k2] * t1[k1 + p * m2 + p * n_pw * m3];
+                    if (m3 >= half) s *= Real(0.5);
+                    t2[k1 + p * k2 + p * p * m3] = s;
+                }
+        for (int k3 = 0; k3 < p; ++k3)
+            for (int k12 = 0; k12 < p * p; ++k12) {
+                std::complex<Real> s = 0;
+                for (int m3 = 0; m3 < h; ++m3) s += t2[k12 + p * p * m3] * W[m3 + n_pw * k3];
+                C[k12 + p * p * k3] += s.real() * Real(2);
+            }
+    }
+
+    Real eval_proxy_at(const Real *C, const Real *r, const Real *cn, Real sc) {
+        std::vector<Real> tx(p), ty(p), tz(p);
+        cheb(p, sc * (r[0] - cn[0]), tx.data());
+        cheb(p, sc * (r[1] - cn[1]), ty.data());
+        cheb(p, sc * (r[2] - cn[2]), tz.data());
+        Real s = 0;
+        for (int c = 0; c < p; ++c)
+            for (int b = 0; b < p; ++b) {
+                const Real tyz = ty[b] * tz[c];
+                Real inner = 0;
+                for (int a = 0; a < p; ++a) inner += C[a + p * b + p * p * c] * tx[a];
+                s += inner * tyz;
+            }
+        return s;
+    }
+
+    // ---------------------------------------------------------------- passes
+    void update_charges(const Real *chgall) {
+        for (int i = 0; i < nsall; ++i) charge_sorted[i] = chgall[src_perm[i]];
+        std::fill(pot_trg_sorted.begin(), pot_trg_sorted.end(), Real(0));
+    }
+
+    void upward_pass() {
+        std::fill(proxy_up.begin(), proxy_up.end(), Real(0));
+        // group work by destination box so threads never share an output
+        std::vector<int> order(c2p_work.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return c2p_work[a].center_box < c2p_work[b].center_box; });
+        std::vector<int> starts;
+        for (size_t i = 0; i < order.size(); ++i)
+            if (i == 0 || c2p_work[order[i]].center_box != c2p_work[order[i - 1]].center_box) starts.push_back(i);
+        starts.push_back(order.size());
+#pragma omp parallel for schedule(dynamic)
+        for (int g = 0; g < (int)starts.size() - 1; ++g)
+            for (int k = starts[g]; k < starts[g + 1]; ++k) {
+                const Work &w = c2p_work[order[k]];
+                charge2proxy(w.src_box, w.center_box, w.level, &proxy_up[up_off[w.center_box]]);
+            }
+        for (int l = n_levels() - 1; l >= 0; --l) {
+            const auto &lv = level_indices[l];
+#pragma omp parallel for schedule(dynamic)
+            for (int i = 0; i < (int)lv.size(); ++i) {
+                const int b = lv[i];
+                if (!has_proxy_from_children[b]) continue;
+                for (int ic = 0; ic < 8; ++ic) {
+                    const int cb = children[b][ic];
+                    if (cb < 0 || !(src_cnt[cb] > 0 && ifpwexp[cb])) continue;
+                    tensor3(true, &proxy_up[up_off[cb]], &c2pR[ic * 3 * p * p], &proxy_up[up_off[b]]);
+                }
+            }
+        }
+    }
+
+    void form_outgoing() {
+#pragma omp parallel for schedule(dynamic)
+        for (int b = 0; b < nboxes; ++b)
+            if (ifpwexp[b] && up_off[b] != -1)
+                proxy2pw(&proxy_up[up_off[b]], poly2pwR[depth[b]].data(), &pw_out[pw_off[b]]);
+    }
+
+    void form_eval(int l) {
+        const auto &boxes = level_indices[l];
+        const Real bsize = boxsize[l];
+        const Real sc = Real(2.0) / bsize;
+        const auto &W = wpwshiftR[l];
+#pragma omp parallel for schedule(dynamic)
+        for (int i = 0; i < (int)boxes.size(); ++i) {
+            const int box = boxes[i];
+            std::vector<std::complex<Real>> pw_in(n_modes);
+            const bool has_output = trg_cnt[box] > 0;
+            const int npts = src_cnt[box] + trg_cnt[box];
+            if (ifpwexp[box] && npts && has_output) {
+                const std::complex<Real> *po = &pw_out[pw_off[box]];
+                const std::complex<Real> *self = &W[13 * n_modes];
+                for (int m = 0; m < n_modes; ++m) pw_in[m] = po[m] * self[m].real();
+                for (int ind = 0; ind < 27; ++ind) {
+                    const int nb = coll[box][ind];
+                    if (nb < 0 || nb == box) continue;
+                    if (is_leaf[box] && is_leaf[nb]) continue;
+                    if (pw_off[nb] == -1) continue;
+                    int si = 0, stride = 1;
+                    for (int d = 0; d < 3; ++d) {
+                        si += ((int)std::lround((center(box)[d] - center(nb)[d]) / bsize) + 1) * stride;
+                        stride *= 3;
+                    }
+                    const std::complex<Real> *src = &pw_out[pw_off[nb]];
+                    const std::complex<Real> *sh = &W[si * n_modes];
+                    for (int m = 0; m < n_modes; ++m) {
+                        const Real ar = src[m].real(), ai = src[m].imag(), cr = sh[m].real(), ci = sh[m].imag();
+                        pw_in[m] += std::complex<Real>(ar * cr - ai * ci, ar * ci + ai * cr);
+                    }
+                }
+                Real *pd = &proxy_down[down_off[box]];
+                if (!down_zeroed[box]) { std::fill(pd, pd + p * p * p, Real(0)); down_zeroed[box] = 1; }
+                pw2proxy(pw_in.data(), pw2polyR[l].data(), pd);
+                if (!iftensprodeval[box]) {
+                    for (int ic = 0; ic < 8; ++ic) {
+                        const int ch = children[box][ic];
+                        if (ch < 0 || !(src_cnt[ch] + trg_cnt[ch])) continue;
+                        tensor3(down_zeroed[ch], pd, &p2cR[ic * 3 * p * p], &proxy_down[down_off[ch]]);
+                        down_zeroed[ch] = 1;
+                    }
+                }
+            }
+            if (iftensprodeval[box] && trg_cnt[box]) {
+                const Real *pd = &proxy_down[down_off[box]];
+                for (int t = 0; t < trg_cnt[box]; ++t) {
+                    const size_t ti = trg_start[box] + t;
+                    pot_trg_sorted[ti] += eval_proxy_at(pd, &r_trg_sorted[3 * ti], center(box), sc);
+                }
+            }
+        }
+    }
+
+    // residual kernel sum over sources for nt targets (vector_kernels.hpp:354-421, 495-533)
+    void residual(Real rsc, Real cen, Real d2max, int ns, const Real *rs, const Real *qs, int nt, const Real *rt,
+                  Real *pot) {
+        const int nc = (int)coeffs.size();
+        const bool split = n_digits < 6;
+        Real cm[64];
+        if (split) {
+            double cd[64], rp = 1;
+            for (int i = 0; i < nc; ++i) { cd[i] = (double)coeffs[i] * rp; rp *= (double)rsc; }
+            for (int i = 0; i < nc; ++i)
+                for (int j = nc - 1; j > i; --j) cd[j - 1] += (double)cen * cd[j];
+            for (int i = 0; i < nc; ++i) cm[i] = (Real)cd[i];
+        }
+        for (int t = 0; t < nt; ++t) {
+            Real acc = 0;
+            for (int s = 0; s < ns; ++s) {
+                const Real dx = rt[3 * t] - rs[3 * s], dy = rt[3 * t + 1] - rs[3 * s + 1], dz = rt[3 * t + 2] - rs[3 * s + 2];
+                const Real r2 = std::fma(dx, dx, std::fma(dy, dy, dz * dz));
+                if (!(r2 > Real(1e-30) && r2 < d2max)) continue;
+                const Real rinv = Real(1) / std::sqrt(r2);
+                Real u;
+                if (split) {
+                    const int se = ((nc - 1) & ~1), so = ((nc - 2) & ~1) + 1;
+                    Real E = cm[se];
+                    for (int i = se - 2; i >= 0; i -= 2) E = std::fma(E, r2, cm[i]);
+                    Real O = cm[so];
+                    for (int i = so - 2; i >= 0; i -= 2) O = std::fma(O, r2, cm[i]);
+                    u = std::fma(E, rinv, O);
+                } else {
+                    const Real x = std::fma(r2, rinv, cen) * rsc;
+                    Real P = coeffs[nc - 1];
+                    for (int i = nc - 2; i >= 0; --i) P = std::fma(P, x, coeffs[i]);
+                    u = P * rinv;
+                }
+                acc = std::fma(u, qs[s], acc);
+            }
+            pot[t] += acc;
+        }
+    }
+
+    struct Contact {
+        int td[3], od[3], nt = 0, no = 0;
+        Real tc[3], lo[3], hi[3], d2max;
+        Contact(const Real *a, const Real *b, Real sa, Real sb, Real d2) : d2max(d2) {
+            for (int d = 0; d < 3; ++d) {
+                if (a[d] + sa == b[d]) { td[nt] = d; tc[nt++] = b[d]; }
+                else if (b[d] + sb == a[d]) { td[nt] = d; tc[nt++] = a[d]; }
+                else { od[no] = d; lo[no] = std::max(a[d], b[d]); hi[no++] = std::min(a[d] + sa, b[d] + sb); }
+            }
+        }
+        bool in_range(const Real *p) const {
+            Real s = 0;
+            for (int t = 0; t < nt; ++t) { Real dd = p[td[t]] - tc[t]; s += dd * dd; }
+            for (int t = 0; t < no; ++t) {
+                Real c = p[od[t]], cl = std::min(std::max(c, lo[t]), hi[t]), dd = c - cl;
+                s += dd * dd;
+            }
+            return s < d2max;
+        }
+    };
+
+    void direct() {
+#pragma omp parallel for schedule(dynamic)
+        for (int idx = 0; idx < (int)direct_work.size(); ++idx) {
+            const int tb = direct_work[idx];
+            const int tl = depth[tb];
+            std::vector<Real> rbuf, qbuf, tbuf, pbuf;
+            std::vector<int> imap;
+            for (int sb : list1[tb]) {
+                int sl = depth[sb];
+                Real bsize = boxsize[sl];
+                if (ifpwexp[sb] && sb == tb) { bsize /= Real(2); sl++; }
+                else if (sl < tl) { bsize = boxsize[tl]; sl = tl; }
+                const Real d2max = bsize * bsize;
+                const Real rsc = 2 * (Real(1) / bsize);
+                const Real cen = -bsize / Real(2);
+                const bool src_larger = depth[sb] < depth[tb], trg_larger = depth[sb] > depth[tb];
+                Real ca[3], cb[3];
+                const Real sa = boxsize[depth[sb]], sz = boxsize[depth[tb]];
+                for (int d = 0; d < 3; ++d) { ca[d] = center(sb)[d] - sa / 2; cb[d] = center(tb)[d] - sz / 2; }
+                int ns = src_cnt[sb];
+                const Real *rs = &r_src_sorted[3 * (size_t)src_start[sb]];
+                const Real *qs = &charge_sorted[src_start[sb]];
+                const bool filter_same = !src_larger && !trg_larger && trg_cnt[tb] >= 64;
+                if (src_larger || filter_same) {
+                    Contact g(ca, cb, sa, sz, d2max);
+                    rbuf.clear(); qbuf.clear();
+                    for (int s = 0; s < ns; ++s)
+                        if (g.in_range(rs + 3 * s)) {
+                            rbuf.insert(rbuf.end(), rs + 3 * s, rs + 3 * s + 3);
+                            qbuf.push_back(qs[s]);
+                        }
+                    ns = (int)qbuf.size();
+                    rs = rbuf.data();
+                    qs = qbuf.data();
+                }
+                if (!ns || !trg_cnt[tb]) continue;
+                int nt = trg_cnt[tb];
+                const Real *rt = &r_trg_sorted[3 * (size_t)trg_start[tb]];
+                Real *pt = &pot_trg_sorted[trg_start[tb]];
+                if (trg_larger) {
+                    Contact g(ca, cb, sa, sz, d2max);
+                    tbuf.clear(); imap.clear();
+                    for (int t = 0; t < nt; ++t)
+                        if (g.in_range(rt + 3 * t)) { tbuf.insert(tbuf.end(), rt + 3 * t, rt + 3 * t + 3); imap.push_back(t); }
+                    pbuf.assign(imap.size(), 0);
+                    if (!imap.empty()) {
+                        residual(rsc, cen, d2max, ns, rs, qs, (int)imap.size(), tbuf.data(), pbuf.data());
+                        for (size_t k = 0; k < imap.size(); ++k) pt[imap[k]] += pbuf[k];
+                    }
+                } else {
+                    residual(rsc, cen, d2max, ns, rs, qs, nt, rt, pt);
+                }
+            }
+        }
+    }
+
+    void self_correction(const Real *q_user) {
+        for (int b = 0; b < nboxes; ++b) {
+            if (!is_leaf[b] || trg_cnt[b] == 0) continue;
+            const Real sc = w0R[depth[b] + ifpwexp[b]];
+            for (int i = 0; i < trg_cnt[b]; ++i) {
+                const int si = trg_start[b] + i;
+                pot_trg_sorted[si] -= sc * q_user[trg_perm[si]];
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- far field
+    void far_field_setup() {
+        const double twopi = 2 * M_PI;
+        const double u0 = umat[0], u3 = umat[3], u4 = umat[4], u6 = umat[6], u7 = umat[7], u8 = umat[8];
+        const double vol = u0 * u4 * u8;
+        const double psi0 = tab.psi0;
+        const double dkmax = (double)beta / (double)rc;
+        const double an1 = u0, an2 = std::sqrt(u3 * u3 + u4 * u4), an3 = std::sqrt(u6 * u6 + u7 * u7 + u8 * u8);
+        ms = std::max(2, 2 * (int)std::ceil(dkmax * an1 / twopi));
+        mt = std::max(2, 2 * (int)std::ceil(dkmax * an2 / twopi));
+        mu = std::max(2, 2 * (int)std::ceil(dkmax * an3 / twopi));
+        const double a11 = 1 / u0, a21 = -u3 / (u0 * u4), a22 = 1 / u4;
+        const double a31 = (u3 * u7 - u6 * u4) / (u0 * u4 * u8), a32 = -u7 / (u4 * u8), a33 = 1 / u8;
+        const double dfac = 4 * M_PI / (psi0 * vol);
+        ff_weight.assign((size_t)ms * mt * mu, 0);
+        size_t n = 0;
+        for (int k = -mu / 2; k <= (mu - 1) / 2; ++k)
+            for (int j = -mt / 2; j <= (mt - 1) / 2; ++j)
+                for (int i = -ms / 2; i <= (ms - 1) / 2; ++i, ++n) {
+                    const double k1 = twopi * a11 * i, k2 = twopi * (a21 * i + a22 * j),
+                                 k3 = twopi * (a31 * i + a32 * j + a33 * k);
+                    const double kk = k1 * k1 + k2 * k2 + k3 * k3;
+                    if (kk < 1e-30) continue;
+                    const double x = std::sqrt(kk) * (double)rc / (double)beta;
+                    if (x <= 1) ff_weight[n] = dfac * (double)tab.pf.eval(x) / kk;
+                }
+    }
+
+    // pot[i] += sum_{m != 0} w_m Re( S(m) e^{i k.x_i} ), S(m) = sum_j q_j e^{-i k.x_j}, evaluated exactly.
+    void far_field(const Real *q, Real *pot) {
+        const double twopi = 2 * M_PI;
+        const double u0 = umat[0], u3 = umat[3], u4 = umat[4], u6 = umat[6], u7 = umat[7], u8 = umat[8];
+        const double a11 = 1 / u0, a21 = -u3 / (u0 * u4), a22 = 1 / u4;
+        const double a31 = (u3 * u7 - u6 * u4) / (u0 * u4 * u8), a32 = -u7 / (u4 * u8), a33 = 1 / u8;
+        // fractional phases theta = 2 pi U^{-1} x (reference coordinate packing, pbc.cpp:398-406)
+        const int N = n_src;
+        std::vector<double> th(3 * (size_t)N);
+        for (int j = 0; j < N; ++j) {
+            const double x = r_src[3 * (size_t)j], y = r_src[3 * (size_t)j + 1], z = r_src[3 * (size_t)j + 2];
+            th[3 * (size_t)j + 0] = twopi * (a11 * x + a21 * y + a31 * z);
+            th[3 * (size_t)j + 1] = twopi * (a22 * y + a32 * z);
+            th[3 * (size_t)j + 2] = twopi * (a33 * z);
+        }
+        // list of nonzero modes in the half space (k > 0) or (k == 0 && j > 0) or (k == 0 && j == 0 && i > 0)
+        struct Mode { int i, j, k; double w; };
+        std::vector<Mode> modes;
+        size_t n = 0;
+        for (int k = -mu / 2; k <= (mu - 1) / 2; ++k)
+            for (int j = -mt / 2; j <= (mt - 1) / 2; ++j)
+                for (int i = -ms / 2; i <= (ms - 1) / 2; ++i, ++n) {
+                    if (ff_weight[n] == 0) continue;
+                    const bool pos = k > 0 || (k == 0 && (j > 0 || (j == 0 && i > 0)));
+                    if (pos) modes.push_back({i, j, k, ff_weight[n]});
+                }
+        const int M = (int)modes.size();
+        std::vector<std::complex<double>> S(M, 0);
+        const int nthr =
+#ifdef _OPENMP
+            omp_get_max_threads();
+#else
+            1;
+#endif
+        std::vector<std::vector<std::complex<double>>> Sp(nthr, std::vector<std::complex<double>>(M, 0));
+#pragma omp parallel
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+            auto &loc = Sp[tid];
+#pragma omp for schedule(static)
+            for (int jpt = 0; jpt < N; ++jpt) {
+                const double *t = &th[3 * (size_t)jpt];
+                const double qj = (double)q[jpt];
+                for (int m = 0; m < M; ++m) {
+                    const double ph = modes[m].i * t[0] + modes[m].j * t[1] + modes[m].k * t[2];
+                    loc[m] += qj * std::complex<double>(std::cos(ph), -std::sin(ph));
+                }
+            }
+        }
+        for (int t = 0; t < nthr; ++t)
+            for (int m = 0; m < M; ++m) S[m] += Sp[t][m];
+#pragma omp parallel for schedule(static)
+        for (int ipt = 0; ipt < N; ++ipt) {
+            const double *t = &th[3 * (size_t)ipt];
+            double acc = 0;
+            for (int m = 0; m < M; ++m) {
+                const double ph = modes[m].i * t[0] + modes[m].j * t[1] + modes[m].k * t[2];
+                acc += 2 * modes[m].w * (S[m].real() * std::cos(ph) - S[m].imag() * std::sin(ph));
+            }
+            pot[ipt] += (Real)acc;
+        }
+    }
+
+    // ---------------------------------------------------------------- evaluate
+    // Stage flags: bit0 upward+PW (dmk), bit1 direct, bit2 self, bit3 far field.
+    std::vector<Real> evaluate(const double *charges, int stages = 15) {
+        std::vector<Real> q(n_src);
+        for (int i = 0; i < n_src; ++i) q[i] = (Real)charges[i];
+        std::vector<Real> chgall(nsall);
+        std::copy(q.begin(), q.end(), chgall.begin());
+        for (size_t j = 0; j < img_src_idx.size(); ++j) chgall[n_src + j] = q[img_src_idx[j]];
+        update_charges(chgall.data());
+        if (stages & 1) {
+            upward_pass();
+            std::fill(down_zeroed.begin(), down_zeroed.end(), 0);
+            form_outgoing();
+            for (int l = 0; l < n_levels(); ++l) form_eval(l);
+        }
+        if (stages & 2) direct();
+        if (stages & 4) self_correction(q.data());
+        std::vector<Real> pot(n_src, 0);
+        for (int i = 0; i < n_src; ++i) pot[trg_perm[i]] = pot_trg_sorted[i];
+        if (stages & 8) far_field(q.data(), pot.data());
+        return pot;
+    }
+};
+
+}  // namespace oracle
