@@ -1,0 +1,59 @@
+/// Device-resident entry points of libpdmk.so.
+///
+/// These extend the drop-in ABI (pdmk_capi.h) for callers that keep charges and
+/// potentials in HBM.  They take handles created by pdmk_create_{double,float} and
+/// run the same evaluate() path as pdmk_evaluate_* without host copies.  The
+/// reference has no device memory; each function names the reference call it
+/// corresponds to.  All return 0 on success and -1 on error (message on stderr).
+#ifndef PDMK_GPU_H
+#define PDMK_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/// Number of points owned by a handle (PeriodicDMKTreeT::n_points,
+/// cpp/include/pdmk/periodic_dmk.hpp:52).  Returns -1 for a NULL handle.
+int pdmk_gpu_n_points(void* h);
+
+/// evaluate() with device pointers: d_charges[N] (FP64, user order) in,
+/// d_pot[N] (double for *_double handles, float for *_float) out, enqueued on
+/// `stream` (a hipStream_t; NULL = the handle's own stream).  Returns after
+/// enqueueing; the caller synchronizes the stream.  Corresponds to
+/// PeriodicDMKTreeT::evaluate, cpp/src/periodic_dmk.cpp:141-186.
+int pdmk_gpu_evaluate_double(void* h, const double* d_charges, double* d_pot, void* stream);
+int pdmk_gpu_evaluate_float(void* h, const double* d_charges, float* d_pot, void* stream);
+
+/// evaluate_timed() on device pointers: runs synchronously and fills
+/// times_ms[0..2] = {nufft_ms, dmk_ms, direct_ms} with the same stage grouping as
+/// EvalTiming (cpp/include/pdmk/eval_timing.hpp:10-19,
+/// cpp/src/periodic_dmk.cpp:188-249), measured with HIP events.
+int pdmk_gpu_evaluate_timed(void* h, const double* d_charges, void* d_pot, double* times_ms);
+
+/// Run only selected stages (bit 0: proxy/plane-wave part, bit 1: near field,
+/// bit 2: self correction, bit 3: far field) on device pointers, synchronously.
+/// Stage-level parity checks use it; stages=15 equals pdmk_gpu_evaluate_*.
+int pdmk_gpu_evaluate_stages(void* h, const double* d_charges, void* d_pot, int stages);
+
+/// Tree and work statistics: out[0..15] = {n_src, nsall, nboxes, n_levels, p, n_pw,
+/// ms, mt, mu, n_leaves, n_list1, n_direct_leaves, n_far_modes, n_digits,
+/// is_float, n_pair_tiles}.
+int pdmk_gpu_stats(void* h, long long* out16);
+
+/// Algorithmic work per evaluate(): out[0..7] = {near-field pairs in support,
+/// near-field candidate pairs, near-field flops (in-support pairs x (2*NC+12)),
+/// proxy/PW transform flops, far-field mode-point products, far-field flops,
+/// bytes of O(N) traffic, near-field kernel HBM bytes}.  Near-field pair counts are
+/// computed on the host at build time.
+int pdmk_gpu_work(void* h, double* out8);
+
+/// Per-box topology (for comparison with a CPU restatement): box7[7*nboxes] =
+/// {depth, parent, nchild, src_count, trg_count, ifpwexp, n_list1}, centers[3*nboxes],
+/// src_perm[nsall] (sorted -> combined index), trg_perm[n_src] (sorted -> user index).
+int pdmk_gpu_tree_arrays(void* h, int* box7, double* centers, int* src_perm, int* trg_perm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif // PDMK_GPU_H

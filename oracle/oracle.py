@@ -110,11 +110,17 @@ class OracleTree:
         lib().oracle_tree_arrays(self._h, _i(box7), _i(sp), _i(tp), _d(cen))
         return box7.reshape(nb, 7), sp, tp, cen.reshape(nb, 3)
 
-    def __del__(self):
+    def close(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().oracle_destroy(h)
-            self._h = None
+        if h and _lib is not None:
+            _lib.oracle_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def pswf(c, x):

@@ -1,0 +1,221 @@
+"""MI355X periodic DMK solver: Python mirror of the reference bindings.
+
+The product is ``periodicdmk_amd/lib/libpdmk.so`` (host C++ + gfx950 HIP kernels), whose
+C ABI (``include/pdmk/pdmk_capi.h``) replaces the reference ``libpdmk_jl``.  This module
+binds it with ctypes the way ``julia/PeriodicDMK/src/PeriodicDMK.jl`` binds the reference:
+
+* ``PDMKSolver(cell, positions; n_digits, n_per_leaf, nz, precision)``
+  (PeriodicDMK.jl:95-143) builds the tree once;
+* ``evaluate(solver, charges)`` / ``solver.evaluate(charges)`` (PeriodicDMK.jl:188-211)
+  evaluates potentials for new charges;
+* ``solver.evaluate_device(d_charges, d_pot, stream)`` is the device-resident variant.
+
+``cell`` has the lattice vectors as columns; ``positions`` is (N, 3).  There is no
+fallback: if the library is not built, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpdmk.so")
+_lib = None
+
+STAGE_PW, STAGE_DIRECT, STAGE_SELF, STAGE_FAR = 1, 2, 4, 8
+STATS_KEYS = ("n_src", "nsall", "nboxes", "n_levels", "p", "n_pw", "ms", "mt", "mu", "n_leaves", "n_list1",
+              "n_direct_leaves", "n_far_modes", "n_digits", "is_float", "n_pair_tiles")
+WORK_KEYS = ("nf_pairs_support", "nf_pairs_candidate", "nf_flops", "transform_flops", "ff_mode_points", "ff_flops",
+             "on_bytes", "nf_bytes")
+
+
+def lib():
+    """Load libpdmk.so from the package directory (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m periodicdmk_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    lp = C.POINTER(C.c_longlong)
+    fp = C.POINTER(C.c_float)
+    sig = {
+        "pdmk_ewald_nufft": (C.c_int, [dp, C.c_int, dp, dp, C.c_double, dp, dp]),
+        "pdmk_optimize_cubic_cover": (C.c_int, [dp, C.c_int, ip, ip, ip, dp, dp, dp, dp, dp]),
+        "pdmk_create_double": (vp, [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int]),
+        "pdmk_evaluate_double": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "pdmk_destroy_double": (None, [vp]),
+        "pdmk_create_float": (vp, [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int]),
+        "pdmk_evaluate_float": (C.c_int, [vp, C.c_int, dp, dp, fp]),
+        "pdmk_destroy_float": (None, [vp]),
+        "pdmk_gpu_n_points": (C.c_int, [vp]),
+        "pdmk_gpu_evaluate_double": (C.c_int, [vp, vp, vp, vp]),
+        "pdmk_gpu_evaluate_float": (C.c_int, [vp, vp, vp, vp]),
+        "pdmk_gpu_evaluate_timed": (C.c_int, [vp, vp, vp, dp]),
+        "pdmk_gpu_evaluate_stages": (C.c_int, [vp, vp, vp, C.c_int]),
+        "pdmk_gpu_stats": (C.c_int, [vp, lp]),
+        "pdmk_gpu_work": (C.c_int, [vp, dp]),
+        "pdmk_gpu_tree_arrays": (C.c_int, [vp, ip, dp, ip, ip]),
+        "pdmk_host_tree": (C.c_int, [dp, C.c_int, dp, C.c_int, C.c_int, lp, ip, dp, ip, ip, dp]),
+        "pdmk_host_tables": (C.c_int, [C.c_int, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "pdmk_host_pswf": (C.c_double, [C.c_double, C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else C.POINTER(C.c_double)()
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int)) if a is not None else C.POINTER(C.c_int)()
+
+
+def _cell_colmajor(cell):
+    cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+    return np.ascontiguousarray(cell.T)  # memory: cell[i + 3 j] = component i of lattice vector j
+
+
+def _positions(positions):
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64))
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError("positions must have shape (N, 3)")
+    return pos
+
+
+class PDMKSolver:
+    """Reusable periodic DMK tree on the GPU (mirror of the Julia ``PDMKSolver``)."""
+
+    def __init__(self, cell, positions, n_digits=6, n_per_leaf=300, nz=4, precision="auto"):
+        if precision == "auto":
+            precision = "float" if n_digits <= 6 else "double"
+        if precision not in ("float", "double"):
+            raise ValueError("precision must be 'float', 'double' or 'auto'")
+        self.precision = precision
+        self.dtype = np.float32 if precision == "float" else np.float64
+        cellc = _cell_colmajor(cell)
+        pos = _positions(positions)
+        create = lib().pdmk_create_float if precision == "float" else lib().pdmk_create_double
+        self._h = create(_d(cellc), pos.shape[0], _d(pos), int(n_digits), int(n_per_leaf), int(nz))
+        if not self._h:
+            raise RuntimeError("pdmk_create failed (see stderr)")
+        self.n = pos.shape[0]
+        self.n_digits = n_digits
+
+    def evaluate(self, charges, positions=None):
+        """Potentials (user order) for ``charges``; ``positions`` is ignored as in the reference."""
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        if q.shape != (self.n,):
+            raise ValueError("charges size != n_src")
+        out = np.empty(self.n, dtype=self.dtype)
+        if self.precision == "float":
+            rc = lib().pdmk_evaluate_float(self._h, self.n, C.POINTER(C.c_double)(), _d(q),
+                                           out.ctypes.data_as(C.POINTER(C.c_float)))
+        else:
+            rc = lib().pdmk_evaluate_double(self._h, self.n, C.POINTER(C.c_double)(), _d(q), _d(out))
+        if rc != 0:
+            raise RuntimeError("pdmk_evaluate failed (see stderr)")
+        return out
+
+    def evaluate_device(self, d_charges: int, d_pot: int, stream: int = 0):
+        """Enqueue evaluate() on device pointers (FP64 charges in, Real potentials out)."""
+        f = lib().pdmk_gpu_evaluate_float if self.precision == "float" else lib().pdmk_gpu_evaluate_double
+        if f(self._h, C.c_void_p(d_charges), C.c_void_p(d_pot), C.c_void_p(stream)) != 0:
+            raise RuntimeError("pdmk_gpu_evaluate failed (see stderr)")
+
+    def evaluate_timed(self, d_charges: int, d_pot: int):
+        t = np.zeros(3)
+        if lib().pdmk_gpu_evaluate_timed(self._h, C.c_void_p(d_charges), C.c_void_p(d_pot), _d(t)) != 0:
+            raise RuntimeError("pdmk_gpu_evaluate_timed failed (see stderr)")
+        return {"nufft_ms": t[0], "dmk_ms": t[1], "direct_ms": t[2]}
+
+    def evaluate_stages(self, d_charges: int, d_pot: int, stages: int):
+        if lib().pdmk_gpu_evaluate_stages(self._h, C.c_void_p(d_charges), C.c_void_p(d_pot), int(stages)) != 0:
+            raise RuntimeError("pdmk_gpu_evaluate_stages failed (see stderr)")
+
+    def stats(self):
+        o = np.zeros(16, dtype=np.int64)
+        lib().pdmk_gpu_stats(self._h, o.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return dict(zip(STATS_KEYS, (int(v) for v in o)))
+
+    def work(self):
+        o = np.zeros(8)
+        lib().pdmk_gpu_work(self._h, _d(o))
+        return dict(zip(WORK_KEYS, (float(v) for v in o)))
+
+    def tree_arrays(self):
+        s = self.stats()
+        nb = s["nboxes"]
+        box7 = np.zeros(7 * nb, dtype=np.int32)
+        cen = np.zeros(3 * nb)
+        sp = np.zeros(s["nsall"], dtype=np.int32)
+        tp = np.zeros(s["n_src"], dtype=np.int32)
+        lib().pdmk_gpu_tree_arrays(self._h, _i(box7), _d(cen), _i(sp), _i(tp))
+        return box7.reshape(nb, 7), sp, tp, cen.reshape(nb, 3)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            (_lib.pdmk_destroy_float if self.precision == "float" else _lib.pdmk_destroy_double)(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def evaluate(solver: PDMKSolver, *args):
+    """``evaluate(solver, charges)`` or ``evaluate(solver, positions, charges)``."""
+    return solver.evaluate(args[-1])
+
+
+def host_tree(cell, positions, n_per_leaf=300, nz=4):
+    """Host-side tree of the library (no GPU): box7, src_perm, trg_perm, centers, geometry."""
+    cellc = _cell_colmajor(cell)
+    pos = _positions(positions)
+    sizes = np.zeros(8, dtype=np.int64)
+    L = lib()
+    if L.pdmk_host_tree(_d(cellc), pos.shape[0], _d(pos), int(n_per_leaf), int(nz),
+                        sizes.ctypes.data_as(C.POINTER(C.c_longlong)), None, None, None, None, None) != 0:
+        raise RuntimeError("pdmk_host_tree failed")
+    nb, nsall, n = int(sizes[2]), int(sizes[1]), int(sizes[0])
+    box7 = np.zeros(7 * nb, dtype=np.int32)
+    cen = np.zeros(3 * nb)
+    sp = np.zeros(nsall, dtype=np.int32)
+    tp = np.zeros(n, dtype=np.int32)
+    geom = np.zeros(16)
+    L.pdmk_host_tree(_d(cellc), pos.shape[0], _d(pos), int(n_per_leaf), int(nz),
+                     sizes.ctypes.data_as(C.POINTER(C.c_longlong)), _i(box7), _d(cen), _i(sp), _i(tp), _d(geom))
+    return {"sizes": sizes, "box7": box7.reshape(nb, 7), "centers": cen.reshape(nb, 3), "src_perm": sp,
+            "trg_perm": tp, "umat": geom[:9].reshape(3, 3).T, "rc": geom[9], "mgrid": geom[10:13].astype(int),
+            "xgrid0": geom[13:16]}
+
+
+def host_tables(n_digits, rc, n_levels, umat_colmajor):
+    umat = np.ascontiguousarray(np.asarray(umat_colmajor, dtype=np.float64).reshape(9))
+    s8 = np.zeros(8)
+    L = lib()
+    if L.pdmk_host_tables(int(n_digits), float(rc), int(n_levels), _d(umat), _d(s8), None, None, None, None, None) != 0:
+        raise RuntimeError("pdmk_host_tables failed")
+    n_pw, p, nm, nf = int(s8[4]), int(s8[5]), int(s8[6]), int(s8[7])
+    w0, rad, p2w, p2c, far = np.zeros(n_levels + 1), np.zeros(nm), np.zeros(2 * n_pw * p), np.zeros(2 * p * p), \
+        np.zeros(4 * nf)
+    L.pdmk_host_tables(int(n_digits), float(rc), int(n_levels), _d(umat), _d(s8), _d(w0), _d(rad), _d(p2w), _d(p2c),
+                       _d(far))
+    return {"beta": s8[0], "psi0": s8[1], "c0": s8[2], "g0d2": s8[3], "n_pw": n_pw, "p": p, "n_modes": nm,
+            "w0": w0, "radial0": rad, "poly2pw": (p2w[0::2] + 1j * p2w[1::2]).reshape(p, n_pw).T,
+            "p2c": p2c.reshape(2, p, p).transpose(0, 2, 1), "far": far.reshape(nf, 4)}
+
+
+def pswf(c, x):
+    return lib().pdmk_host_pswf(float(c), float(x))

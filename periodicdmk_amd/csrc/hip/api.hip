@@ -1,0 +1,285 @@
+// C++ API (PeriodicDMKTreeT, AutoPeriodicDMKTree) and the C ABI of libpdmk.so.
+//
+// Reference interfaces replaced:
+//   PeriodicDMKTreeT ctor / evaluate / evaluate_timed ... cpp/src/periodic_dmk.cpp:141-308
+//   AutoPeriodicDMKTree .................................. cpp/src/periodic_dmk.cpp:319-357
+//   pdmk_* C ABI ......................................... cpp/src/pdmk_capi.cpp:20-166
+#include <pdmk/pdmk_capi.h>
+#include <pdmk/pdmk_gpu.h>
+#include <pdmk/periodic_dmk.hpp>
+
+#include "plan.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+
+namespace pdmk {
+
+template <typename Real, int DIM>
+PeriodicDMKTreeT<Real, DIM>::PeriodicDMKTreeT(const MatND<DIM> &cell, const std::vector<VecND<DIM>> &positions,
+                                              int n_digits, int n_per_leaf, int nz,
+                                              std::optional<std::array<double, DIM>> bloch) {
+    if (bloch.has_value())
+        throw std::logic_error("PeriodicDMKTreeT: quasi-periodic (bloch) evaluation is not provided by this build");
+    double c[9];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) c[i + 3 * j] = cell[j][i];
+    std::vector<double> pos(3 * positions.size());
+    for (size_t i = 0; i < positions.size(); ++i)
+        for (int d = 0; d < 3; ++d) pos[3 * i + d] = positions[i][d];
+    p_ = std::make_unique<gpu::Plan<Real>>(c, pos.data(), (int)positions.size(), n_digits, n_per_leaf, nz);
+    n_src_ = (int)positions.size();
+    rc_ = p_->rc;
+}
+
+template <typename Real, int DIM>
+PeriodicDMKTreeT<Real, DIM>::~PeriodicDMKTreeT() = default;
+
+template <typename Real, int DIM>
+std::vector<Real> PeriodicDMKTreeT<Real, DIM>::evaluate(const std::vector<double> &charges) {
+    if ((int)charges.size() != n_src_) throw std::invalid_argument("charges size != n_src");
+    std::vector<Real> pot(n_src_);
+    p_->evaluate_host(charges.data(), pot.data(), nullptr);
+    return pot;
+}
+
+template <typename Real, int DIM>
+std::vector<Real> PeriodicDMKTreeT<Real, DIM>::evaluate_timed(const std::vector<double> &charges, EvalTiming &timing) {
+    if ((int)charges.size() != n_src_) throw std::invalid_argument("charges size != n_src");
+    std::vector<Real> pot(n_src_);
+    double t[3] = {0, 0, 0};
+    p_->evaluate_host(charges.data(), pot.data(), t);
+    timing.nufft_ms = t[0];
+    timing.dmk_ms = t[1];
+    timing.direct_ms = t[2];
+    timing.build_ms = 0;
+    return pot;
+}
+
+template <typename Real, int DIM>
+std::vector<std::complex<Real>> PeriodicDMKTreeT<Real, DIM>::evaluate_complex(const std::vector<double> &) {
+    throw std::logic_error("evaluate_complex: quasi-periodic evaluation is not provided by this build");
+}
+
+template <typename Real, int DIM>
+void PeriodicDMKTreeT<Real, DIM>::evaluate_device(const double *d_charges, Real *d_pot, void *stream) {
+    p_->evaluate(d_charges, d_pot, static_cast<hipStream_t>(stream));
+}
+
+template class PeriodicDMKTreeT<double, 3>;
+template class PeriodicDMKTreeT<float, 3>;
+
+AutoPeriodicDMKTree::AutoPeriodicDMKTree(const Mat3 &cell, const std::vector<Vec3> &positions, int n_digits,
+                                         int n_per_leaf, int nz) {
+    if (n_digits <= 6) {
+        single_precision_ = true;
+        impl_ = std::make_unique<PeriodicDMKTreeT<float, 3>>(cell, positions, n_digits, n_per_leaf, nz);
+    } else {
+        single_precision_ = false;
+        impl_ = std::make_unique<PeriodicDMKTreeT<double, 3>>(cell, positions, n_digits, n_per_leaf, nz);
+    }
+}
+
+std::variant<std::vector<float>, std::vector<double>> AutoPeriodicDMKTree::evaluate(const std::vector<double> &q) {
+    if (single_precision_) return std::get<std::unique_ptr<PeriodicDMKTreeT<float, 3>>>(impl_)->evaluate(q);
+    return std::get<std::unique_ptr<PeriodicDMKTreeT<double, 3>>>(impl_)->evaluate(q);
+}
+
+AutoPeriodicDMKTree make_periodic_dmk_tree(const Mat3 &cell, const std::vector<Vec3> &positions, int n_digits,
+                                           int n_per_leaf, int nz) {
+    return AutoPeriodicDMKTree(cell, positions, n_digits, n_per_leaf, nz);
+}
+
+}  // namespace pdmk
+
+// ====================================================================== C ABI
+namespace {
+
+// Handles carry a precision tag so that device-pointer entry points can check them.
+struct Handle {
+    int is_float;
+    void *tree;
+};
+
+template <typename Real>
+pdmk::PeriodicDMKTreeT<Real, 3> *tree_of(void *h) {
+    auto *H = static_cast<Handle *>(h);
+    return static_cast<pdmk::PeriodicDMKTreeT<Real, 3> *>(H->tree);
+}
+
+pdmk::Mat3 to_mat3(const double *c) {
+    pdmk::Mat3 h;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) h[j][i] = c[i + 3 * j];
+    return h;
+}
+
+std::vector<pdmk::Vec3> to_positions(const double *p, int N) {
+    if (N < 0) throw std::invalid_argument("negative N");
+    std::vector<pdmk::Vec3> v(N);
+    for (int i = 0; i < N; ++i) v[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    return v;
+}
+
+template <typename Real>
+void *create_impl(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz) {
+    try {
+        if (!cell || !pos) throw std::invalid_argument("null cell or positions");
+        auto *t = new pdmk::PeriodicDMKTreeT<Real, 3>(to_mat3(cell), to_positions(pos, N), n_digits, n_per_leaf, nz);
+        return new Handle{sizeof(Real) == 4, t};
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_create: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+template <typename Real, typename Out>
+int evaluate_impl(void *h, const double *charges, Out *pot_out) {
+    if (!h) return -1;
+    try {
+        if (static_cast<Handle *>(h)->is_float != (sizeof(Real) == 4)) throw std::invalid_argument("handle precision mismatch");
+        auto *t = tree_of<Real>(h);
+        const int N = t->n_points();
+        std::vector<double> q(charges, charges + N);
+        auto pot = t->evaluate(q);
+        for (int i = 0; i < N; ++i) pot_out[i] = static_cast<Out>(pot[i]);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_evaluate: %s\n", e.what());
+        return -1;
+    }
+}
+
+template <typename Real>
+void destroy_impl(void *h) {
+    if (!h) return;
+    auto *H = static_cast<Handle *>(h);
+    delete static_cast<pdmk::PeriodicDMKTreeT<Real, 3> *>(H->tree);
+    delete H;
+}
+
+template <typename F>
+int guarded(const char *what, F &&f) {
+    try {
+        return f();
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "%s: %s\n", what, e.what());
+        return -1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdmk_ewald_nufft(const double *, int, const double *, const double *, double, double *, double *) {
+    std::fprintf(stderr, "pdmk_ewald_nufft: the FP64 Ewald reference is not provided by this build\n");
+    return -1;
+}
+
+int pdmk_optimize_cubic_cover(const double *, int nz, int *, int *, int *, double *, double *, double *, double *,
+                              double *) {
+    if (nz <= 0) return 1;
+    std::fprintf(stderr, "pdmk_optimize_cubic_cover: not provided by this build\n");
+    return -1;
+}
+
+void *pdmk_create_double(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz) {
+    return create_impl<double>(cell, N, pos, n_digits, n_per_leaf, nz);
+}
+int pdmk_evaluate_double(void *h, int, const double *, const double *charges, double *pot_out) {
+    return evaluate_impl<double, double>(h, charges, pot_out);
+}
+void pdmk_destroy_double(void *h) { destroy_impl<double>(h); }
+
+void *pdmk_create_float(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz) {
+    return create_impl<float>(cell, N, pos, n_digits, n_per_leaf, nz);
+}
+int pdmk_evaluate_float(void *h, int, const double *, const double *charges, float *pot_out) {
+    return evaluate_impl<float, float>(h, charges, pot_out);
+}
+void pdmk_destroy_float(void *h) { destroy_impl<float>(h); }
+
+// ---------------------------------------------------------------- device-resident entry points
+int pdmk_gpu_n_points(void *h) {
+    if (!h) return -1;
+    auto *H = static_cast<Handle *>(h);
+    return H->is_float ? tree_of<float>(h)->n_points() : tree_of<double>(h)->n_points();
+}
+
+int pdmk_gpu_evaluate_double(void *h, const double *d_charges, double *d_pot, void *stream) {
+    if (!h || static_cast<Handle *>(h)->is_float) return -1;
+    return guarded("pdmk_gpu_evaluate_double", [&] {
+        tree_of<double>(h)->evaluate_device(d_charges, d_pot, stream);
+        return 0;
+    });
+}
+
+int pdmk_gpu_evaluate_float(void *h, const double *d_charges, float *d_pot, void *stream) {
+    if (!h || !static_cast<Handle *>(h)->is_float) return -1;
+    return guarded("pdmk_gpu_evaluate_float", [&] {
+        tree_of<float>(h)->evaluate_device(d_charges, d_pot, stream);
+        return 0;
+    });
+}
+
+int pdmk_gpu_evaluate_timed(void *h, const double *d_charges, void *d_pot, double *times_ms) {
+    if (!h || !times_ms) return -1;
+    return guarded("pdmk_gpu_evaluate_timed", [&] {
+        if (static_cast<Handle *>(h)->is_float)
+            tree_of<float>(h)->plan().evaluate(d_charges, static_cast<float *>(d_pot), nullptr, 15, times_ms);
+        else
+            tree_of<double>(h)->plan().evaluate(d_charges, static_cast<double *>(d_pot), nullptr, 15, times_ms);
+        return 0;
+    });
+}
+
+int pdmk_gpu_evaluate_stages(void *h, const double *d_charges, void *d_pot, int stages) {
+    if (!h) return -1;
+    return guarded("pdmk_gpu_evaluate_stages", [&] {
+        double t[3];
+        if (static_cast<Handle *>(h)->is_float)
+            tree_of<float>(h)->plan().evaluate(d_charges, static_cast<float *>(d_pot), nullptr, stages, t);
+        else
+            tree_of<double>(h)->plan().evaluate(d_charges, static_cast<double *>(d_pot), nullptr, stages, t);
+        return 0;
+    });
+}
+
+int pdmk_gpu_stats(void *h, long long *o) {
+    if (!h || !o) return -1;
+    const pdmk::gpu::Stats &s =
+        static_cast<Handle *>(h)->is_float ? tree_of<float>(h)->plan().stats : tree_of<double>(h)->plan().stats;
+    const long long v[16] = {s.n_src,    s.nsall,     s.nboxes,          s.n_levels,    s.p,        s.n_pw,
+                             s.ms,       s.mt,        s.mu,              s.n_leaves,    s.n_list1,  s.n_direct_leaves,
+                             s.n_far_modes, s.n_digits, s.is_float,      s.n_pair_tiles};
+    std::memcpy(o, v, sizeof(v));
+    return 0;
+}
+
+int pdmk_gpu_work(void *h, double *o) {
+    if (!h || !o) return -1;
+    const pdmk::gpu::Work &w =
+        static_cast<Handle *>(h)->is_float ? tree_of<float>(h)->plan().work : tree_of<double>(h)->plan().work;
+    const double v[8] = {w.nf_pairs_support, w.nf_pairs_candidate, w.nf_flops, w.transform_flops,
+                         w.ff_mode_points,   w.ff_flops,           w.on_bytes, w.nf_bytes};
+    std::memcpy(o, v, sizeof(v));
+    return 0;
+}
+
+int pdmk_gpu_tree_arrays(void *h, int *box7, double *centers, int *src_perm, int *trg_perm) {
+    if (!h) return -1;
+    auto copy = [&](auto &pl) {
+        if (box7) std::memcpy(box7, pl.box7.data(), sizeof(int) * pl.box7.size());
+        if (centers) std::memcpy(centers, pl.box_centers.data(), sizeof(double) * pl.box_centers.size());
+        if (src_perm) std::memcpy(src_perm, pl.src_perm_host.data(), sizeof(int) * pl.src_perm_host.size());
+        if (trg_perm) std::memcpy(trg_perm, pl.trg_perm_host.data(), sizeof(int) * pl.trg_perm_host.size());
+    };
+    if (static_cast<Handle *>(h)->is_float) copy(tree_of<float>(h)->plan());
+    else copy(tree_of<double>(h)->plan());
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,572 @@
+// Device plan construction and the evaluate() launch sequence.
+#include "plan.hpp"
+
+#include "../host/tables.hpp"
+#include "../host/tree_build.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace pdmk::gpu {
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+DevBuf::~DevBuf() {
+    if (p) (void)hipFree(p);
+}
+
+void DevBuf::alloc(size_t nbytes) {
+    if (p) { check(hipFree(p), "hipFree"); p = nullptr; }
+    bytes = nbytes;
+    if (nbytes) check(hipMalloc(&p, nbytes), "hipMalloc");
+}
+
+namespace {
+
+constexpr int kMaxScratchBlocks = 2048;
+
+int nc_for_digits(int d) {
+    static const int nc[] = {5, 5, 5, 6, 9, 11, 12, 15, 17, 18, 22, 23, 25};
+    return nc[std::min(12, std::max(0, d))];
+}
+
+struct CounterKernel {};
+
+// Exact count of near-field pairs inside the support (1e-30 < r^2 < bs^2), once per tree.
+template <class R>
+__global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg, const int *__restrict__ leaf_trg,
+                             const int *__restrict__ l1_ptr, const int *__restrict__ l1_src, const R *__restrict__ l1_bs,
+                             unsigned long long *__restrict__ counter) {
+    const int lf = blockIdx.x;
+    const int tbeg = leaf_trg[2 * lf], tcnt = leaf_trg[2 * lf + 1];
+    unsigned long long c = 0;
+    for (int t = threadIdx.x; t < tcnt; t += blockDim.x) {
+        const P4<R> me = trg[tbeg + t];
+        for (int e = l1_ptr[lf]; e < l1_ptr[lf + 1]; ++e) {
+            const R bs = l1_bs[e], d2 = bs * bs;
+            for (int s = 0; s < l1_src[2 * e + 1]; ++s) {
+                const P4<R> q = src[l1_src[2 * e] + s];
+                const R dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+                const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                c += (r2 > R(1e-30) && r2 < d2) ? 1 : 0;
+            }
+        }
+    }
+    atomicAdd(counter, c);
+}
+
+}  // namespace
+
+template <typename Real>
+Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user, int n_per_leaf, int nz) {
+    if (n_digits_user < 1 || n_digits_user > 12) throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
+    const host::Geometry g = host::make_geometry(cell, pos, N, nz);
+    const host::Tree t = host::build_tree(g, n_per_leaf);
+    const host::DmkTables T = host::make_tables(n_digits_user, g.rc, t.nlevels, g.umat);
+    n = N;
+    rc = g.rc;
+    nsall = g.nsall();
+    nboxes = t.nboxes;
+    nlevels = t.nlevels;
+    p = T.p;
+    n_pw = T.n_pw;
+    n_digits = T.n_digits;
+    nc = (int)T.residual.size();
+    if (nc != nc_for_digits(n_digits)) throw std::logic_error("PeriodicDMK: residual table mismatch");
+
+    check(hipStreamCreate(&stream), "hipStreamCreate");
+    for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
+
+    // ---------------- sources and targets (sorted order)
+    std::vector<P4<Real>> srcv(nsall);
+    std::vector<int> su(nsall);
+    for (int i = 0; i < nsall; ++i) {
+        srcv[i] = {(Real)t.src_sorted[3 * (size_t)i], (Real)t.src_sorted[3 * (size_t)i + 1],
+                   (Real)t.src_sorted[3 * (size_t)i + 2], Real(0)};
+        const int o = t.src_order[i];
+        su[i] = o < N ? o : g.img_src[o - N];
+    }
+    std::vector<P4<Real>> trgv(n);
+    std::vector<double> tw0(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        trgv[i] = {(Real)t.trg_sorted[3 * (size_t)i], (Real)t.trg_sorted[3 * (size_t)i + 1],
+                   (Real)t.trg_sorted[3 * (size_t)i + 2], Real(0)};
+    for (int b = 0; b < nboxes; ++b)
+        if (t.leaf[b] && t.trg_count[b] > 0) {
+            const double w = T.w0[t.level[b] + t.pwexp[b]];
+            for (int i = 0; i < t.trg_count[b]; ++i) tw0[t.trg_begin[b] + i] = w;
+        }
+    src.upload(srcv);
+    src_user.upload(su);
+    trg.upload(trgv);
+    trg_user.upload(t.trg_order);
+    trg_w0.upload(tw0);
+    pot_sorted.alloc(sizeof(Real) * (size_t)n);
+    q_dev.alloc(sizeof(double) * (size_t)n);
+    pot_dev.alloc(sizeof(Real) * (size_t)n);
+    far.alloc(sizeof(double) * (size_t)n);
+
+    // ---------------- proxy slots
+    std::vector<int> up_of(nboxes, -1), down_of(nboxes, -1);
+    for (int b = 0; b < nboxes; ++b) {
+        if (t.pwexp[b] && t.src_count[b] > 0) up_of[b] = n_up++;
+        if ((t.pwexp[b] || t.tpeval[b]) && t.src_count[b] + t.trg_count[b] > 0) down_of[b] = n_down++;
+    }
+    const size_t P3 = (size_t)p * p * p;
+    const int H = (n_pw + 1) / 2, NM = n_pw * n_pw * H;
+    proxy_up.alloc(sizeof(Real) * P3 * std::max(1, n_up));
+    proxy_down.alloc(sizeof(Real) * P3 * std::max(1, n_down));
+    pw_out.alloc(2 * sizeof(Real) * (size_t)NM * std::max(1, n_up));
+
+    // ---------------- upward: one charge->proxy group per proxy slot (possibly without sources)
+    {
+        std::unordered_map<int, const host::Tree::UpGroup *> grp_of;
+        for (const auto &gr : t.up_groups) grp_of[gr.box] = &gr;
+        std::vector<int> slot(n_up), rng(n_up + 1, 0), rb, rcnt;
+        std::vector<Real> geom(4 * (size_t)n_up);
+        for (int b = 0; b < nboxes; ++b) {
+            const int k = up_of[b];
+            if (k < 0) continue;
+            slot[k] = k;
+            for (int d = 0; d < 3; ++d) geom[4 * k + d] = (Real)t.center[3 * b + d];
+            geom[4 * k + 3] = (Real)(2.0 / t.boxsize[t.level[b]]);
+            rng[k + 1] = 0;
+            auto it = grp_of.find(b);
+            if (it != grp_of.end())
+                for (int sb : it->second->src_boxes) {
+                    rb.push_back(t.src_begin[sb]);
+                    rcnt.push_back(t.src_count[sb]);
+                    rng[k + 1]++;
+                }
+        }
+        // ranges were appended in box order == slot order
+        for (int k = 0; k < n_up; ++k) rng[k + 1] += rng[k];
+        up_slot.upload(slot);
+        up_geom.upload(geom);
+        up_rng.upload(rng);
+        rng_begin.upload(rb);
+        rng_count.upload(rcnt);
+        double c2p_src = 0;
+        for (size_t i = 0; i < rcnt.size(); ++i) c2p_src += rcnt[i];
+        work.transform_flops += 2.0 * (double)P3 * c2p_src;
+    }
+    auto make_t3 = [&](std::vector<int> &out, std::vector<int> &add, std::vector<int> &ptr, std::vector<int> &in,
+                       std::vector<int> &code) {
+        Tensor3Launch L;
+        L.n_items = (int)out.size();
+        L.out.upload(out);
+        L.add.upload(add);
+        L.ptr.upload(ptr);
+        L.in_slot.upload(in);
+        L.in_code.upload(code);
+        work.transform_flops += 6.0 * std::pow((double)p, 4) * (double)in.size();
+        return L;
+    };
+    for (int l = nlevels - 1; l >= 0; --l) {
+        std::vector<int> out, add, ptr{0}, in, code;
+        for (int b : t.by_level[l]) {
+            if (!t.proxy_from_children[b]) continue;
+            out.push_back(up_of[b]);
+            add.push_back(1);
+            for (int k = 0; k < 8; ++k) {
+                const int ch = t.first_child[b] + k;
+                if (t.pwexp[ch] && t.src_count[ch] > 0) { in.push_back(up_of[ch]); code.push_back(k); }
+            }
+            ptr.push_back((int)in.size());
+        }
+        if (!out.empty()) c2p_levels.push_back(make_t3(out, add, ptr, in, code));
+    }
+    {
+        std::vector<Real> m(2 * (size_t)p * p);
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < p * p; ++i) m[(size_t)s * p * p + i] = (Real)T.p2c[s][i];
+        mats.upload(m);
+    }
+
+    // ---------------- plane-wave tables
+    {
+        std::vector<Cx<Real>> a(T.poly2pw.size()), b(T.pw2poly.size()), s(T.shift0.size());
+        for (size_t i = 0; i < a.size(); ++i) a[i] = {(Real)T.poly2pw[i].real(), (Real)T.poly2pw[i].imag()};
+        for (size_t i = 0; i < b.size(); ++i) b[i] = {(Real)T.pw2poly[i].real(), (Real)T.pw2poly[i].imag()};
+        for (size_t i = 0; i < s.size(); ++i) s[i] = {(Real)T.shift0[i].real(), (Real)T.shift0[i].imag()};
+        poly2pw.upload(a);
+        pw2poly.upload(b);
+        shift0.upload(s);
+        work.transform_flops += (double)n_up * 8.0 * ((double)p * p * H * p + (double)p * n_pw * H * p + (double)NM * p);
+    }
+
+    // ---------------- downward per level
+    std::vector<uint8_t> set_by_parent(nboxes, 0);
+    for (int l = 0; l < nlevels; ++l) {
+        std::vector<int> down, zero, nptr{0}, nslot, nsten;
+        std::vector<int> out, add, ptr{0}, in, code;
+        for (int b : t.by_level[l]) {
+            const bool has_out = t.trg_count[b] > 0;
+            if (!(t.pwexp[b] && t.src_count[b] + t.trg_count[b] > 0 && has_out)) continue;
+            down.push_back(down_of[b]);
+            zero.push_back(set_by_parent[b] ? 0 : 1);
+            nslot.push_back(up_of[b]);
+            nsten.push_back(13);
+            for (int s = 0; s < 27; ++s) {
+                if (s == 13) continue;
+                const int nb = t.colleague(b, s);
+                if (nb < 0) continue;
+                if (t.leaf[b] && t.leaf[nb]) continue;
+                if (up_of[nb] < 0) continue;
+                nslot.push_back(up_of[nb]);
+                nsten.push_back(26 - s);  // stencil of (centre_box - centre_nbr)
+            }
+            nptr.push_back((int)nslot.size());
+            if (!t.tpeval[b] && !t.leaf[b])
+                for (int k = 0; k < 8; ++k) {
+                    const int ch = t.first_child[b] + k;
+                    if (t.src_count[ch] + t.trg_count[ch] == 0) continue;
+                    out.push_back(down_of[ch]);
+                    add.push_back(0);
+                    in.push_back(down_of[b]);
+                    code.push_back(k);
+                    ptr.push_back((int)in.size());
+                    set_by_parent[ch] = 1;
+                }
+        }
+        EvalLevel E;
+        E.level = l;
+        E.n_items = (int)down.size();
+        E.down.upload(down);
+        E.zero.upload(zero);
+        E.nbr_ptr.upload(nptr);
+        E.nbr_slot.upload(nslot);
+        E.nbr_sten.upload(nsten);
+        work.transform_flops += 8.0 * (double)nslot.size() * NM;
+        work.transform_flops += (double)E.n_items * 8.0 *
+                                ((double)p * n_pw * H * n_pw + (double)p * p * H * n_pw + (double)P3 * H / 2.0);
+        eval_levels.push_back(std::move(E));
+        p2c_levels.push_back(make_t3(out, add, ptr, in, code));
+    }
+    {
+        std::vector<int> ld, lt;
+        std::vector<Real> lg;
+        for (int b = 0; b < nboxes; ++b) {
+            if (!(t.tpeval[b] && t.trg_count[b] > 0)) continue;
+            ld.push_back(down_of[b]);
+            for (int d = 0; d < 3; ++d) lg.push_back((Real)t.center[3 * b + d]);
+            lg.push_back((Real)(2.0 / t.boxsize[t.level[b]]));
+            lt.push_back(t.trg_begin[b]);
+            lt.push_back(t.trg_count[b]);
+            work.transform_flops += 2.0 * (double)P3 * t.trg_count[b];
+        }
+        n_leaf_eval = (int)ld.size();
+        leaf_down.upload(ld);
+        leaf_geom.upload(lg);
+        leaf_trg.upload(lt);
+    }
+
+    // ---------------- near field
+    {
+        std::vector<int> tr, ptr{0}, sr;
+        std::vector<Real> bsv;
+        double cand = 0;
+        for (int b = 0; b < nboxes; ++b) {
+            if (!t.leaf[b] || t.trg_count[b] == 0 || t.l1_begin[b + 1] == t.l1_begin[b]) continue;
+            tr.push_back(t.trg_begin[b]);
+            tr.push_back(t.trg_count[b]);
+            for (int e = t.l1_begin[b]; e < t.l1_begin[b + 1]; ++e) {
+                const int sb = t.l1_box[e];
+                double bs = t.boxsize[t.level[sb]];
+                if (t.pwexp[sb] && sb == b) bs *= 0.5;
+                else if (t.level[sb] < t.level[b]) bs = t.boxsize[t.level[b]];
+                sr.push_back(t.src_begin[sb]);
+                sr.push_back(t.src_count[sb]);
+                bsv.push_back((Real)bs);
+                cand += (double)t.trg_count[b] * t.src_count[sb];
+            }
+            ptr.push_back((int)bsv.size());
+        }
+        n_nf_leaves = (int)ptr.size() - 1;
+        nf_trg.upload(tr);
+        nf_ptr.upload(ptr);
+        nf_src.upload(sr);
+        nf_bs.upload(bsv);
+        std::vector<Real> cf(T.residual.begin(), T.residual.end());
+        coeffs.upload(cf);
+        work.nf_pairs_candidate = cand;
+        stats.n_list1 = (long long)bsv.size();
+        // near-field kernel HBM bytes: each list entry streams its sources once
+        double sb_bytes = 0;
+        for (size_t e = 0; e < bsv.size(); ++e) sb_bytes += (double)sr[2 * e + 1] * sizeof(P4<Real>);
+        work.nf_bytes = sb_bytes + (double)n * (sizeof(P4<Real>) + 2 * sizeof(Real));
+    }
+
+    // ---------------- far field
+    {
+        ms = T.ms; mt = T.mt; mu = T.mu;
+        std::vector<double> th(3 * (size_t)n);
+        const double *A = T.ainv;
+        for (int u = 0; u < n; ++u) {
+            const double x = g.r_src[3 * (size_t)u], y = g.r_src[3 * (size_t)u + 1], z = g.r_src[3 * (size_t)u + 2];
+            double f[3] = {A[0] * x + A[1] * y + A[3] * z, A[2] * y + A[4] * z, A[5] * z};
+            for (int d = 0; d < 3; ++d) {
+                f[d] -= std::floor(f[d]);
+                th[3 * (size_t)u + d] = 2.0 * M_PI * f[d];
+            }
+        }
+        theta.upload(th);
+        n_ff_modes = (int)T.mode_w.size();
+        ff_w.upload(T.mode_w);
+        std::vector<int> cijk, cm0, clen, ljk, li, lptr;
+        const auto &M = T.mode_ijk;
+        for (int m = 0; m < n_ff_modes;) {
+            int e = m + 1;
+            while (e < n_ff_modes && M[3 * e + 1] == M[3 * m + 1] && M[3 * e + 2] == M[3 * m + 2] &&
+                   M[3 * e] == M[3 * (e - 1)] + 1)
+                ++e;
+            ljk.push_back(M[3 * m + 1]);
+            ljk.push_back(M[3 * m + 2]);
+            li.push_back(M[3 * m]);
+            li.push_back(M[3 * (e - 1)]);
+            lptr.push_back(m);
+            for (int c = m; c < e; c += FF_CHUNK) {
+                cijk.push_back(M[3 * c]);
+                cijk.push_back(M[3 * c + 1]);
+                cijk.push_back(M[3 * c + 2]);
+                cm0.push_back(c);
+                clen.push_back(std::min(FF_CHUNK, e - c));
+            }
+            m = e;
+        }
+        n_ff_lines = (int)lptr.size();
+        n_ff_chunks = (int)cm0.size();
+        ff_chunk_ijk.upload(cijk);
+        ff_chunk_m0.upload(cm0);
+        ff_chunk_len.upload(clen);
+        ff_line_jk.upload(ljk);
+        ff_line_i.upload(li);
+        ff_line_ptr.upload(lptr);
+        ff_blocks = std::max(1, std::min(512, (n + FF_TP - 1) / FF_TP));
+        ff_Sp.alloc(sizeof(double2) * (size_t)ff_blocks * std::max(1, n_ff_modes));
+        ff_S.alloc(sizeof(double2) * (size_t)std::max(1, n_ff_modes));
+        work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
+        work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
+    }
+
+    // ---------------- scratch for the transform kernels
+    {
+        size_t per = std::max({2 * P3 * sizeof(Real), (size_t)(p * p * H + p * n_pw * H) * 2 * sizeof(Real),
+                               (size_t)(NM + p * n_pw * H + p * p * H) * 2 * sizeof(Real)});
+        int items = std::max(n_up, 1);
+        for (auto &E : eval_levels) items = std::max(items, E.n_items);
+        for (auto &L : c2p_levels) items = std::max(items, L.n_items);
+        for (auto &L : p2c_levels) items = std::max(items, L.n_items);
+        scratch_blocks = std::min(kMaxScratchBlocks, items);
+        scratch.alloc(per * scratch_blocks);
+    }
+
+    // ---------------- statistics and exact near-field pair count
+    stats.n_src = n; stats.nsall = nsall; stats.nboxes = nboxes; stats.n_levels = nlevels; stats.p = p;
+    stats.n_pw = n_pw; stats.ms = ms; stats.mt = mt; stats.mu = mu; stats.n_direct_leaves = n_nf_leaves;
+    stats.n_far_modes = n_ff_modes; stats.n_digits = n_digits; stats.is_float = sizeof(Real) == 4;
+    for (int b = 0; b < nboxes; ++b) stats.n_leaves += t.leaf[b];
+    if (n_nf_leaves > 0) {
+        DevBuf cnt;
+        cnt.alloc(sizeof(unsigned long long));
+        check(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), stream), "hipMemsetAsync");
+        hipLaunchKernelGGL(k_near_count<Real>, dim3(n_nf_leaves), dim3(256), 0, stream, src.as<P4<Real>>(),
+                           trg.as<P4<Real>>(), nf_trg.as<int>(), nf_ptr.as<int>(), nf_src.as<int>(), nf_bs.as<Real>(),
+                           cnt.as<unsigned long long>());
+        check(hipGetLastError(), "k_near_count launch");
+        unsigned long long c = 0;
+        check(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        work.nf_pairs_support = (double)c;
+    }
+    work.nf_flops = work.nf_pairs_support * (2.0 * nc + 12.0);
+    work.on_bytes = (double)nsall * (sizeof(int) + sizeof(Real) + sizeof(double)) +
+                    (double)n * (2 * sizeof(Real) + 2 * sizeof(double) + sizeof(int) + sizeof(double) + sizeof(Real));
+
+    // ---------------- topology export
+    box7.resize(7 * (size_t)nboxes);
+    box_centers.resize(3 * (size_t)nboxes);
+    for (int b = 0; b < nboxes; ++b) {
+        int v[7] = {t.level[b], t.parent[b], t.leaf[b] ? 0 : 8, t.src_count[b], t.trg_count[b], t.pwexp[b],
+                    t.l1_begin[b + 1] - t.l1_begin[b]};
+        for (int k = 0; k < 7; ++k) box7[7 * (size_t)b + k] = v[k];
+        for (int d = 0; d < 3; ++d) box_centers[3 * (size_t)b + d] = t.center[3 * b + d];
+    }
+    src_perm_host = t.src_order;
+    trg_perm_host = t.trg_order;
+}
+
+template <typename Real>
+Plan<Real>::~Plan() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+template <class Real, int P>
+void launch_transforms(Plan<Real> &pl, hipStream_t s) {
+    using C = Cx<Real>;
+    const int H = (pl.n_pw + 1) / 2;
+    if (pl.n_up > 0) {
+        const int bt = std::min(512, ((P * P + 63) / 64) * 64);
+        hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(bt), 0, s, pl.src.template as<P4<Real>>(),
+                           pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(), pl.up_rng.template as<int>(),
+                           pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(), pl.proxy_up.template as<Real>());
+        check(hipGetLastError(), "k_charge2proxy");
+    }
+    for (auto &L : pl.c2p_levels) {
+        const int nb = std::min(L.n_items, pl.scratch_blocks);
+        hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(nb), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
+                           pl.proxy_up.template as<Real>(), L.out.template as<int>(), L.add.template as<int>(),
+                           L.ptr.template as<int>(), L.in_slot.template as<int>(), L.in_code.template as<int>(),
+                           pl.mats.template as<Real>(), 1, L.n_items, pl.scratch.template as<Real>());
+        check(hipGetLastError(), "k_tensor3 c2p");
+    }
+    if (pl.n_up > 0) {
+        const int nb = std::min(pl.n_up, pl.scratch_blocks);
+        hipLaunchKernelGGL((k_proxy2pw<Real, P>), dim3(nb), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
+                           pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(), pl.n_pw, pl.n_up,
+                           pl.scratch.template as<C>());
+        check(hipGetLastError(), "k_proxy2pw");
+    }
+    for (size_t l = 0; l < pl.eval_levels.size(); ++l) {
+        auto &E = pl.eval_levels[l];
+        if (E.n_items > 0) {
+            const int nb = std::min(E.n_items, pl.scratch_blocks);
+            const Real lscale = (Real)std::ldexp(1.0, E.level);
+            hipLaunchKernelGGL((k_shift_pw2proxy<Real, P>), dim3(nb), dim3(256), 0, s, pl.pw_out.template as<C>(),
+                               pl.shift0.template as<C>(), lscale, E.down.template as<int>(), E.zero.template as<int>(),
+                               E.nbr_ptr.template as<int>(), E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(),
+                               pl.pw2poly.template as<C>(), pl.n_pw, E.n_items, pl.proxy_down.template as<Real>(),
+                               pl.scratch.template as<C>());
+            check(hipGetLastError(), "k_shift_pw2proxy");
+        }
+        auto &L = pl.p2c_levels[l];
+        if (L.n_items > 0) {
+            const int nb = std::min(L.n_items, pl.scratch_blocks);
+            hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(nb), dim3(256), 0, s, pl.proxy_down.template as<Real>(),
+                               pl.proxy_down.template as<Real>(), L.out.template as<int>(), L.add.template as<int>(),
+                               L.ptr.template as<int>(), L.in_slot.template as<int>(), L.in_code.template as<int>(),
+                               pl.mats.template as<Real>(), 0, L.n_items, pl.scratch.template as<Real>());
+            check(hipGetLastError(), "k_tensor3 p2c");
+        }
+    }
+    (void)H;
+    if (pl.n_leaf_eval > 0) {
+        hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(256), 0, s,
+                           pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
+                           pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(), pl.trg.template as<P4<Real>>(),
+                           pl.pot_sorted.template as<Real>());
+        check(hipGetLastError(), "k_eval_targets");
+    }
+}
+
+template <class Real, int NC, bool SPLIT>
+void launch_near(Plan<Real> &pl, hipStream_t s) {
+    if (pl.n_nf_leaves == 0) return;
+    hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3(pl.n_nf_leaves), dim3(256), 0, s,
+                       pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_trg.template as<int>(),
+                       pl.nf_ptr.template as<int>(), pl.nf_src.template as<int>(), pl.nf_bs.template as<Real>(),
+                       pl.coeffs.template as<Real>(), pl.pot_sorted.template as<Real>());
+    check(hipGetLastError(), "k_near_field");
+}
+
+template <class Real>
+void launch_far(Plan<Real> &pl, const double *d_q, hipStream_t s) {
+    if (pl.n_ff_modes == 0) {
+        check(hipMemsetAsync(pl.far.p, 0, sizeof(double) * pl.n, s), "hipMemsetAsync far");
+        return;
+    }
+    const size_t lds = sizeof(double2) * FF_TP * (pl.ms + pl.mt + pl.mu);
+    dim3 grid(pl.ff_blocks, (pl.n_ff_chunks + 255) / 256);
+    hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, pl.theta.template as<double>(), d_q, pl.n, pl.ms, pl.mt,
+                       pl.mu, pl.ff_chunk_ijk.template as<int>(), pl.ff_chunk_m0.template as<int>(),
+                       pl.ff_chunk_len.template as<int>(), pl.n_ff_chunks, pl.n_ff_modes, pl.ff_Sp.template as<double2>());
+    check(hipGetLastError(), "k_ff_structure");
+    hipLaunchKernelGGL(k_ff_reduce, dim3((pl.n_ff_modes + 255) / 256), dim3(256), 0, s, pl.ff_Sp.template as<double2>(),
+                       pl.ff_blocks, pl.ff_w.template as<double>(), pl.n_ff_modes, pl.ff_S.template as<double2>());
+    check(hipGetLastError(), "k_ff_reduce");
+    hipLaunchKernelGGL(k_ff_eval, dim3((pl.n + 255) / 256), dim3(256), 0, s, pl.theta.template as<double>(), pl.n,
+                       pl.ff_line_jk.template as<int>(), pl.ff_line_i.template as<int>(), pl.ff_line_ptr.template as<int>(),
+                       pl.n_ff_lines, pl.ff_S.template as<double2>(), pl.far.template as<double>());
+    check(hipGetLastError(), "k_ff_eval");
+}
+
+}  // namespace
+
+template <typename Real>
+void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int stages, double *times_ms) {
+    if (!s) s = stream;
+    if (times_ms) check(hipEventRecord(ev[0], s), "hipEventRecord");
+    // far field first: it depends only on the user-order charges
+    if (stages & 8) launch_far<Real>(*this, d_q, s);
+    if (times_ms) check(hipEventRecord(ev[1], s), "hipEventRecord");
+    {
+        const int m = std::max(nsall, n);
+        hipLaunchKernelGGL(k_gather_charges<Real>, dim3((m + 255) / 256), dim3(256), 0, s, d_q, src_user.as<int>(),
+                           src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), n);
+        check(hipGetLastError(), "k_gather_charges");
+    }
+    if (stages & 1) {
+        switch (p) {
+        case 9: launch_transforms<Real, 9>(*this, s); break;
+        case 18: launch_transforms<Real, 18>(*this, s); break;
+        case 28: launch_transforms<Real, 28>(*this, s); break;
+        case 38: launch_transforms<Real, 38>(*this, s); break;
+        default: throw std::logic_error("unsupported proxy order");
+        }
+    }
+    if (times_ms) check(hipEventRecord(ev[2], s), "hipEventRecord");
+    if (stages & 2) {
+        switch (n_digits) {
+        case 1: case 2: launch_near<Real, 5, true>(*this, s); break;
+        case 3: launch_near<Real, 6, true>(*this, s); break;
+        case 4: launch_near<Real, 9, true>(*this, s); break;
+        case 5: launch_near<Real, 11, true>(*this, s); break;
+        case 6: launch_near<Real, 12, false>(*this, s); break;
+        case 7: launch_near<Real, 15, false>(*this, s); break;
+        case 8: launch_near<Real, 17, false>(*this, s); break;
+        case 9: launch_near<Real, 18, false>(*this, s); break;
+        case 10: launch_near<Real, 22, false>(*this, s); break;
+        case 11: launch_near<Real, 23, false>(*this, s); break;
+        case 12: launch_near<Real, 25, false>(*this, s); break;
+        default: throw std::logic_error("unsupported digit count");
+        }
+    }
+    hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(), trg_w0.as<double>(),
+                       trg_user.as<int>(), d_q, far.as<double>(), d_pot, n, (stages & 4) ? 1 : 0, (stages & 8) ? 1 : 0);
+    check(hipGetLastError(), "k_finalize");
+    if (times_ms) {
+        check(hipEventRecord(ev[3], s), "hipEventRecord");
+        check(hipEventSynchronize(ev[3]), "hipEventSynchronize");
+        float a = 0, b = 0, c = 0;
+        check(hipEventElapsedTime(&a, ev[0], ev[1]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&b, ev[1], ev[2]), "hipEventElapsedTime");
+        check(hipEventElapsedTime(&c, ev[2], ev[3]), "hipEventElapsedTime");
+        times_ms[0] = a;  // nufft_ms
+        times_ms[1] = b;  // dmk_ms
+        times_ms[2] = c;  // direct_ms
+    }
+}
+
+template <typename Real>
+void Plan<Real>::evaluate_host(const double *q, Real *pot, double *times_ms) {
+    check(hipMemcpyAsync(q_dev.p, q, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    evaluate(q_dev.as<double>(), pot_dev.as<Real>(), stream, 15, times_ms);
+    check(hipMemcpyAsync(pot, pot_dev.p, sizeof(Real) * (size_t)n, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+template struct Plan<float>;
+template struct Plan<double>;
+
+}  // namespace pdmk::gpu

@@ -1,0 +1,110 @@
+// Device plan of one periodic DMK tree: everything evaluate() needs, resident in
+// HBM, plus the per-stage launch sequence.  Built once by the constructor of
+// PeriodicDMKTreeT (host tree + tables are flattened and uploaded); evaluate() only
+// moves charges in and potentials out.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace pdmk::gpu {
+
+void check(hipError_t e, const char *what);
+
+// RAII device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            std::swap(p, o.p);
+            std::swap(bytes, o.bytes);
+        }
+        return *this;
+    }
+    ~DevBuf();
+    void alloc(size_t nbytes);
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    template <class T> void upload(const std::vector<T> &v) {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+};
+
+struct Stats {
+    long long n_src = 0, nsall = 0, nboxes = 0, n_levels = 0, p = 0, n_pw = 0, ms = 0, mt = 0, mu = 0;
+    long long n_leaves = 0, n_list1 = 0, n_direct_leaves = 0, n_far_modes = 0, n_digits = 0, is_float = 0,
+              n_pair_tiles = 0;
+};
+
+struct Work {
+    double nf_pairs_support = 0, nf_pairs_candidate = 0, nf_flops = 0, transform_flops = 0;
+    double ff_mode_points = 0, ff_flops = 0, on_bytes = 0, nf_bytes = 0;
+};
+
+template <typename Real>
+struct Plan {
+    // sizes
+    int n = 0, nsall = 0, nboxes = 0, nlevels = 0, p = 0, n_pw = 0, nc = 0, n_digits = 0;
+    double rc = 0;
+    int n_up = 0, n_down = 0;     // proxy slots (upward == outgoing plane-wave slots)
+    int ms = 0, mt = 0, mu = 0, n_ff_modes = 0, n_ff_chunks = 0, n_ff_lines = 0, ff_blocks = 0;
+    Stats stats;
+    Work work;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    // O(N) data
+    DevBuf src, src_user, trg, trg_user, trg_w0, pot_sorted, q_dev, pot_dev, far;
+    // upward
+    DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, proxy_up;
+    struct Tensor3Launch { int n_items; DevBuf out, add, ptr, in_slot, in_code; };
+    std::vector<Tensor3Launch> c2p_levels;  // deepest first
+    DevBuf mats;                            // p2c minus / plus (p x p each)
+    // plane waves
+    DevBuf pw_out, poly2pw, pw2poly, shift0;
+    struct EvalLevel { int level, n_items; DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten; };
+    std::vector<EvalLevel> eval_levels;
+    std::vector<Tensor3Launch> p2c_levels;  // one per level (may be empty)
+    DevBuf proxy_down;
+    int n_leaf_eval = 0;
+    DevBuf leaf_down, leaf_geom, leaf_trg;
+    // near field
+    int n_nf_leaves = 0;
+    DevBuf nf_trg, nf_ptr, nf_src, nf_bs, coeffs;
+    // far field
+    DevBuf theta, ff_w, ff_chunk_ijk, ff_chunk_m0, ff_chunk_len, ff_line_jk, ff_line_i, ff_line_ptr, ff_Sp, ff_S;
+    // scratch for transform kernels
+    DevBuf scratch;
+    int scratch_blocks = 0;
+
+    Plan(const double *cell_colmajor, const double *pos, int N, int n_digits, int n_per_leaf, int nz);
+    ~Plan();
+    Plan(const Plan &) = delete;
+    Plan &operator=(const Plan &) = delete;
+
+    // Enqueue evaluate() on `s` (nullptr = own stream).  d_q: FP64 charges (user order),
+    // d_pot: Real potentials (user order).  stages: bit0 proxy/PW, bit1 near field,
+    // bit2 self correction, bit3 far field.  times (optional, synchronous): EvalTiming split.
+    void evaluate(const double *d_q, Real *d_pot, hipStream_t s, int stages = 15, double *times_ms = nullptr);
+
+    // host buffers in, host buffers out
+    void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
+
+    // topology export for parity tests
+    std::vector<int> box7;
+    std::vector<double> box_centers;
+    std::vector<int> src_perm_host, trg_perm_host;
+};
+
+extern template struct Plan<float>;
+extern template struct Plan<double>;
+
+}  // namespace pdmk::gpu
