@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: evaluate() throughput of the MI355X periodic DMK solver.
+
+Workload (the reference metric's configuration): cubic cell L = 1, N = 1,000,000
+positions uniform in the cell (splitmix64 seed 11), neutral U[-1,1] charges (seeds
+100..107, one fresh vector per step), n_per_leaf = 300, nz = 4.  The tree is built once
+(not timed); each step is one evaluate() with charges and potentials resident in HBM.
+
+Three paths are timed in one run:
+  * mixed6  -- n_digits = 6, FP32 kernels, FP64 far field and final accumulation (the value);
+  * fp64_6  -- n_digits = 6, plain FP64 path;
+  * fp64_9  -- n_digits = 9, plain FP64 path.
+The CPU baseline is the CPU oracle (a C++/OpenMP restatement of the reference
+evaluate()) timed on this host on the same workload.
+
+Prints ONE JSON line.  ``--gpus`` above 1 is not supported (single-GPU build).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "evaluate() Mpts/s @N=1e6, 6 & 9 digits: tuned mixed vs plain FP64 GPU"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def _run_path(torch, P, W, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, charges_dev):
+    t0 = time.time()
+    s = P.PDMKSolver(cell, pos, n_digits=n_digits, n_per_leaf=npl, nz=nz, precision=precision)
+    build_s = time.time() - t0
+    dt = torch.float32 if precision == "float" else torch.float64
+    out = torch.empty(n, device="cuda", dtype=dt)
+    stream = torch.cuda.current_stream().cuda_stream
+    for k in range(warmup):
+        s.evaluate_device(charges_dev[k % charges_dev.shape[0]].data_ptr(), out.data_ptr(), stream)
+    s.enable_kernel_timing(max(1, steps))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.evaluate_device(charges_dev[k % charges_dev.shape[0]].data_ptr(), out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ktimes, ncalls = s.kernel_times()
+    ms = 1000.0 * wall / steps
+    # potentials of the last step, for the error check against the CPU oracle
+    last_q = (steps - 1) % charges_dev.shape[0]
+    pot = out.double().cpu().numpy()
+    res = {
+        "precision": precision, "n_digits": n_digits, "ms_per_step": ms, "mpts_per_s": n / (ms / 1000.0) / 1e6,
+        "build_s": build_s, "kernel_ms": ktimes, "stats": s.stats(), "work": s.work(), "last_charge_index": last_q,
+    }
+    s.close()
+    return res, pot
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--npl", type=int, default=300)
+    ap.add_argument("--nz", type=int, default=4)
+    ap.add_argument("--paths", default="mixed6,fp64_6,fp64_9", help="comma list of mixed6, fp64_6, fp64_9")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only, "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import periodicdmk_amd as P
+    from periodicdmk_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    n = args.n
+    cell = W.cubic_cell()
+    pos = W.uniform_positions(n, 11, cell)
+    qs = np.stack([W.neutral_uniform_charges(n, 100 + k) for k in range(8)])
+    charges_dev = torch.tensor(qs, device="cuda", dtype=torch.float64)
+
+    paths = {"mixed6": ("float", 6), "fp64_6": ("double", 6), "fp64_9": ("double", 9)}
+    wanted = [p for p in args.paths.split(",") if p]
+    results, pots = {}, {}
+    for name in wanted:
+        prec, nd = paths[name]
+        print(f"[bench] {name}: building and timing", file=sys.stderr, flush=True)
+        results[name], pots[name] = _run_path(torch, P, W, cell, pos, n, prec, nd, args.steps, args.warmup,
+                                              args.npl, args.nz, charges_dev)
+        print(f"[bench] {name}: {results[name]['ms_per_step']:.3f} ms/step "
+              f"({results[name]['mpts_per_s']:.1f} Mpts/s) kernels {results[name]['kernel_ms']}",
+              file=sys.stderr, flush=True)
+
+    # CPU oracle on the same workload: timed baseline and error reference
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        ref = {}
+        for nd in sorted({paths[nm][1] for nm in wanted}):
+            print(f"[bench] CPU oracle, n_digits={nd}", file=sys.stderr, flush=True)
+            t0 = time.time()
+            o = O.OracleTree(cell, pos, nd, args.npl, args.nz, precision="double")
+            tb = time.time() - t0
+            qi = (args.steps - 1) % 8
+            t0 = time.perf_counter()
+            ref[nd] = o.evaluate(qs[qi])
+            te = time.perf_counter() - t0
+            o.close()
+            if nd == 6:
+                cpu = {"value": n / te / 1e6, "unit": "Mpts/s", "cores": threads, "kind": "port",
+                       "sample": f"full workload (N={n}, n_digits=6, FP64), one evaluate() after a "
+                                 f"{tb:.1f} s tree build; CPU oracle restating the reference evaluate() "
+                                 f"with an exact direct-DFT far field", "seconds": te}
+        for nm in wanted:
+            r = ref[paths[nm][1]]
+            d = pots[nm] - r
+            results[nm]["err_vs_cpu_oracle_rel_l2"] = float(np.linalg.norm(d) / np.linalg.norm(r))
+            results[nm]["err_vs_cpu_oracle_max_rel"] = float(np.max(np.abs(d)) / np.max(np.abs(r)))
+
+    head = results.get("mixed6") or next(iter(results.values()))
+    w = head["work"]
+    nf_ms = head["kernel_ms"]["near_field"]
+    peak = PEAK_FP32_TFLOPS if head["precision"] == "float" else PEAK_FP64_TFLOPS
+    achieved = w["nf_flops"] / (nf_ms / 1000.0) / 1e12 if nf_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "near_field_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": round(head["mpts_per_s"], 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64" if head["precision"] == "float" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cubic L=1, N={n} uniform positions (splitmix64 seed 11), neutral U[-1,1] charges "
+                               f"(seeds 100-107, fresh vector per step), n_per_leaf={args.npl}, nz={args.nz}; "
+                               f"value = n_digits=6 mixed path",
+                   "n_points": n, "n_digits": head["n_digits"], "n_per_leaf": args.npl, "nz": args.nz},
+        "roofline": {"bound": "mfma", "engine": "VALU (near-field residual kernel, vector FP32)" if
+                     head["precision"] == "float" else "VALU (near-field residual kernel, vector FP64)",
+                     "kernel": "k_near_field", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "flops_per_launch": w["nf_flops"], "pairs_in_support": w["nf_pairs_support"],
+                     "pairs_candidate": w["nf_pairs_candidate"], "kernel_ms": round(nf_ms, 4)},
+        "cpu_baseline": cpu,
+        "paths": {k: {kk: vv for kk, vv in v.items() if kk not in ("stats",)} for k, v in results.items()},
+    }
+    if "mixed6" in results and "fp64_6" in results:
+        line["mixed_vs_fp64_6"] = round(results["fp64_6"]["ms_per_step"] / results["mixed6"]["ms_per_step"], 3)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

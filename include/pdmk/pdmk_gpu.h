@@ -47,6 +47,14 @@ int pdmk_gpu_stats(void* h, long long* out16);
 /// computed on the host at build time.
 int pdmk_gpu_work(void* h, double* out8);
 
+/// Kernel-group timing with HIP events on the evaluate stream (measurement aid; no
+/// reference counterpart).  enable: keep the last `slots` evaluate() calls (0 turns it
+/// off).  times: after the caller synchronized, avg_ms[0..4] = mean duration per call of
+/// {far field, charge gather, proxy/plane-wave transforms, near field, finalize};
+/// returns the number of calls averaged.
+int pdmk_gpu_enable_kernel_timing(void* h, int slots);
+int pdmk_gpu_kernel_times(void* h, double* avg_ms5);
+
 /// Per-box topology (for comparison with a CPU restatement): box7[7*nboxes] =
 /// {depth, parent, nchild, src_count, trg_count, ifpwexp, n_list1}, centers[3*nboxes],
 /// src_perm[nsall] (sorted -> combined index), trg_perm[n_src] (sorted -> user index).

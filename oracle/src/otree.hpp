@@ -984,6 +984,12 @@ struct Tree {
 #else
             1;
 #endif
+        // per-point axis factors e^{-i m theta_d}, m in [-m_d/2, (m_d-1)/2], built by recurrence
+        auto axis = [](double th, int m, std::complex<double> *e) {
+            const std::complex<double> step = std::polar(1.0, -th);
+            std::complex<double> cur = std::polar(1.0, th * (m / 2));
+            for (int k = 0; k < m; ++k) { e[k] = cur; cur *= step; }
+        };
         std::vector<std::vector<std::complex<double>>> Sp(nthr, std::vector<std::complex<double>>(M, 0));
 #pragma omp parallel
         {
@@ -992,27 +998,37 @@ struct Tree {
             tid = omp_get_thread_num();
 #endif
             auto &loc = Sp[tid];
+            std::vector<std::complex<double>> e1(ms), e2(mt), e3(mu);
 #pragma omp for schedule(static)
             for (int jpt = 0; jpt < N; ++jpt) {
                 const double *t = &th[3 * (size_t)jpt];
+                axis(t[0], ms, e1.data());
+                axis(t[1], mt, e2.data());
+                axis(t[2], mu, e3.data());
                 const double qj = (double)q[jpt];
-                for (int m = 0; m < M; ++m) {
-                    const double ph = modes[m].i * t[0] + modes[m].j * t[1] + modes[m].k * t[2];
-                    loc[m] += qj * std::complex<double>(std::cos(ph), -std::sin(ph));
-                }
+                for (int m = 0; m < M; ++m)
+                    loc[m] += qj * e1[modes[m].i + ms / 2] * e2[modes[m].j + mt / 2] * e3[modes[m].k + mu / 2];
             }
         }
         for (int t = 0; t < nthr; ++t)
             for (int m = 0; m < M; ++m) S[m] += Sp[t][m];
-#pragma omp parallel for schedule(static)
-        for (int ipt = 0; ipt < N; ++ipt) {
-            const double *t = &th[3 * (size_t)ipt];
-            double acc = 0;
-            for (int m = 0; m < M; ++m) {
-                const double ph = modes[m].i * t[0] + modes[m].j * t[1] + modes[m].k * t[2];
-                acc += 2 * modes[m].w * (S[m].real() * std::cos(ph) - S[m].imag() * std::sin(ph));
+#pragma omp parallel
+        {
+            std::vector<std::complex<double>> e1(ms), e2(mt), e3(mu);
+#pragma omp for schedule(static)
+            for (int ipt = 0; ipt < N; ++ipt) {
+                const double *t = &th[3 * (size_t)ipt];
+                axis(t[0], ms, e1.data());
+                axis(t[1], mt, e2.data());
+                axis(t[2], mu, e3.data());
+                double acc = 0;
+                for (int m = 0; m < M; ++m) {
+                    // Re(S e^{+i k.x}) with e^{+i k.x} = conj(e1 e2 e3)
+                    const std::complex<double> e = e1[modes[m].i + ms / 2] * e2[modes[m].j + mt / 2] * e3[modes[m].k + mu / 2];
+                    acc += 2 * modes[m].w * (S[m].real() * e.real() + S[m].imag() * e.imag());
+                }
+                pot[ipt] += (Real)acc;
             }
-            pot[ipt] += (Real)acc;
         }
     }
 

@@ -59,6 +59,8 @@ def lib():
         "pdmk_gpu_stats": (C.c_int, [vp, lp]),
         "pdmk_gpu_work": (C.c_int, [vp, dp]),
         "pdmk_gpu_tree_arrays": (C.c_int, [vp, ip, dp, ip, ip]),
+        "pdmk_gpu_enable_kernel_timing": (C.c_int, [vp, C.c_int]),
+        "pdmk_gpu_kernel_times": (C.c_int, [vp, dp]),
         "pdmk_host_tree": (C.c_int, [dp, C.c_int, dp, C.c_int, C.c_int, lp, ip, dp, ip, ip, dp]),
         "pdmk_host_tables": (C.c_int, [C.c_int, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
         "pdmk_host_pswf": (C.c_double, [C.c_double, C.c_double]),
@@ -150,6 +152,20 @@ class PDMKSolver:
         o = np.zeros(8)
         lib().pdmk_gpu_work(self._h, _d(o))
         return dict(zip(WORK_KEYS, (float(v) for v in o)))
+
+    KERNEL_GROUPS = ("far_field", "gather", "transforms", "near_field", "finalize")
+
+    def enable_kernel_timing(self, slots=64):
+        if lib().pdmk_gpu_enable_kernel_timing(self._h, int(slots)) != 0:
+            raise RuntimeError("pdmk_gpu_enable_kernel_timing failed")
+
+    def kernel_times(self):
+        """Mean ms per call of each kernel group over the recorded calls (stream must be synchronized)."""
+        o = np.zeros(5)
+        n = lib().pdmk_gpu_kernel_times(self._h, _d(o))
+        if n < 0:
+            raise RuntimeError("pdmk_gpu_kernel_times failed")
+        return dict(zip(self.KERNEL_GROUPS, (float(v) for v in o))), int(n)
 
     def tree_arrays(self):
         s = self.stats()

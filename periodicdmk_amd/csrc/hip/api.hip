@@ -269,6 +269,25 @@ int pdmk_gpu_work(void *h, double *o) {
     return 0;
 }
 
+int pdmk_gpu_enable_kernel_timing(void *h, int slots) {
+    if (!h) return -1;
+    return guarded("pdmk_gpu_enable_kernel_timing", [&] {
+        if (static_cast<Handle *>(h)->is_float) tree_of<float>(h)->plan().enable_kernel_timing(slots);
+        else tree_of<double>(h)->plan().enable_kernel_timing(slots);
+        return 0;
+    });
+}
+
+int pdmk_gpu_kernel_times(void *h, double *avg_ms5) {
+    if (!h || !avg_ms5) return -1;
+    return guarded("pdmk_gpu_kernel_times", [&] {
+        int n = 0;
+        if (static_cast<Handle *>(h)->is_float) tree_of<float>(h)->plan().kernel_times(avg_ms5, &n);
+        else tree_of<double>(h)->plan().kernel_times(avg_ms5, &n);
+        return n;
+    });
+}
+
 int pdmk_gpu_tree_arrays(void *h, int *box7, double *centers, int *src_perm, int *trg_perm) {
     if (!h) return -1;
     auto copy = [&](auto &pl) {

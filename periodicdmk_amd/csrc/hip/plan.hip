@@ -405,6 +405,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
 template <typename Real>
 Plan<Real>::~Plan() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : kev) (void)hipEventDestroy(e);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -506,9 +507,16 @@ void launch_far(Plan<Real> &pl, const double *d_q, hipStream_t s) {
 template <typename Real>
 void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int stages, double *times_ms) {
     if (!s) s = stream;
+    hipEvent_t *K = nullptr;
+    if (kslots > 0) K = &kev[(size_t)(kcall++ % kslots) * (kGroups + 1)];
+    auto mark = [&](int g) {
+        if (K) check(hipEventRecord(K[g], s), "hipEventRecord");
+    };
     if (times_ms) check(hipEventRecord(ev[0], s), "hipEventRecord");
+    mark(0);
     // far field first: it depends only on the user-order charges
     if (stages & 8) launch_far<Real>(*this, d_q, s);
+    mark(1);
     if (times_ms) check(hipEventRecord(ev[1], s), "hipEventRecord");
     {
         const int m = std::max(nsall, n);
@@ -516,6 +524,7 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
                            src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), n);
         check(hipGetLastError(), "k_gather_charges");
     }
+    mark(2);
     if (stages & 1) {
         switch (p) {
         case 9: launch_transforms<Real, 9>(*this, s); break;
@@ -525,6 +534,7 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
         default: throw std::logic_error("unsupported proxy order");
         }
     }
+    mark(3);
     if (times_ms) check(hipEventRecord(ev[2], s), "hipEventRecord");
     if (stages & 2) {
         switch (n_digits) {
@@ -542,9 +552,11 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
         default: throw std::logic_error("unsupported digit count");
         }
     }
+    mark(4);
     hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(), trg_w0.as<double>(),
                        trg_user.as<int>(), d_q, far.as<double>(), d_pot, n, (stages & 4) ? 1 : 0, (stages & 8) ? 1 : 0);
     check(hipGetLastError(), "k_finalize");
+    mark(5);
     if (times_ms) {
         check(hipEventRecord(ev[3], s), "hipEventRecord");
         check(hipEventSynchronize(ev[3]), "hipEventSynchronize");
@@ -564,6 +576,31 @@ void Plan<Real>::evaluate_host(const double *q, Real *pot, double *times_ms) {
     evaluate(q_dev.as<double>(), pot_dev.as<Real>(), stream, 15, times_ms);
     check(hipMemcpyAsync(pot, pot_dev.p, sizeof(Real) * (size_t)n, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync D2H");
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+template <typename Real>
+void Plan<Real>::enable_kernel_timing(int slots) {
+    for (auto e : kev) (void)hipEventDestroy(e);
+    kev.clear();
+    kslots = std::max(0, slots);
+    kcall = 0;
+    kev.resize((size_t)kslots * (kGroups + 1));
+    for (auto &e : kev) check(hipEventCreate(&e), "hipEventCreate");
+}
+
+template <typename Real>
+int Plan<Real>::kernel_times(double *avg_ms, int *n_calls) {
+    const int nc = (int)std::min<long>(kcall, kslots);
+    for (int g = 0; g < kGroups; ++g) avg_ms[g] = 0;
+    for (int c = 0; c < nc; ++c)
+        for (int g = 0; g < kGroups; ++g) {
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, kev[(size_t)c * (kGroups + 1) + g], kev[(size_t)c * (kGroups + 1) + g + 1]),
+                  "hipEventElapsedTime");
+            avg_ms[g] += ms / nc;
+        }
+    if (n_calls) *n_calls = nc;
+    return nc;
 }
 
 template struct Plan<float>;

@@ -98,6 +98,17 @@ struct Plan {
     // host buffers in, host buffers out
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
 
+    // Per-kernel-group timing with HIP events on the evaluate stream.  Groups:
+    // 0 far field, 1 charge gather, 2 proxy/plane-wave transforms, 3 near field,
+    // 4 finalize.  A ring of `slots` calls is kept; kernel_times() averages the
+    // recorded calls (after the caller has synchronized the stream).
+    static constexpr int kGroups = 5;
+    void enable_kernel_timing(int slots);
+    int kernel_times(double *avg_ms, int *n_calls);
+    std::vector<hipEvent_t> kev;
+    int kslots = 0;
+    long kcall = 0;
+
     // topology export for parity tests
     std::vector<int> box7;
     std::vector<double> box_centers;
