@@ -12,6 +12,16 @@
 extern "C" {
 #endif
 
+/// Device-memory helpers for callers without a HIP runtime of their own (Python
+/// tests and the benchmark use them so that one process holds one HIP runtime).
+/// No reference counterpart.  malloc returns NULL on failure; the others return 0 on
+/// success and -1 on error.  kind: 1 host->device, 2 device->host, 3 device->device.
+int   pdmk_gpu_device_count(void);
+void* pdmk_gpu_malloc(unsigned long long bytes);
+int   pdmk_gpu_free(void* p);
+int   pdmk_gpu_memcpy(void* dst, const void* src, unsigned long long bytes, int kind);
+int   pdmk_gpu_synchronize(void);
+
 /// Number of points owned by a handle (PeriodicDMKTreeT::n_points,
 /// cpp/include/pdmk/periodic_dmk.hpp:52).  Returns -1 for a NULL handle.
 int pdmk_gpu_n_points(void* h);

@@ -51,6 +51,11 @@ def lib():
         "pdmk_create_float": (vp, [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int]),
         "pdmk_evaluate_float": (C.c_int, [vp, C.c_int, dp, dp, fp]),
         "pdmk_destroy_float": (None, [vp]),
+        "pdmk_gpu_device_count": (C.c_int, []),
+        "pdmk_gpu_malloc": (vp, [C.c_ulonglong]),
+        "pdmk_gpu_free": (C.c_int, [vp]),
+        "pdmk_gpu_memcpy": (C.c_int, [vp, vp, C.c_ulonglong, C.c_int]),
+        "pdmk_gpu_synchronize": (C.c_int, []),
         "pdmk_gpu_n_points": (C.c_int, [vp]),
         "pdmk_gpu_evaluate_double": (C.c_int, [vp, vp, vp, vp]),
         "pdmk_gpu_evaluate_float": (C.c_int, [vp, vp, vp, vp]),
@@ -91,6 +96,61 @@ def _positions(positions):
     if pos.ndim != 2 or pos.shape[1] != 3:
         raise ValueError("positions must have shape (N, 3)")
     return pos
+
+
+def device_count() -> int:
+    return int(lib().pdmk_gpu_device_count())
+
+
+def synchronize() -> None:
+    if lib().pdmk_gpu_synchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
+class DeviceArray:
+    """A 1-D device buffer allocated through libpdmk.so (one HIP runtime per process)."""
+
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = lib().pdmk_gpu_malloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError(f"device allocation of {self.nbytes} bytes failed")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError("size mismatch")
+        if lib().pdmk_gpu_memcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.nbytes, 1) != 0:
+            raise RuntimeError("host->device copy failed")
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        if lib().pdmk_gpu_memcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes, 2) != 0:
+            raise RuntimeError("device->host copy failed")
+        return out
+
+    def offset(self, elements: int) -> int:
+        return self.ptr + int(elements) * self.dtype.itemsize
+
+    def free(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.pdmk_gpu_free(C.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class PDMKSolver:
@@ -142,6 +202,16 @@ class PDMKSolver:
     def evaluate_stages(self, d_charges: int, d_pot: int, stages: int):
         if lib().pdmk_gpu_evaluate_stages(self._h, C.c_void_p(d_charges), C.c_void_p(d_pot), int(stages)) != 0:
             raise RuntimeError("pdmk_gpu_evaluate_stages failed (see stderr)")
+
+    def run_stages(self, charges, stages: int):
+        """Host arrays in/out: run only the selected stages (bits as in STAGE_*)."""
+        qd = DeviceArray.from_numpy(np.asarray(charges, dtype=np.float64))
+        out = DeviceArray(self.n, self.dtype)
+        self.evaluate_stages(qd.ptr, out.ptr, stages)
+        res = out.numpy()
+        qd.free()
+        out.free()
+        return res
 
     def stats(self):
         o = np.zeros(16, dtype=np.int64)

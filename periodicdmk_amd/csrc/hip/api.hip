@@ -202,6 +202,38 @@ int pdmk_evaluate_float(void *h, int, const double *, const double *charges, flo
 }
 void pdmk_destroy_float(void *h) { destroy_impl<float>(h); }
 
+// ---------------------------------------------------------------- device memory helpers
+int pdmk_gpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void *pdmk_gpu_malloc(unsigned long long bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        std::fprintf(stderr, "pdmk_gpu_malloc: hipMalloc of %llu bytes failed\n", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+int pdmk_gpu_free(void *p) { return (p && hipFree(p) != hipSuccess) ? -1 : 0; }
+
+int pdmk_gpu_memcpy(void *dst, const void *src, unsigned long long bytes, int kind) {
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                                      : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    if (kind < 1 || kind > 3) return -1;
+    const hipError_t e = hipMemcpy(dst, src, bytes, k);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "pdmk_gpu_memcpy: %s\n", hipGetErrorString(e));
+        return -1;
+    }
+    return 0;
+}
+
+int pdmk_gpu_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+
 // ---------------------------------------------------------------- device-resident entry points
 int pdmk_gpu_n_points(void *h) {
     if (!h) return -1;

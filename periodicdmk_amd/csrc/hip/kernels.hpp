@@ -58,15 +58,17 @@ __global__ void k_finalize(const R *__restrict__ pot_sorted, const double *__res
 // One workgroup per destination box.  Thread t owns the (b, c) column
 // C[0..P-1, b, c] of the proxy coefficients and accumulates
 //   C[a,b,c] = sum_s T_a(x_s) T_b(y_s) T_c(z_s) q_s,  x_s = sc (r_s - centre).
+// Sources hold coordinates relative to their leaf centre; each range lies in one
+// leaf and carries rng_off = leaf centre - box centre, so x_s = sc (local_s + off).
 template <class R, int P>
 __global__ __launch_bounds__(1024) void k_charge2proxy(const P4<R> *__restrict__ src, const int *__restrict__ grp_slot,
                                                         const R *__restrict__ grp_geom, const int *__restrict__ grp_rng,
                                                         const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
-                                                        R *__restrict__ proxy) {
+                                                        const R *__restrict__ rng_off, R *__restrict__ proxy) {
     constexpr int TS = 32;
     __shared__ R Tx[TS * P], Ty[TS * P], Tz[TS * P];
     const int g = blockIdx.x;
-    const R cx = grp_geom[4 * g], cy = grp_geom[4 * g + 1], cz = grp_geom[4 * g + 2], sc = grp_geom[4 * g + 3];
+    const R sc = grp_geom[4 * g + 3];
     R *out = proxy + (size_t)grp_slot[g] * (P * P * P);
     for (int pc0 = 0; pc0 < P * P; pc0 += blockDim.x) {
         const int pc = pc0 + threadIdx.x;
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(1024) void k_charge2proxy(const P4<R> *__restrict__
                     R *T = (d == 0 ? Tx : (d == 1 ? Ty : Tz)) + s * P;
                     if (s0 + s < cnt) {
                         const P4<R> pt = src[beg + s0 + s];
-                        const R x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) - (d == 0 ? cx : (d == 1 ? cy : cz)));
+                        const R x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) + rng_off[4 * r + d]);
                         const R m = (d == 2) ? pt.w : R(1);
                         R t0 = m, t1 = x * m;
                         T[0] = t0;
@@ -323,24 +325,32 @@ __global__ __launch_bounds__(256) void k_eval_targets(const R *__restrict__ down
 // every source box in list1, sources are staged through LDS and the residual
 // u = P((r - bs/2) 2/bs) / r, 1e-30 < r^2 < bs^2, is accumulated.  For fewer than 6
 // digits P is used in r (even/odd split), as in laplace_3d_poly_all_pairs.
+// Coordinates are relative to each point's leaf centre; l1_geom[e] = (centre of the
+// target leaf - centre of the source leaf, bs), so r = |(t + off) - s|.  Each list entry
+// is summed in R and the per-target total in FP64.
 template <class R, int NC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
                                                      const int *__restrict__ leaf_trg, const int *__restrict__ l1_ptr,
-                                                     const int *__restrict__ l1_src, const R *__restrict__ l1_bs,
+                                                     const int *__restrict__ l1_src, const P4<R> *__restrict__ l1_geom,
                                                      const R *__restrict__ coeffs, R *__restrict__ pot_sorted) {
     constexpr int TILE = 256;
     __shared__ P4<R> tile[TILE];
     __shared__ R cf[64];
+    R c[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
     const int lf = blockIdx.x;
     const int tbeg = leaf_trg[2 * lf], tcnt = leaf_trg[2 * lf + 1];
     for (int rep = 0; rep * blockDim.x < tcnt; ++rep) {
         const int t = rep * blockDim.x + threadIdx.x;
         const bool active = t < tcnt;
-        P4<R> me = active ? trg[tbeg + t] : P4<R>{R(0), R(0), R(0), R(0)};
-        R acc = 0;
+        const P4<R> me = active ? trg[tbeg + t] : P4<R>{R(0), R(0), R(0), R(0)};
+        double acc = 0;
         for (int e = l1_ptr[lf]; e < l1_ptr[lf + 1]; ++e) {
             const int sbeg = l1_src[2 * e], scnt = l1_src[2 * e + 1];
-            const R bs = l1_bs[e];
+            const P4<R> gm = l1_geom[e];
+            const R bs = gm.w;
+            const R tx = me.x + gm.x, ty = me.y + gm.y, tz = me.z + gm.z;
             const R d2max = bs * bs, rsc = R(2) / bs, cen = -bs / R(2);
             __syncthreads();
             if (SPLIT && threadIdx.x == 0) {
@@ -352,6 +362,7 @@ __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ sr
                     for (int j = NC - 1; j > i; --j) cd[j - 1] += (double)cen * cd[j];
                 for (int i = 0; i < NC; ++i) cf[i] = (R)cd[i];
             }
+            R ea = 0;
             for (int s0 = 0; s0 < scnt; s0 += TILE) {
                 const int ns = min(TILE, scnt - s0);
                 __syncthreads();
@@ -360,7 +371,7 @@ __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ sr
                 if (active) {
                     for (int s = 0; s < ns; ++s) {
                         const P4<R> q = tile[s];
-                        const R dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+                        const R dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
                         const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                         const R rinv = rsqrt_full<R>(r2);
                         R u;
@@ -375,18 +386,19 @@ __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ sr
                             u = fma(E, rinv, O);
                         } else {
                             const R x = fma(r2, rinv, cen) * rsc;
-                            R p = coeffs[NC - 1];
+                            R p = c[NC - 1];
 #pragma unroll
-                            for (int i = NC - 2; i >= 0; --i) p = fma(p, x, coeffs[i]);
+                            for (int i = NC - 2; i >= 0; --i) p = fma(p, x, c[i]);
                             u = p * rinv;
                         }
                         const bool in = (r2 > R(1e-30)) && (r2 < d2max);
-                        acc = in ? fma(u, q.w, acc) : acc;
+                        ea = in ? fma(u, q.w, ea) : ea;
                     }
                 }
             }
+            acc += (double)ea;
         }
-        if (active) pot_sorted[tbeg + t] += acc;
+        if (active) pot_sorted[tbeg + t] += (R)acc;
     }
 }
 

@@ -41,7 +41,7 @@ struct CounterKernel {};
 // Exact count of near-field pairs inside the support (1e-30 < r^2 < bs^2), once per tree.
 template <class R>
 __global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg, const int *__restrict__ leaf_trg,
-                             const int *__restrict__ l1_ptr, const int *__restrict__ l1_src, const R *__restrict__ l1_bs,
+                             const int *__restrict__ l1_ptr, const int *__restrict__ l1_src, const P4<R> *__restrict__ l1_geom,
                              unsigned long long *__restrict__ counter) {
     const int lf = blockIdx.x;
     const int tbeg = leaf_trg[2 * lf], tcnt = leaf_trg[2 * lf + 1];
@@ -49,10 +49,11 @@ __global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restr
     for (int t = threadIdx.x; t < tcnt; t += blockDim.x) {
         const P4<R> me = trg[tbeg + t];
         for (int e = l1_ptr[lf]; e < l1_ptr[lf + 1]; ++e) {
-            const R bs = l1_bs[e], d2 = bs * bs;
+            const P4<R> gm = l1_geom[e];
+            const R bs = gm.w, d2 = bs * bs;
             for (int s = 0; s < l1_src[2 * e + 1]; ++s) {
                 const P4<R> q = src[l1_src[2 * e] + s];
-                const R dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+                const R dx = me.x + gm.x - q.x, dy = me.y + gm.y - q.y, dz = me.z + gm.z - q.z;
                 const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                 c += (r2 > R(1e-30) && r2 < d2) ? 1 : 0;
             }
@@ -83,20 +84,32 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     check(hipStreamCreate(&stream), "hipStreamCreate");
     for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
 
-    // ---------------- sources and targets (sorted order)
+    // ---------------- sources and targets (sorted order), coordinates relative to their
+    // leaf centre: offsets between boxes are formed in FP64 on the host, so FP32 kernels
+    // only see small local coordinates
+    std::vector<int> src_leaf(nsall, -1), trg_leaf(n, -1);
+    for (int b = 0; b < nboxes; ++b) {
+        if (!t.leaf[b]) continue;
+        for (int i = 0; i < t.src_count[b]; ++i) src_leaf[t.src_begin[b] + i] = b;
+        for (int i = 0; i < t.trg_count[b]; ++i) trg_leaf[t.trg_begin[b] + i] = b;
+    }
+    auto local = [&](const std::vector<double> &xyz, size_t i, int b, int d) {
+        return (Real)(xyz[3 * i + d] - t.center[3 * (size_t)b + d]);
+    };
     std::vector<P4<Real>> srcv(nsall);
     std::vector<int> su(nsall);
     for (int i = 0; i < nsall; ++i) {
-        srcv[i] = {(Real)t.src_sorted[3 * (size_t)i], (Real)t.src_sorted[3 * (size_t)i + 1],
-                   (Real)t.src_sorted[3 * (size_t)i + 2], Real(0)};
+        const int b = src_leaf[i];
+        srcv[i] = {local(t.src_sorted, i, b, 0), local(t.src_sorted, i, b, 1), local(t.src_sorted, i, b, 2), Real(0)};
         const int o = t.src_order[i];
         su[i] = o < N ? o : g.img_src[o - N];
     }
     std::vector<P4<Real>> trgv(n);
     std::vector<double> tw0(n, 0.0);
-    for (int i = 0; i < n; ++i)
-        trgv[i] = {(Real)t.trg_sorted[3 * (size_t)i], (Real)t.trg_sorted[3 * (size_t)i + 1],
-                   (Real)t.trg_sorted[3 * (size_t)i + 2], Real(0)};
+    for (int i = 0; i < n; ++i) {
+        const int b = trg_leaf[i];
+        trgv[i] = {local(t.trg_sorted, i, b, 0), local(t.trg_sorted, i, b, 1), local(t.trg_sorted, i, b, 2), Real(0)};
+    }
     for (int b = 0; b < nboxes; ++b)
         if (t.leaf[b] && t.trg_count[b] > 0) {
             const double w = T.w0[t.level[b] + t.pwexp[b]];
@@ -129,7 +142,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         std::unordered_map<int, const host::Tree::UpGroup *> grp_of;
         for (const auto &gr : t.up_groups) grp_of[gr.box] = &gr;
         std::vector<int> slot(n_up), rng(n_up + 1, 0), rb, rcnt;
-        std::vector<Real> geom(4 * (size_t)n_up);
+        std::vector<Real> geom(4 * (size_t)n_up), roff;
         for (int b = 0; b < nboxes; ++b) {
             const int k = up_of[b];
             if (k < 0) continue;
@@ -138,12 +151,21 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             geom[4 * k + 3] = (Real)(2.0 / t.boxsize[t.level[b]]);
             rng[k + 1] = 0;
             auto it = grp_of.find(b);
-            if (it != grp_of.end())
-                for (int sb : it->second->src_boxes) {
-                    rb.push_back(t.src_begin[sb]);
-                    rcnt.push_back(t.src_count[sb]);
+            if (it == grp_of.end()) continue;
+            for (int sb : it->second->src_boxes) {
+                // split the source range at leaf boundaries: one centre offset per piece
+                const int end = t.src_begin[sb] + t.src_count[sb];
+                for (int i = t.src_begin[sb]; i < end;) {
+                    const int lf = src_leaf[i];
+                    const int stop = std::min(end, t.src_begin[lf] + t.src_count[lf]);
+                    rb.push_back(i);
+                    rcnt.push_back(stop - i);
+                    for (int d = 0; d < 3; ++d) roff.push_back((Real)(t.center[3 * lf + d] - t.center[3 * b + d]));
+                    roff.push_back(Real(0));
                     rng[k + 1]++;
+                    i = stop;
                 }
+            }
         }
         // ranges were appended in box order == slot order
         for (int k = 0; k < n_up; ++k) rng[k + 1] += rng[k];
@@ -152,6 +174,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         up_rng.upload(rng);
         rng_begin.upload(rb);
         rng_count.upload(rcnt);
+        rng_off.upload(roff);
         double c2p_src = 0;
         for (size_t i = 0; i < rcnt.size(); ++i) c2p_src += rcnt[i];
         work.transform_flops += 2.0 * (double)P3 * c2p_src;
@@ -255,7 +278,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         for (int b = 0; b < nboxes; ++b) {
             if (!(t.tpeval[b] && t.trg_count[b] > 0)) continue;
             ld.push_back(down_of[b]);
-            for (int d = 0; d < 3; ++d) lg.push_back((Real)t.center[3 * b + d]);
+            for (int d = 0; d < 3; ++d) lg.push_back(Real(0));  // targets are relative to this leaf's centre
             lg.push_back((Real)(2.0 / t.boxsize[t.level[b]]));
             lt.push_back(t.trg_begin[b]);
             lt.push_back(t.trg_count[b]);
@@ -270,7 +293,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     // ---------------- near field
     {
         std::vector<int> tr, ptr{0}, sr;
-        std::vector<Real> bsv;
+        std::vector<P4<Real>> bsv;  // (centre of target leaf - centre of source leaf, bs)
         double cand = 0;
         for (int b = 0; b < nboxes; ++b) {
             if (!t.leaf[b] || t.trg_count[b] == 0 || t.l1_begin[b + 1] == t.l1_begin[b]) continue;
@@ -283,7 +306,9 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 else if (t.level[sb] < t.level[b]) bs = t.boxsize[t.level[b]];
                 sr.push_back(t.src_begin[sb]);
                 sr.push_back(t.src_count[sb]);
-                bsv.push_back((Real)bs);
+                bsv.push_back({(Real)(t.center[3 * b] - t.center[3 * sb]),
+                               (Real)(t.center[3 * b + 1] - t.center[3 * sb + 1]),
+                               (Real)(t.center[3 * b + 2] - t.center[3 * sb + 2]), (Real)bs});
                 cand += (double)t.trg_count[b] * t.src_count[sb];
             }
             ptr.push_back((int)bsv.size());
@@ -377,7 +402,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         cnt.alloc(sizeof(unsigned long long));
         check(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), stream), "hipMemsetAsync");
         hipLaunchKernelGGL(k_near_count<Real>, dim3(n_nf_leaves), dim3(256), 0, stream, src.as<P4<Real>>(),
-                           trg.as<P4<Real>>(), nf_trg.as<int>(), nf_ptr.as<int>(), nf_src.as<int>(), nf_bs.as<Real>(),
+                           trg.as<P4<Real>>(), nf_trg.as<int>(), nf_ptr.as<int>(), nf_src.as<int>(), nf_bs.as<P4<Real>>(),
                            cnt.as<unsigned long long>());
         check(hipGetLastError(), "k_near_count launch");
         unsigned long long c = 0;
@@ -421,7 +446,8 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         const int bt = std::min(512, ((P * P + 63) / 64) * 64);
         hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(bt), 0, s, pl.src.template as<P4<Real>>(),
                            pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(), pl.up_rng.template as<int>(),
-                           pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(), pl.proxy_up.template as<Real>());
+                           pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(), pl.rng_off.template as<Real>(),
+                           pl.proxy_up.template as<Real>());
         check(hipGetLastError(), "k_charge2proxy");
     }
     for (auto &L : pl.c2p_levels) {
@@ -476,7 +502,7 @@ void launch_near(Plan<Real> &pl, hipStream_t s) {
     if (pl.n_nf_leaves == 0) return;
     hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3(pl.n_nf_leaves), dim3(256), 0, s,
                        pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_trg.template as<int>(),
-                       pl.nf_ptr.template as<int>(), pl.nf_src.template as<int>(), pl.nf_bs.template as<Real>(),
+                       pl.nf_ptr.template as<int>(), pl.nf_src.template as<int>(), pl.nf_bs.template as<P4<Real>>(),
                        pl.coeffs.template as<Real>(), pl.pot_sorted.template as<Real>());
     check(hipGetLastError(), "k_near_field");
 }

@@ -64,7 +64,7 @@ struct Plan {
     // O(N) data
     DevBuf src, src_user, trg, trg_user, trg_w0, pot_sorted, q_dev, pot_dev, far;
     // upward
-    DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, proxy_up;
+    DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, rng_off, proxy_up;
     struct Tensor3Launch { int n_items; DevBuf out, add, ptr, in_slot, in_code; };
     std::vector<Tensor3Launch> c2p_levels;  // deepest first
     DevBuf mats;                            // p2c minus / plus (p x p each)

@@ -1,0 +1,191 @@
+"""GPU parity of the MI355X evaluate() path against the CPU oracle and FP64 Ewald.
+
+Each stage of evaluate() (bit 0 proxy/plane-wave part, bit 1 near field, bit 2 self
+correction, bit 3 far field, 15 = all) is compared with the oracle's restatement of the
+reference stage on the same seeded inputs:
+  * FP64 kernels: relative L2 difference < 1e-12 (same tree, same tables up to rounding,
+    different summation order only);
+  * FP32 kernels (n_digits <= 6): relative L2 difference < 1e-5 against the FP64 oracle
+    (FP32 positions, transforms and near-field sums).
+End-to-end checks against Ewald follow cpp/tests/test_periodic_dmk.cpp:20-68 (< 1e-4).
+"""
+import numpy as np
+import pytest
+
+from periodicdmk_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+STAGES = (1, 2, 4, 8, 15)
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def gpu_stages(solver, q, stages):
+    return solver.run_stages(q, stages).astype(np.float64)
+
+
+def case(name):
+    if name == "cubic":
+        c = W.cubic_cell()
+        return c, W.uniform_positions(3000, 1, c), W.neutral_uniform_charges(3000, 2), 40
+    if name == "triclinic":
+        c = W.triclinic_cell()
+        return c, W.uniform_positions(4000, 3, c), W.neutral_uniform_charges(4000, 4), 40
+    if name == "clustered":
+        c = W.triclinic_cell()
+        return c, W.gaussian_mixture(6000, 8, 0.02, 21, c), W.pm1_charges(6000, 22), 30
+    if name == "single_level":  # N = 1e4, n_per_leaf 300: every leaf is a level-0 box
+        c = W.cubic_cell()
+        return c, W.uniform_positions(10_000, 1, c), W.neutral_uniform_charges(10_000, 2), 300
+    if name == "long_cell":  # |a2| > |a1|: the lattice frame reorders the axes
+        c = np.array([[1.0, 0.0, 0.1], [0.0, 1.7, 0.0], [0.0, 0.2, 1.2]])
+        return c, W.uniform_positions(3000, 9, c), W.neutral_uniform_charges(3000, 10), 40
+    raise KeyError(name)
+
+
+PARAMS = [("cubic", "double", d) for d in (3, 6, 9, 12)] + [
+    ("cubic", "float", 6), ("cubic", "float", 3),
+    ("triclinic", "double", 6), ("triclinic", "double", 9), ("triclinic", "float", 6),
+    ("clustered", "double", 6), ("clustered", "double", 9), ("clustered", "float", 6),
+    ("single_level", "double", 6), ("single_level", "float", 6),
+    ("long_cell", "double", 6),
+]
+
+
+@pytest.mark.parametrize("name,precision,digits", PARAMS)
+def test_stage_parity(pdmk, oracle_mod, cuda, name, precision, digits):
+    cell, pos, q, npl = case(name)
+    s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=npl, nz=4, precision=precision)
+    o = oracle_mod.OracleTree(cell, pos, digits, npl, 4, precision="double")
+    tol = 1e-12 if precision == "double" else 1e-5
+    for st in STAGES:
+        g = gpu_stages(s, q, st)
+        r = o.evaluate(q, st)
+        assert rel(g, r) < tol, (st, rel(g, r))
+    s.close()
+
+
+def test_tree_topology_matches_oracle(pdmk, oracle_mod, cuda):
+    cell, pos, q, npl = case("clustered")
+    s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision="double")
+    box7, sp, tp, cen = s.tree_arrays()
+    o = oracle_mod.OracleTree(cell, pos, 6, npl, 4)
+    ob7, osp, otp, ocen = o.tree_arrays()
+    np.testing.assert_array_equal(box7, ob7)
+    np.testing.assert_array_equal(sp, osp)
+    np.testing.assert_array_equal(tp, otp)
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-5), ("float", 1e-4)])
+def test_reference_cubic_case_vs_ewald(pdmk, oracle_mod, cuda, precision, tol):
+    L = 3.0
+    rng = np.random.default_rng(42)
+    pos = rng.uniform(0.03 * L, 0.97 * L, (200, 3))
+    q = rng.normal(size=200)
+    q[0] -= q.sum()
+    ref, _ = oracle_mod.ewald_direct(np.eye(3) * L, pos, q, 5.0)
+    s = pdmk.PDMKSolver(np.eye(3) * L, pos, n_digits=6, n_per_leaf=20, precision=precision)
+    assert rel(s.evaluate(q), ref) < tol
+
+
+def test_reference_triclinic_case_vs_ewald(pdmk, oracle_mod, cuda):
+    cell = np.array([[5.0, 1.0, 0.5], [0.0, 4.5, 0.8], [0.0, 0.0, 4.0]])
+    rng = np.random.default_rng(77)
+    pos = rng.uniform(0.01, 0.99, (100, 3)) @ cell.T
+    q = rng.normal(size=100)
+    q[0] -= q.sum()
+    ref, _ = oracle_mod.ewald_direct(cell, pos, q, 5.0)
+    s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=20, precision="double")
+    assert rel(s.evaluate(q), ref) < 1e-5
+
+
+def test_madelung_on_gpu(pdmk, cuda):
+    pos = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5], [.5, 0, 0], [0, .5, 0], [0, 0, .5], [.5, .5, .5]])
+    q = np.array([1, 1, 1, 1, -1, -1, -1, -1.0])
+    for prec in ("double", "float"):
+        s = pdmk.PDMKSolver(np.eye(3), pos, n_digits=6, n_per_leaf=4, precision=prec)
+        m = -0.5 * np.dot(q, s.evaluate(q).astype(np.float64)) / 8
+        assert abs(m - 1.7475645946331822) < 1e-4
+
+
+def test_9_digits_vs_golden_ewald_cfg1(pdmk, cuda):
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cfg1_ewald.npz"))
+    cell, pos, q, npl = case("single_level")
+    s = pdmk.PDMKSolver(cell, pos, n_digits=9, n_per_leaf=npl, precision="double")
+    assert rel(s.evaluate(q), g["ewald"]) < 1e-7
+    s6 = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision="float")
+    assert rel(s6.evaluate(q).astype(np.float64), g["ewald"]) < 1e-5
+
+
+def test_repeat_evaluate_is_deterministic(pdmk, cuda):
+    cell, pos, q, npl = case("cubic")
+    rng = np.random.default_rng(5)
+    q2 = rng.normal(size=q.shape)
+    for prec in ("double", "float"):
+        s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision=prec)
+        p1, p2, p1b = s.evaluate(q), s.evaluate(q2), s.evaluate(q)
+        np.testing.assert_array_equal(p1, p1b)
+        assert np.sum(np.abs(p1 - p2)) > 1e-6
+
+
+def test_device_entry_points(pdmk, cuda):
+    import ctypes as C
+    cell, pos, q, npl = case("cubic")
+    for prec in ("double", "float"):
+        s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision=prec)
+        host = s.evaluate(q)
+        qd = pdmk.DeviceArray.from_numpy(q)
+        out = pdmk.DeviceArray(s.n, s.dtype)
+        s.evaluate_device(qd.ptr, out.ptr, 0)
+        pdmk.synchronize()
+        np.testing.assert_array_equal(out.numpy(), host)
+        t = s.evaluate_timed(qd.ptr, out.ptr)
+        assert all(v >= 0 for v in t.values()) and sum(t.values()) > 0
+        np.testing.assert_array_equal(out.numpy(), host)
+        # a handle of the other precision is rejected
+        other = pdmk.lib().pdmk_gpu_evaluate_double if prec == "float" else pdmk.lib().pdmk_gpu_evaluate_float
+        assert other(s._h, C.c_void_p(qd.ptr), C.c_void_p(out.ptr), None) == -1
+
+
+def test_edge_inputs(pdmk, oracle_mod, cuda):
+    # one charge; points on the cell boundary; coincident points; non-neutral charges
+    s = pdmk.PDMKSolver(np.eye(3) * 2.0, np.array([[0.5, 0.5, 0.5]]), n_digits=6, n_per_leaf=20, precision="double")
+    assert np.all(np.isfinite(s.evaluate(np.array([1.0]))))
+    pos = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.25, 0.25, 0.25], [0.25, 0.25, 0.25], [0.999, 0.1, 0.0]])
+    q = np.array([1.0, -0.5, 0.25, 0.5, 0.3])
+    for prec, tol in (("double", 1e-12), ("float", 1e-5)):
+        s = pdmk.PDMKSolver(np.eye(3), pos, n_digits=6, n_per_leaf=2, precision=prec)
+        o = oracle_mod.OracleTree(np.eye(3), pos, 6, 2, 4)
+        assert rel(s.evaluate(q).astype(np.float64), o.evaluate(q)) < tol
+
+
+def test_wrong_charge_count_raises(pdmk, cuda):
+    cell, pos, q, npl = case("cubic")
+    s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision="double")
+    with pytest.raises(ValueError):
+        s.evaluate(q[:-1])
+
+
+def test_full_size_workload_parity_and_linearity(pdmk, oracle_mod, cuda):
+    """N = 1e6 cubic workload: FP64 path equals the oracle; the FP32 path is linear in q."""
+    n = 1_000_000
+    cell = W.cubic_cell()
+    pos = W.uniform_positions(n, 11, cell)
+    q1 = W.neutral_uniform_charges(n, 100)
+    q2 = W.neutral_uniform_charges(n, 101)
+    s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=300, precision="double")
+    o = oracle_mod.OracleTree(cell, pos, 6, 300, 4)
+    g = s.evaluate(q1)
+    assert rel(g, o.evaluate(q1)) < 1e-12
+    s.close()
+    o.close()
+    f = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=300, precision="float")
+    a, b = f.evaluate(q1).astype(np.float64), f.evaluate(q2).astype(np.float64)
+    ab = f.evaluate(0.5 * q1 - 2.0 * q2).astype(np.float64)
+    assert rel(ab, 0.5 * a - 2.0 * b) < 1e-5
+    assert rel(a, g) < 1e-5
