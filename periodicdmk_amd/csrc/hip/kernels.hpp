@@ -6,9 +6,9 @@
 //   k_charge2proxy ..... proxy::charge2proxycharge_3d (cpp/src/dmk/proxy.cpp:153-212)
 //   k_tensor3 .......... tensorprod::transform_3d c2p / p2c (cpp/src/dmk/tensorprod.cpp:39-75)
 //   k_proxy2pw ......... proxy::proxycharge2pw_3d (cpp/src/dmk/proxy.cpp:42-90)
-//   k_shift_pw2proxy ... form_eval_expansions shift + planewave_to_proxy_potential
-//                        (cpp/src/dmk/pbc_tree.cpp:1294-1369, cpp/include/dmk/tree.hpp:287-316,
-//                         cpp/src/dmk/planewave.cpp:49-105)
+//   k_pw_shift ......... form_eval_expansions shifts with fused radialft
+//                        (cpp/src/dmk/pbc_tree.cpp:1294-1369, cpp/include/dmk/tree.hpp:287-316)
+//   k_pw2proxy ......... planewave_to_proxy_potential (cpp/src/dmk/planewave.cpp:49-105)
 //   k_eval_targets ..... proxy::eval_targets_3d (cpp/src/dmk/proxy.cpp:307-448)
 //   k_near_field ....... evaluate_direct_interactions + laplace_3d_poly_all_pairs
 //                        (cpp/src/dmk/pbc_tree.cpp:1442-1681, cpp/include/dmk/vector_kernels.hpp:354-421)
@@ -55,36 +55,53 @@ __global__ void k_finalize(const R *__restrict__ pot_sorted, const double *__res
 }
 
 // ------------------------------------------------------------------ upward pass
-// One workgroup per destination box.  Thread t owns the (b, c) column
-// C[0..P-1, b, c] of the proxy coefficients and accumulates
-//   C[a,b,c] = sum_s T_a(x_s) T_b(y_s) T_c(z_s) q_s,  x_s = sc (r_s - centre).
-// Sources hold coordinates relative to their leaf centre; each range lies in one
-// leaf and carries rng_off = leaf centre - box centre, so x_s = sc (local_s + off).
+// One workgroup per destination box:
+//   C[c][b][a] = sum_s T_a(x_s) T_b(y_s) T_c(z_s) q_s,  x_s = sc (r_s - centre).
+// Sources hold coordinates relative to their leaf centre; each range lies in one leaf and
+// carries rng_off = leaf centre - box centre, so x_s = sc (local_s + off).  Per tile of
+// C2P_TS sources the Chebyshev values are tabulated in LDS (q folded into T_c); thread
+// (c, bb) then owns the RB x P output block C[c][bb RB .. bb RB + RB - 1][0 .. P-1] in
+// registers, so each source costs RB P FMAs per thread against ceil(P/4) + 2 LDS reads.
+constexpr int C2P_TS = 64;
+
+// register block RB (b values per thread) and workgroup size per order; output blocks beyond
+// one workgroup's threads are processed in further passes over the sources
+template <int P> struct C2pCfg {
+    static constexpr int RB = P >= 28 ? 2 : 3;
+    static constexpr int NB = (P + RB - 1) / RB;
+    static constexpr int cols = NB * P;
+    static constexpr int threads = P >= 38 ? 256 : (cols + 63) / 64 * 64;
+};
+
 template <class R, int P>
-__global__ __launch_bounds__(1024) void k_charge2proxy(const P4<R> *__restrict__ src, const int *__restrict__ grp_slot,
-                                                        const R *__restrict__ grp_geom, const int *__restrict__ grp_rng,
-                                                        const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
-                                                        const R *__restrict__ rng_off, R *__restrict__ proxy) {
-    constexpr int TS = 32;
-    __shared__ R Tx[TS * P], Ty[TS * P], Tz[TS * P];
+__global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
+    const P4<R> *__restrict__ src, const int *__restrict__ grp_slot, const R *__restrict__ grp_geom,
+    const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
+    const R *__restrict__ rng_off, R *__restrict__ proxy) {
+    constexpr int RB = C2pCfg<P>::RB, NB = C2pCfg<P>::NB, NT = C2pCfg<P>::threads;
+    constexpr int PX = (P + 3) & ~3, PY = NB * RB;  // padded table rows
+    __shared__ R Tx[C2P_TS * PX], Ty[C2P_TS * PY], Tz[C2P_TS * P];
     const int g = blockIdx.x;
     const R sc = grp_geom[4 * g + 3];
-    R *out = proxy + (size_t)grp_slot[g] * (P * P * P);
-    for (int pc0 = 0; pc0 < P * P; pc0 += blockDim.x) {
-        const int pc = pc0 + threadIdx.x;
-        const bool active = pc < P * P;
-        const int b = active ? pc % P : 0, c = active ? pc / P : 0;
-        R acc[P];
+    for (int col0 = 0; col0 < NB * P; col0 += NT) {
+        const int col = col0 + (int)threadIdx.x;
+        const int c = col / NB, bb = col - c * NB;
+        const bool active = col < NB * P;
+        R acc[RB][P];
 #pragma unroll
-        for (int a = 0; a < P; ++a) acc[a] = R(0);
+        for (int j = 0; j < RB; ++j)
+#pragma unroll
+            for (int a = 0; a < P; ++a) acc[j][a] = R(0);
         for (int r = grp_rng[g]; r < grp_rng[g + 1]; ++r) {
             const int beg = rng_begin[r], cnt = rng_count[r];
-            for (int s0 = 0; s0 < cnt; s0 += TS) {
+            for (int s0 = 0; s0 < cnt; s0 += C2P_TS) {
+                const int ns = min(C2P_TS, cnt - s0);
                 __syncthreads();
-                for (int t = threadIdx.x; t < 3 * TS; t += blockDim.x) {
-                    const int s = t % TS, d = t / TS;
-                    R *T = (d == 0 ? Tx : (d == 1 ? Ty : Tz)) + s * P;
-                    if (s0 + s < cnt) {
+                for (int t = threadIdx.x; t < 3 * C2P_TS; t += NT) {
+                    const int s = t % C2P_TS, d = t / C2P_TS;
+                    R *T = d == 0 ? Tx + s * PX : (d == 1 ? Ty + s * PY : Tz + s * P);
+                    const int len = d == 0 ? PX : (d == 1 ? PY : P);
+                    if (s < ns) {
                         const P4<R> pt = src[beg + s0 + s];
                         const R x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) + rng_off[4 * r + d]);
                         const R m = (d == 2) ? pt.w : R(1);
@@ -97,189 +114,325 @@ __global__ __launch_bounds__(1024) void k_charge2proxy(const P4<R> *__restrict__
                             t0 = t1;
                             t1 = t2;
                         }
+                        for (int k = P; k < len; ++k) T[k] = R(0);
                     } else {
-                        for (int k = 0; k < P; ++k) T[k] = R(0);
+                        for (int k = 0; k < len; ++k) T[k] = R(0);
                     }
                 }
                 __syncthreads();
                 if (active) {
-                    const int ns = min(TS, cnt - s0);
                     for (int s = 0; s < ns; ++s) {
-                        const R w = Ty[s * P + b] * Tz[s * P + c];
+                        R tx[P];
 #pragma unroll
-                        for (int a = 0; a < P; ++a) acc[a] = fma(Tx[s * P + a], w, acc[a]);
+                        for (int a = 0; a < P; ++a) tx[a] = Tx[s * PX + a];
+                        const R z = Tz[s * P + c];
+#pragma unroll
+                        for (int j = 0; j < RB; ++j) {
+                            const R w = Ty[s * PY + bb * RB + j] * z;
+#pragma unroll
+                            for (int a = 0; a < P; ++a) acc[j][a] = fma(tx[a], w, acc[j][a]);
+                        }
                     }
                 }
             }
         }
-        if (active)
+        if (active) {
+            R *out = proxy + (size_t)grp_slot[g] * (P * P * P) + (size_t)P * P * c;
 #pragma unroll
-            for (int a = 0; a < P; ++a) out[a + P * b + P * P * c] = acc[a];
+            for (int j = 0; j < RB; ++j) {
+                const int b = bb * RB + j;
+                if (b < P)
+#pragma unroll
+                    for (int a = 0; a < P; ++a) out[a + P * b] = acc[j][a];
+            }
+        }
     }
 }
 
-// 3-mode tensor transform out(a',b',c') (+)= sum U0(a',a) U1(b',b) U2(c',c) in(a,b,c), with
-// U_d the minus/plus parent<->child matrix selected by bit d of the child code (transposed for
-// child -> parent).  Item i writes out slot item_out[i]; its inputs are the CSR range
-// item_ptr[i]..item_ptr[i+1] of (in slot, child code); item_add[i] = accumulate into the output.
+// ------------------------------------------------------------------ tensor transforms
+// All transforms are sequences of mode products of a small 3-D tensor with small dense
+// matrices.  One workgroup owns one output box; a thread owns one column of the mode being
+// contracted (the column sits in registers), the matrix entries are read with wave-uniform
+// addresses (scalar loads), and intermediates live in LDS with row strides of an odd number
+// of elements, so 64-bit LDS accesses by consecutive lanes hit distinct banks.  When the
+// intermediates of a whole box do not fit the LDS budget the box is processed in slices of
+// the first contraction's output index (upward, c2p/p2c) or of the first output index of the
+// downward contraction; each slice then runs all three steps on its own.
+constexpr int kTfLdsBytes = 64 * 1024;
+
+template <int N> struct OddStride { static constexpr int v = (N % 2) ? N : N + 1; };
+
+// slices per pass so that `per_slice` bytes x slices stays within the LDS budget
+constexpr int tf_slices(int total, int per_slice) {
+    const int k = kTfLdsBytes / per_slice;
+    return k < 1 ? 1 : (k > total ? total : k);
+}
+
+// 3-mode tensor transform out(a',b',c') (+)= sum_children U0(a',a) U1(b',b) U2(c',c) in(a,b,c)
+// (tensor layout [c][b][a], a fastest).  U_d is the minus/plus parent<->child matrix selected
+// by bit d of the child code, transposed for child -> parent.  Item i writes out slot
+// item_out[i] from the CSR range item_ptr[i]..item_ptr[i+1] of (in slot, child code);
+// item_add[i] = accumulate into the existing output.
 template <class R, int P>
 __global__ __launch_bounds__(256) void k_tensor3(const R *__restrict__ in_base, R *__restrict__ out_base,
                                                   const int *__restrict__ item_out, const int *__restrict__ item_add,
                                                   const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
                                                   const int *__restrict__ in_code, const R *__restrict__ mats,
-                                                  int transpose, int n_items, R *__restrict__ scratch) {
-    constexpr int P3 = P * P * P;
-    __shared__ R U[3][P * P];
-    R *t1 = scratch + (size_t)blockIdx.x * 2 * P3;
-    R *t2 = t1 + P3;
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-        R *out = out_base + (size_t)item_out[it] * P3;
-        bool add = item_add[it] != 0;
-        for (int e = item_ptr[it]; e < item_ptr[it + 1]; ++e) {
-            const R *in = in_base + (size_t)in_slot[e] * P3;
+                                                  int transpose, int n_items) {
+    constexpr int PP = P * P, TS = OddStride<P>::v;
+    constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R));
+    __shared__ R T1[KC * PP];       // [k][b][a]
+    __shared__ R T2[KC * P * TS];   // [k][b'][a] rows of TS
+    const int it = blockIdx.x;
+    if (it >= n_items) return;
+    R *out = out_base + (size_t)item_out[it] * P * PP;
+    const bool add = item_add[it] != 0;
+    const int e0 = item_ptr[it], e1 = item_ptr[it + 1];
+    // U_d(r, c) = M_s[r + P c] (parent -> child) or M_s[c + P r] (child -> parent)
+    const int sr = transpose ? P : 1, sc = transpose ? 1 : P;
+    for (int k0 = 0; k0 < P; k0 += KC) {
+        const int nk = min(KC, P - k0);
+        // x-step accumulators: thread column (k, b'), outputs a' -- persistent over children
+        constexpr int XC = (KC * P + 255) / 256;
+        R acc[XC][P];
+#pragma unroll
+        for (int q = 0; q < XC; ++q)
+#pragma unroll
+            for (int a = 0; a < P; ++a) acc[q][a] = R(0);
+        for (int e = e0; e < e1; ++e) {
+            const R *in = in_base + (size_t)in_slot[e] * P * PP;
             const int code = in_code[e];
+            const R *M0 = mats + (code & 1) * PP, *M1 = mats + ((code >> 1) & 1) * PP, *M2 = mats + ((code >> 2) & 1) * PP;
             __syncthreads();
-            for (int t = threadIdx.x; t < 3 * P * P; t += blockDim.x) {
-                const int d = t / (P * P), rc = t % (P * P), r = rc % P, c = rc / P;
-                const R *M = mats + ((code >> d) & 1) * P * P;
-                U[d][r + P * c] = transpose ? M[c + P * r] : M[r + P * c];
+            // z: T1[k][b][a] = sum_c U2(k0 + k, c) in[c][b][a]
+            for (int j = threadIdx.x; j < PP; j += blockDim.x) {
+                R x[P];
+#pragma unroll
+                for (int c = 0; c < P; ++c) x[c] = in[c * PP + j];
+                for (int k = 0; k < nk; ++k) {
+                    R s = 0;
+#pragma unroll
+                    for (int c = 0; c < P; ++c) s = fma(M2[(k0 + k) * sr + c * sc], x[c], s);
+                    T1[k * PP + j] = s;
+                }
             }
             __syncthreads();
-            for (int o = threadIdx.x; o < P3; o += blockDim.x) {  // z mode
-                const int ab = o % (P * P), c2 = o / (P * P);
-                R s = 0;
-                for (int c = 0; c < P; ++c) s = fma(U[2][c2 + P * c], in[ab + P * P * c], s);
-                t1[o] = s;
+            // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]
+            for (int w = threadIdx.x; w < nk * P; w += blockDim.x) {
+                const int k = w / P, a = w - k * P;
+                R x[P];
+#pragma unroll
+                for (int b = 0; b < P; ++b) x[b] = T1[k * PP + b * P + a];
+                for (int b2 = 0; b2 < P; ++b2) {
+                    R s = 0;
+#pragma unroll
+                    for (int b = 0; b < P; ++b) s = fma(M1[b2 * sr + b * sc], x[b], s);
+                    T2[(k * P + b2) * TS + a] = s;
+                }
             }
             __syncthreads();
-            for (int o = threadIdx.x; o < P3; o += blockDim.x) {  // y mode
-                const int a = o % P, b2 = (o / P) % P, c2 = o / (P * P);
-                R s = 0;
-                for (int b = 0; b < P; ++b) s = fma(U[1][b2 + P * b], t1[a + P * b + P * P * c2], s);
-                t2[o] = s;
+            // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a], column (k, b')
+#pragma unroll
+            for (int q = 0; q < XC; ++q) {
+                const int w = threadIdx.x + q * 256;
+                if (w < nk * P) {
+                    R x[P];
+#pragma unroll
+                    for (int a = 0; a < P; ++a) x[a] = T2[w * TS + a];
+#pragma unroll
+                    for (int a2 = 0; a2 < P; ++a2) {
+                        R s = acc[q][a2];
+#pragma unroll
+                        for (int a = 0; a < P; ++a) s = fma(M0[a2 * sr + a * sc], x[a], s);
+                        acc[q][a2] = s;
+                    }
+                }
             }
-            __syncthreads();
-            for (int o = threadIdx.x; o < P3; o += blockDim.x) {  // x mode
-                const int a2 = o % P, bc = o / P;
-                R s = 0;
-                for (int a = 0; a < P; ++a) s = fma(U[0][a2 + P * a], t2[a + P * bc], s);
-                out[o] = add ? out[o] + s : s;
+        }
+#pragma unroll
+        for (int q = 0; q < XC; ++q) {
+            const int w = threadIdx.x + q * 256;
+            if (w < nk * P) {
+                R *dst = out + (size_t)k0 * PP + (size_t)w * P;  // [c' = k0 + k][b'][a']
+#pragma unroll
+                for (int a2 = 0; a2 < P; ++a2) dst[a2] = add ? dst[a2] + acc[q][a2] : acc[q][a2];
             }
-            add = true;
         }
     }
 }
 
 // ------------------------------------------------------------------ plane waves
-// pw(k1,k2,k3) = sum P(k1,m1) P(k2,m2) P(k3,m3) C(m1,m2,m3), k3 < H  (P = poly2pw, n_pw x p)
-template <class R, int P>
+// Plane-wave mode layout: m = k1 * (H n) + k3 * n + k2 ([x][z][y], y fastest), k3 < H.
+// Outgoing expansion pw(k1,k2,k3) = sum W(k1,a) W(k2,b) W(k3,c) C[c][b][a], W = poly2pw
+// (W(k, j) at W[k + n j]); one workgroup per box, sliced over k3.
+template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
-                                                   const Cx<R> *__restrict__ poly2pw, int n_pw, int n_boxes,
-                                                   Cx<R> *__restrict__ scratch) {
-    const int H = (n_pw + 1) / 2, nm = n_pw * n_pw * H;
-    Cx<R> *t1 = scratch + (size_t)blockIdx.x * (P * P * H + P * n_pw * H);
-    Cx<R> *t2 = t1 + P * P * H;
-    for (int bx = blockIdx.x; bx < n_boxes; bx += gridDim.x) {
-        const R *C = proxy + (size_t)bx * P * P * P;
-        Cx<R> *pw = pw_out + (size_t)bx * nm;
-        __syncthreads();
-        for (int o = threadIdx.x; o < P * P * H; o += blockDim.x) {
-            const int m12 = o % (P * P), k3 = o / (P * P);
-            R sr = 0, si = 0;
-            for (int m3 = 0; m3 < P; ++m3) {
-                const Cx<R> w = poly2pw[k3 + n_pw * m3];
-                const R c = C[m12 + P * P * m3];
-                sr = fma(w.re, c, sr);
-                si = fma(w.im, c, si);
+                                                   const Cx<R> *__restrict__ W, int n_boxes) {
+    constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, TS = OddStride<P>::v;
+    constexpr int K3 = tf_slices(H, (PP + NPW * TS) * (int)sizeof(Cx<R>));
+    __shared__ Cx<R> T1[K3 * PP];        // [k][b][a]
+    __shared__ Cx<R> T2[K3 * NPW * TS];  // [k][k2][a] rows of TS
+    const int bx = blockIdx.x;
+    if (bx >= n_boxes) return;
+    const R *C = proxy + (size_t)bx * P * PP;
+    Cx<R> *pw = pw_out + (size_t)bx * NPW * COL;
+    for (int k30 = 0; k30 < H; k30 += K3) {
+        const int nk = min(K3, H - k30);
+        // z: T1[k][b][a] = sum_c W(k30 + k, c) C[c][b][a]
+        for (int j = threadIdx.x; j < PP; j += blockDim.x) {
+            R x[P];
+#pragma unroll
+            for (int c = 0; c < P; ++c) x[c] = C[c * PP + j];
+            for (int k = 0; k < nk; ++k) {
+                R sr = 0, si = 0;
+#pragma unroll
+                for (int c = 0; c < P; ++c) {
+                    const Cx<R> w = W[k30 + k + NPW * c];
+                    sr = fma(w.re, x[c], sr);
+                    si = fma(w.im, x[c], si);
+                }
+                T1[k * PP + j] = {sr, si};
             }
-            t1[o] = {sr, si};
         }
         __syncthreads();
-        for (int o = threadIdx.x; o < P * n_pw * H; o += blockDim.x) {
-            const int m1 = o % P, k2 = (o / P) % n_pw, k3 = o / (P * n_pw);
-            R sr = 0, si = 0;
-            for (int m2 = 0; m2 < P; ++m2) {
-                const Cx<R> w = poly2pw[k2 + n_pw * m2], v = t1[m1 + P * m2 + P * P * k3];
-                sr = fma(w.re, v.re, fma(-w.im, v.im, sr));
-                si = fma(w.re, v.im, fma(w.im, v.re, si));
+        // y: T2[k][k2][a] = sum_b W(k2, b) T1[k][b][a]
+        for (int w = threadIdx.x; w < nk * P; w += blockDim.x) {
+            const int k = w / P, a = w - k * P;
+            Cx<R> v[P];
+#pragma unroll
+            for (int b = 0; b < P; ++b) v[b] = T1[k * PP + b * P + a];
+            for (int k2 = 0; k2 < NPW; ++k2) {
+                R sr = 0, si = 0;
+#pragma unroll
+                for (int b = 0; b < P; ++b) {
+                    const Cx<R> m = W[k2 + NPW * b];
+                    sr = fma(m.re, v[b].re, fma(-m.im, v[b].im, sr));
+                    si = fma(m.re, v[b].im, fma(m.im, v[b].re, si));
+                }
+                T2[(k * NPW + k2) * TS + a] = {sr, si};
             }
-            t2[o] = {sr, si};
         }
         __syncthreads();
-        for (int o = threadIdx.x; o < nm; o += blockDim.x) {
-            const int k1 = o % n_pw, k23 = o / n_pw;
-            R sr = 0, si = 0;
-            for (int m1 = 0; m1 < P; ++m1) {
-                const Cx<R> w = poly2pw[k1 + n_pw * m1], v = t2[m1 + P * k23];
-                sr = fma(w.re, v.re, fma(-w.im, v.im, sr));
-                si = fma(w.re, v.im, fma(w.im, v.re, si));
+        // x: pw[k1][k30 + k][k2] = sum_a W(k1, a) T2[k][k2][a], column w = k n + k2
+        for (int w = threadIdx.x; w < nk * NPW; w += blockDim.x) {
+            Cx<R> v[P];
+#pragma unroll
+            for (int a = 0; a < P; ++a) v[a] = T2[w * TS + a];
+            Cx<R> *dst = pw + (size_t)k30 * NPW + w;
+            for (int k1 = 0; k1 < NPW; ++k1) {
+                R sr = 0, si = 0;
+#pragma unroll
+                for (int a = 0; a < P; ++a) {
+                    const Cx<R> m = W[k1 + NPW * a];
+                    sr = fma(m.re, v[a].re, fma(-m.im, v[a].im, sr));
+                    si = fma(m.re, v[a].im, fma(m.im, v[a].re, si));
+                }
+                dst[(size_t)k1 * COL] = {sr, si};
             }
-            pw[o] = {sr, si};
         }
+        __syncthreads();
     }
 }
 
-// For each box of one level: pw_in = sum over (neighbour, stencil) of pw_out[nbr] * shift[stencil] * lscale
-// (the self entry has a real shift), then
-//   down(k1,k2,k3) (+)= 2 Re sum W(m1,k1) W(m2,k2) h(m3) W(m3,k3) pw_in(m1,m2,m3), h = 1/2 for m3 >= n_pw/2.
-template <class R, int P>
-__global__ __launch_bounds__(256) void k_shift_pw2proxy(const Cx<R> *__restrict__ pw_out, const Cx<R> *__restrict__ shift0,
-                                                         R lscale, const int *__restrict__ box_down,
-                                                         const int *__restrict__ box_zero, const int *__restrict__ nbr_ptr,
-                                                         const int *__restrict__ nbr_slot, const int *__restrict__ nbr_stencil,
-                                                         const Cx<R> *__restrict__ pw2poly, int n_pw, int n_items,
-                                                         R *__restrict__ down_base, Cx<R> *__restrict__ scratch) {
-    const int H = (n_pw + 1) / 2, nm = n_pw * n_pw * H, half = n_pw / 2;
-    Cx<R> *pin = scratch + (size_t)blockIdx.x * (nm + P * n_pw * H + P * P * H);
-    Cx<R> *t1 = pin + nm;
-    Cx<R> *t2 = t1 + P * n_pw * H;
-    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-        __syncthreads();
-        for (int m = threadIdx.x; m < nm; m += blockDim.x) {
-            R sr = 0, si = 0;
-            for (int e = nbr_ptr[it]; e < nbr_ptr[it + 1]; ++e) {
-                const Cx<R> a = pw_out[(size_t)nbr_slot[e] * nm + m];
-                const Cx<R> s = shift0[(size_t)nbr_stencil[e] * nm + m];
-                sr = fma(a.re, s.re, fma(-a.im, s.im, sr));
-                si = fma(a.re, s.im, fma(a.im, s.re, si));
+// Incoming expansion of one level: pin[i][m] = lscale sum_e pw_out[slot_e][m] shift0[sten_e][m]
+// over box i's CSR range of (neighbour slot, stencil); shift0 holds the level-0 translation
+// times the radial Fourier factor, lscale = 2^level scales the latter to this level.
+template <class R>
+__global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_out, const Cx<R> *__restrict__ shift0,
+                                                   R lscale, const int *__restrict__ nbr_ptr,
+                                                   const int *__restrict__ nbr_slot, const int *__restrict__ nbr_sten,
+                                                   int nm, int item0, Cx<R> *__restrict__ pin) {
+    const int it = item0 + blockIdx.y;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    R sr = 0, si = 0;
+    for (int e = nbr_ptr[it]; e < nbr_ptr[it + 1]; ++e) {
+        const Cx<R> a = pw_out[(size_t)nbr_slot[e] * nm + m];
+        const Cx<R> s = shift0[(size_t)nbr_sten[e] * nm + m];
+        sr = fma(a.re, s.re, fma(-a.im, s.im, sr));
+        si = fma(a.re, s.im, fma(a.im, s.re, si));
+    }
+    pin[(size_t)blockIdx.y * nm + m] = {sr * lscale, si * lscale};
+}
+
+// down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
+// (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  One
+// workgroup per box, sliced over a.
+template <class R, int P, int NPW>
+__global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
+                                                   const int *__restrict__ box_down, const int *__restrict__ box_zero,
+                                                   int item0, int n_items, R *__restrict__ down_base) {
+    constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, half = NPW / 2, TS2 = OddStride<P>::v;
+    constexpr int AC = tf_slices(P, (COL + H * TS2) * (int)sizeof(Cx<R>));
+    __shared__ Cx<R> U1[AC * COL];       // [a][k3][k2]
+    __shared__ Cx<R> U2[AC * H * TS2];   // [a][k3][b] rows of TS2
+    const int it = blockIdx.x;
+    if (it >= n_items) return;
+    const Cx<R> *X = pin + (size_t)it * NPW * COL;
+    R *down = down_base + (size_t)box_down[item0 + it] * P * PP;
+    const bool zero = box_zero[item0 + it] != 0;
+    for (int a0 = 0; a0 < P; a0 += AC) {
+        const int na = min(AC, P - a0);
+        // x: U1[a][k3][k2] = sum_k1 V(k1, a0 + a) pin[k1][k3][k2], column = (k3, k2)
+        for (int col = threadIdx.x; col < COL; col += blockDim.x) {
+            Cx<R> acc[AC];
+#pragma unroll
+            for (int a = 0; a < AC; ++a) acc[a] = {R(0), R(0)};
+            for (int k1 = 0; k1 < NPW; ++k1) {
+                const Cx<R> x = X[(size_t)k1 * COL + col];
+#pragma unroll
+                for (int a = 0; a < AC; ++a) {
+                    if (a < na) {
+                        const Cx<R> m = V[k1 + NPW * (a0 + a)];
+                        acc[a].re = fma(m.re, x.re, fma(-m.im, x.im, acc[a].re));
+                        acc[a].im = fma(m.re, x.im, fma(m.im, x.re, acc[a].im));
+                    }
+                }
             }
-            pin[m] = {sr * lscale, si * lscale};
+#pragma unroll
+            for (int a = 0; a < AC; ++a)
+                if (a < na) U1[a * COL + col] = acc[a];
         }
         __syncthreads();
-        for (int o = threadIdx.x; o < P * n_pw * H; o += blockDim.x) {  // contract m1 -> k1
-            const int k1 = o % P, m2 = (o / P) % n_pw, m3 = o / (P * n_pw);
-            R sr = 0, si = 0;
-            for (int m1 = 0; m1 < n_pw; ++m1) {
-                const Cx<R> w = pw2poly[m1 + n_pw * k1], v = pin[m1 + n_pw * m2 + n_pw * n_pw * m3];
-                sr = fma(w.re, v.re, fma(-w.im, v.im, sr));
-                si = fma(w.re, v.im, fma(w.im, v.re, si));
+        // y: U2[a][k3][b] = h(k3) sum_k2 V(k2, b) U1[a][k3][k2], column w = a H + k3
+        for (int w = threadIdx.x; w < na * H; w += blockDim.x) {
+            const int k3 = w % H;
+            Cx<R> acc[P];
+#pragma unroll
+            for (int b = 0; b < P; ++b) acc[b] = {R(0), R(0)};
+            for (int k2 = 0; k2 < NPW; ++k2) {
+                const Cx<R> x = U1[w * NPW + k2];
+#pragma unroll
+                for (int b = 0; b < P; ++b) {
+                    const Cx<R> m = V[k2 + NPW * b];
+                    acc[b].re = fma(m.re, x.re, fma(-m.im, x.im, acc[b].re));
+                    acc[b].im = fma(m.re, x.im, fma(m.im, x.re, acc[b].im));
+                }
             }
-            t1[o] = {sr, si};
+            const R hw = (k3 >= half) ? R(0.5) : R(1);
+#pragma unroll
+            for (int b = 0; b < P; ++b) U2[w * TS2 + b] = {acc[b].re * hw, acc[b].im * hw};
         }
         __syncthreads();
-        for (int o = threadIdx.x; o < P * P * H; o += blockDim.x) {  // contract m2 -> k2, half weight
-            const int k1 = o % P, k2 = (o / P) % P, m3 = o / (P * P);
-            R sr = 0, si = 0;
-            for (int m2 = 0; m2 < n_pw; ++m2) {
-                const Cx<R> w = pw2poly[m2 + n_pw * k2], v = t1[k1 + P * m2 + P * n_pw * m3];
-                sr = fma(w.re, v.re, fma(-w.im, v.im, sr));
-                si = fma(w.re, v.im, fma(w.im, v.re, si));
+        // z: down[c][b][a0 + a] (+)= 2 Re sum_k3 V(k3, c) U2[a][k3][b], column w = b na + a
+        for (int w = threadIdx.x; w < na * P; w += blockDim.x) {
+            const int b = w / na, a = w - b * na;
+            R acc[P];
+#pragma unroll
+            for (int c = 0; c < P; ++c) acc[c] = R(0);
+            for (int k3 = 0; k3 < H; ++k3) {
+                const Cx<R> x = U2[(a * H + k3) * TS2 + b];
+#pragma unroll
+                for (int c = 0; c < P; ++c) {
+                    const Cx<R> m = V[k3 + NPW * c];
+                    acc[c] = fma(m.re, x.re, fma(-m.im, x.im, acc[c]));
+                }
             }
-            const R hw = (m3 >= half) ? R(0.5) : R(1);
-            t2[o] = {sr * hw, si * hw};
+            R *dst = down + (size_t)b * P + a0 + a;
+#pragma unroll
+            for (int c = 0; c < P; ++c) dst[(size_t)c * PP] = (zero ? R(0) : dst[(size_t)c * PP]) + R(2) * acc[c];
         }
         __syncthreads();
-        R *down = down_base + (size_t)box_down[it] * P * P * P;
-        const bool zero = box_zero[it] != 0;
-        for (int o = threadIdx.x; o < P * P * P; o += blockDim.x) {  // contract m3 -> k3, 2 Re
-            const int k12 = o % (P * P), k3 = o / (P * P);
-            R s = 0;
-            for (int m3 = 0; m3 < H; ++m3) {
-                const Cx<R> w = pw2poly[m3 + n_pw * k3], v = t2[k12 + P * P * m3];
-                s = fma(v.re, w.re, fma(-v.im, w.im, s));
-            }
-            down[o] = (zero ? R(0) : down[o]) + R(2) * s;
-        }
     }
 }
 
@@ -321,85 +474,122 @@ __global__ __launch_bounds__(256) void k_eval_targets(const R *__restrict__ down
 }
 
 // ------------------------------------------------------------------ near field
-// One workgroup per target leaf; each thread owns one target (loop if more).  For
-// every source box in list1, sources are staged through LDS and the residual
-// u = P((r - bs/2) 2/bs) / r, 1e-30 < r^2 < bs^2, is accumulated.  For fewer than 6
-// digits P is used in r (even/odd split), as in laplace_3d_poly_all_pairs.
-// Coordinates are relative to each point's leaf centre; l1_geom[e] = (centre of the
-// target leaf - centre of the source leaf, bs), so r = |(t + off) - s|.  Each list entry
-// is summed in R and the per-target total in FP64.
+// Residual u = P((r - bs/2) 2/bs) / r for 1e-30 < r^2 < bs^2, summed over the list1
+// sources of every target.  For fewer than 6 digits P is applied in r^2 with an even/odd
+// split (monomial coefficients per bs, prepared on the host), as laplace_3d_poly_all_pairs
+// does.
+//
+// Work item = one wavefront = up to 64 targets of one leaf (consecutive in the leaf's
+// Morton order, so the item is spatially compact; wbox = its bounding box).  Sources of
+// every leaf are cut into chunks of NF_CHUNK consecutive (Morton-ordered) points with
+// bounding boxes cbox.  For each list1 entry the 64 lanes test 64 chunks at once against
+// the item's box (shifted into the source leaf's frame) and only chunks that can hold an
+// in-support pair are walked; the test is evaluated with the same rounded operations as
+// the per-pair r^2, so it never drops a pair the per-pair test would keep.  Sources are
+// read with wave-uniform addresses; the polynomial is skipped when no lane of the wave is
+// in support for a source.
+//
+// Coordinates are relative to each point's leaf centre; geom[e] = (centre of the target
+// leaf - centre of the source leaf, bs), so r = |(t + off) - s|.  Each list entry is
+// summed in R and the per-target total in FP64.
+constexpr int NF_CHUNK = 8;
+constexpr int NF_SPLIT_STRIDE = 16;  // coefficient stride of the even/odd table
+
+struct NfEntry { int sbeg, scnt, cbase, cfidx; };
+
 template <class R, int NC, bool SPLIT>
 __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
-                                                     const int *__restrict__ leaf_trg, const int *__restrict__ l1_ptr,
-                                                     const int *__restrict__ l1_src, const P4<R> *__restrict__ l1_geom,
-                                                     const R *__restrict__ coeffs, R *__restrict__ pot_sorted) {
-    constexpr int TILE = 256;
-    __shared__ P4<R> tile[TILE];
-    __shared__ R cf[64];
+                                                     const int4 *__restrict__ items, const P4<R> *__restrict__ wbox,
+                                                     const int *__restrict__ l1_ptr, const NfEntry *__restrict__ l1,
+                                                     const P4<R> *__restrict__ geom, const P4<R> *__restrict__ cbox,
+                                                     const R *__restrict__ coeffs, R *__restrict__ pot_sorted, int n_items) {
+    // wave index made wave-uniform explicitly, so everything derived from it lives in SGPRs
+    const int item = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= n_items) return;  // whole wavefront
+    const int lane = threadIdx.x & 63;
+    const int4 it = items[item];  // (first target, count, target-leaf list index, -)
+    const bool active = lane < it.y;
+    const P4<R> me = active ? trg[it.x + lane] : P4<R>{R(1e18), R(1e18), R(1e18), R(0)};
+    const P4<R> blo = wbox[2 * item], bhi = wbox[2 * item + 1];
     R c[NC];
+    if (!SPLIT) {
 #pragma unroll
-    for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
-    const int lf = blockIdx.x;
-    const int tbeg = leaf_trg[2 * lf], tcnt = leaf_trg[2 * lf + 1];
-    for (int rep = 0; rep * blockDim.x < tcnt; ++rep) {
-        const int t = rep * blockDim.x + threadIdx.x;
-        const bool active = t < tcnt;
-        const P4<R> me = active ? trg[tbeg + t] : P4<R>{R(0), R(0), R(0), R(0)};
-        double acc = 0;
-        for (int e = l1_ptr[lf]; e < l1_ptr[lf + 1]; ++e) {
-            const int sbeg = l1_src[2 * e], scnt = l1_src[2 * e + 1];
-            const P4<R> gm = l1_geom[e];
-            const R bs = gm.w;
-            const R tx = me.x + gm.x, ty = me.y + gm.y, tz = me.z + gm.z;
-            const R d2max = bs * bs, rsc = R(2) / bs, cen = -bs / R(2);
-            __syncthreads();
-            if (SPLIT && threadIdx.x == 0) {
-                // monomial coefficients in r: sum c_i rsc^i (r + cen)^i
-                double cd[NC];
-                double rp = 1;
-                for (int i = 0; i < NC; ++i) { cd[i] = (double)coeffs[i] * rp; rp *= (double)rsc; }
-                for (int i = 0; i < NC; ++i)
-                    for (int j = NC - 1; j > i; --j) cd[j - 1] += (double)cen * cd[j];
-                for (int i = 0; i < NC; ++i) cf[i] = (R)cd[i];
+        for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    }
+    double acc = 0;
+    for (int e = l1_ptr[it.z]; e < l1_ptr[it.z + 1]; ++e) {
+        const NfEntry E = l1[e];
+        const P4<R> gm = geom[e];
+        const R bs = gm.w, d2max = bs * bs, rsc = R(2) / bs, cen = -bs / R(2);
+        const R tx = me.x + gm.x, ty = me.y + gm.y, tz = me.z + gm.z;
+        const R lx = blo.x + gm.x, ly = blo.y + gm.y, lz = blo.z + gm.z;
+        const R hx = bhi.x + gm.x, hy = bhi.y + gm.y, hz = bhi.z + gm.z;
+        if (SPLIT) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) c[i] = coeffs[E.cfidx * NF_SPLIT_STRIDE + i];
+        }
+        const int nch = (E.scnt + NF_CHUNK - 1) / NF_CHUNK;
+        R ea = 0;
+        for (int c0 = 0; c0 < nch; c0 += 64) {
+            bool keep = false;
+            if (c0 + lane < nch) {
+                const P4<R> cl = cbox[2 * (E.cbase + c0 + lane)], ch = cbox[2 * (E.cbase + c0 + lane) + 1];
+                const R dx = fmax(fmax(cl.x - hx, lx - ch.x), R(0));
+                const R dy = fmax(fmax(cl.y - hy, ly - ch.y), R(0));
+                const R dz = fmax(fmax(cl.z - hz, lz - ch.z), R(0));
+                keep = fma(dx, dx, fma(dy, dy, dz * dz)) < d2max;
             }
-            R ea = 0;
-            for (int s0 = 0; s0 < scnt; s0 += TILE) {
-                const int ns = min(TILE, scnt - s0);
-                __syncthreads();
-                if ((int)threadIdx.x < ns) tile[threadIdx.x] = src[sbeg + s0 + threadIdx.x];
-                __syncthreads();
-                if (active) {
-                    for (int s = 0; s < ns; ++s) {
-                        const P4<R> q = tile[s];
+            unsigned long long mask = __ballot(keep);
+            while (mask) {
+                const int k = __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                const int s0 = E.sbeg + (c0 + k) * NF_CHUNK;
+                const int ns = min(NF_CHUNK, E.sbeg + E.scnt - s0);
+                // the source array is padded by NF_CHUNK points, so a whole chunk is always
+                // readable; points past the leaf's range are masked by j < ns
+                P4<R> qs[NF_CHUNK];
+#pragma unroll
+                for (int j = 0; j < NF_CHUNK; ++j) qs[j] = src[s0 + j];
+#pragma unroll
+                for (int j = 0; j < NF_CHUNK; ++j) {
+                    {
+                        const P4<R> q = qs[j];
                         const R dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
                         const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        const R rinv = rsqrt_full<R>(r2);
-                        R u;
-                        if (SPLIT) {
-                            constexpr int SE = (NC - 1) & ~1, SO = ((NC - 2) & ~1) + 1;
-                            R E = cf[SE];
+                        if (j < ns && r2 < d2max && r2 > R(1e-30)) {
+                            const R rinv = rsqrt_full<R>(r2);
+                            R u;
+                            if (SPLIT) {
+                                constexpr int SE = (NC - 1) & ~1, SO = ((NC - 2) & ~1) + 1;
+                                R Ev = c[SE];
 #pragma unroll
-                            for (int i = SE - 2; i >= 0; i -= 2) E = fma(E, r2, cf[i]);
-                            R O = cf[SO];
+                                for (int i = SE - 2; i >= 0; i -= 2) Ev = fma(Ev, r2, c[i]);
+                                R Od = c[SO];
 #pragma unroll
-                            for (int i = SO - 2; i >= 0; i -= 2) O = fma(O, r2, cf[i]);
-                            u = fma(E, rinv, O);
-                        } else {
-                            const R x = fma(r2, rinv, cen) * rsc;
-                            R p = c[NC - 1];
+                                for (int i = SO - 2; i >= 0; i -= 2) Od = fma(Od, r2, c[i]);
+                                u = fma(Ev, rinv, Od);
+                            } else {
+                                const R x = fma(r2, rinv, cen) * rsc;
+                                R p = c[NC - 1];
 #pragma unroll
-                            for (int i = NC - 2; i >= 0; --i) p = fma(p, x, c[i]);
-                            u = p * rinv;
+                                for (int i = NC - 2; i >= 0; --i) p = fma(p, x, c[i]);
+                                u = p * rinv;
+                            }
+                            ea = fma(u, q.w, ea);
                         }
-                        const bool in = (r2 > R(1e-30)) && (r2 < d2max);
-                        ea = in ? fma(u, q.w, ea) : ea;
                     }
                 }
             }
-            acc += (double)ea;
         }
-        if (active) pot_sorted[tbeg + t] += (R)acc;
+        acc += (double)ea;
     }
+    if (active) pot_sorted[it.x + lane] += (R)acc;
+}
+
+template <class R>
+__global__ void k_fill(R *__restrict__ p, R v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------------------ far field

@@ -29,19 +29,18 @@ void DevBuf::alloc(size_t nbytes) {
 
 namespace {
 
-constexpr int kMaxScratchBlocks = 2048;
+constexpr size_t kPinBytes = size_t(512) << 20;  // incoming plane-wave scratch cap
 
 int nc_for_digits(int d) {
     static const int nc[] = {5, 5, 5, 6, 9, 11, 12, 15, 17, 18, 22, 23, 25};
     return nc[std::min(12, std::max(0, d))];
 }
 
-struct CounterKernel {};
 
 // Exact count of near-field pairs inside the support (1e-30 < r^2 < bs^2), once per tree.
 template <class R>
 __global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg, const int *__restrict__ leaf_trg,
-                             const int *__restrict__ l1_ptr, const int *__restrict__ l1_src, const P4<R> *__restrict__ l1_geom,
+                             const int *__restrict__ l1_ptr, const NfEntry *__restrict__ l1, const P4<R> *__restrict__ geom,
                              unsigned long long *__restrict__ counter) {
     const int lf = blockIdx.x;
     const int tbeg = leaf_trg[2 * lf], tcnt = leaf_trg[2 * lf + 1];
@@ -49,10 +48,10 @@ __global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restr
     for (int t = threadIdx.x; t < tcnt; t += blockDim.x) {
         const P4<R> me = trg[tbeg + t];
         for (int e = l1_ptr[lf]; e < l1_ptr[lf + 1]; ++e) {
-            const P4<R> gm = l1_geom[e];
+            const P4<R> gm = geom[e];
             const R bs = gm.w, d2 = bs * bs;
-            for (int s = 0; s < l1_src[2 * e + 1]; ++s) {
-                const P4<R> q = src[l1_src[2 * e] + s];
+            for (int s = 0; s < l1[e].scnt; ++s) {
+                const P4<R> q = src[l1[e].sbeg + s];
                 const R dx = me.x + gm.x - q.x, dy = me.y + gm.y - q.y, dz = me.z + gm.z - q.z;
                 const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                 c += (r2 > R(1e-30) && r2 < d2) ? 1 : 0;
@@ -87,11 +86,32 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     // ---------------- sources and targets (sorted order), coordinates relative to their
     // leaf centre: offsets between boxes are formed in FP64 on the host, so FP32 kernels
     // only see small local coordinates
-    std::vector<int> src_leaf(nsall, -1), trg_leaf(n, -1);
+    // Inside each leaf the points are further ordered by a Morton code of their position
+    // in the leaf, so that runs of consecutive points are spatially compact (near-field
+    // work items and source chunks); box ranges are unchanged by this.
+    std::vector<int> src_leaf(nsall, -1), trg_leaf(n, -1), src_ord(nsall), trg_ord(n);
+    auto morton_order = [&](const std::vector<double> &xyz, int beg, int cnt, int b, int *ord) {
+        std::vector<std::pair<unsigned long long, int>> key((size_t)cnt);
+        const double inv = 1.0 / t.boxsize[t.level[b]];
+        for (int i = 0; i < cnt; ++i) {
+            unsigned long long code = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double u = (xyz[3 * (size_t)(beg + i) + d] - t.center[3 * (size_t)b + d]) * inv + 0.5;
+                const unsigned long long q = (unsigned long long)std::min(1023.0, std::max(0.0, std::floor(u * 1024.0)));
+                for (int bit = 0; bit < 10; ++bit) code |= ((q >> bit) & 1ull) << (3 * bit + d);
+            }
+            key[i] = {code, beg + i};
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const auto &a, const auto &c) { return a.first < c.first; });
+        for (int i = 0; i < cnt; ++i) ord[beg + i] = key[i].second;
+    };
     for (int b = 0; b < nboxes; ++b) {
         if (!t.leaf[b]) continue;
         for (int i = 0; i < t.src_count[b]; ++i) src_leaf[t.src_begin[b] + i] = b;
         for (int i = 0; i < t.trg_count[b]; ++i) trg_leaf[t.trg_begin[b] + i] = b;
+        morton_order(t.src_sorted, t.src_begin[b], t.src_count[b], b, src_ord.data());
+        morton_order(t.trg_sorted, t.trg_begin[b], t.trg_count[b], b, trg_ord.data());
     }
     auto local = [&](const std::vector<double> &xyz, size_t i, int b, int d) {
         return (Real)(xyz[3 * i + d] - t.center[3 * (size_t)b + d]);
@@ -99,26 +119,30 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     std::vector<P4<Real>> srcv(nsall);
     std::vector<int> su(nsall);
     for (int i = 0; i < nsall; ++i) {
-        const int b = src_leaf[i];
-        srcv[i] = {local(t.src_sorted, i, b, 0), local(t.src_sorted, i, b, 1), local(t.src_sorted, i, b, 2), Real(0)};
-        const int o = t.src_order[i];
+        const int b = src_leaf[i], j = src_ord[i];
+        srcv[i] = {local(t.src_sorted, j, b, 0), local(t.src_sorted, j, b, 1), local(t.src_sorted, j, b, 2), Real(0)};
+        const int o = t.src_order[j];
         su[i] = o < N ? o : g.img_src[o - N];
     }
     std::vector<P4<Real>> trgv(n);
+    std::vector<int> tu(n);
     std::vector<double> tw0(n, 0.0);
     for (int i = 0; i < n; ++i) {
-        const int b = trg_leaf[i];
-        trgv[i] = {local(t.trg_sorted, i, b, 0), local(t.trg_sorted, i, b, 1), local(t.trg_sorted, i, b, 2), Real(0)};
+        const int b = trg_leaf[i], j = trg_ord[i];
+        trgv[i] = {local(t.trg_sorted, j, b, 0), local(t.trg_sorted, j, b, 1), local(t.trg_sorted, j, b, 2), Real(0)};
+        tu[i] = t.trg_order[j];
     }
     for (int b = 0; b < nboxes; ++b)
         if (t.leaf[b] && t.trg_count[b] > 0) {
             const double w = T.w0[t.level[b] + t.pwexp[b]];
             for (int i = 0; i < t.trg_count[b]; ++i) tw0[t.trg_begin[b] + i] = w;
         }
+    srcv.resize((size_t)nsall + NF_CHUNK, P4<Real>{Real(0), Real(0), Real(0), Real(0)});  // chunk padding
     src.upload(srcv);
+    srcv.resize(nsall);
     src_user.upload(su);
     trg.upload(trgv);
-    trg_user.upload(t.trg_order);
+    trg_user.upload(tu);
     trg_w0.upload(tw0);
     pot_sorted.alloc(sizeof(Real) * (size_t)n);
     q_dev.alloc(sizeof(double) * (size_t)n);
@@ -292,11 +316,57 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
 
     // ---------------- near field
     {
-        std::vector<int> tr, ptr{0}, sr;
-        std::vector<P4<Real>> bsv;  // (centre of target leaf - centre of source leaf, bs)
-        double cand = 0;
+        // source chunks of every leaf: bounding boxes of NF_CHUNK consecutive points, taken
+        // over the rounded coordinates the kernel reads
+        std::vector<int> cbase(nboxes, -1);
+        std::vector<P4<Real>> cb;
+        auto box_of = [](const P4<Real> *pts, int cnt, P4<Real> &lo, P4<Real> &hi) {
+            lo = pts[0];
+            hi = pts[0];
+            for (int i = 1; i < cnt; ++i) {
+                lo.x = std::min(lo.x, pts[i].x); lo.y = std::min(lo.y, pts[i].y); lo.z = std::min(lo.z, pts[i].z);
+                hi.x = std::max(hi.x, pts[i].x); hi.y = std::max(hi.y, pts[i].y); hi.z = std::max(hi.z, pts[i].z);
+            }
+            lo.w = hi.w = Real(0);
+        };
+        for (int b = 0; b < nboxes; ++b) {
+            if (!t.leaf[b] || t.src_count[b] == 0) continue;
+            cbase[b] = (int)(cb.size() / 2);
+            for (int c = 0; c < t.src_count[b]; c += NF_CHUNK) {
+                P4<Real> lo, hi;
+                box_of(&srcv[t.src_begin[b] + c], std::min(NF_CHUNK, t.src_count[b] - c), lo, hi);
+                cb.push_back(lo);
+                cb.push_back(hi);
+            }
+        }
+        // even/odd (r^2) monomial coefficients, one row per distinct bs:
+        // sum_i c_i rsc^i (r + cen)^i expanded in powers of r
+        const bool split = n_digits < 6;
+        std::vector<double> bs_seen;
+        std::vector<Real> split_tab;
+        auto cf_index = [&](double bs) {
+            if (!split) return 0;
+            for (size_t k = 0; k < bs_seen.size(); ++k)
+                if (bs_seen[k] == bs) return (int)k;
+            bs_seen.push_back(bs);
+            const double rsc = 2.0 / bs, cen = -bs / 2.0;
+            std::vector<double> cd(nc);
+            double rp = 1;
+            for (int i = 0; i < nc; ++i) { cd[i] = T.residual[i] * rp; rp *= rsc; }
+            for (int i = 0; i < nc; ++i)
+                for (int j = nc - 1; j > i; --j) cd[j - 1] += cen * cd[j];
+            for (int i = 0; i < NF_SPLIT_STRIDE; ++i) split_tab.push_back(i < nc ? (Real)cd[i] : Real(0));
+            return (int)bs_seen.size() - 1;
+        };
+        std::vector<int> tr, ptr{0};
+        std::vector<NfEntry> l1v;
+        std::vector<P4<Real>> gv;  // (centre of target leaf - centre of source leaf, bs)
+        std::vector<int4> itv;
+        std::vector<P4<Real>> wb;
+        double cand = 0, src_pts = 0;
         for (int b = 0; b < nboxes; ++b) {
             if (!t.leaf[b] || t.trg_count[b] == 0 || t.l1_begin[b + 1] == t.l1_begin[b]) continue;
+            const int li = (int)ptr.size() - 1;
             tr.push_back(t.trg_begin[b]);
             tr.push_back(t.trg_count[b]);
             for (int e = t.l1_begin[b]; e < t.l1_begin[b + 1]; ++e) {
@@ -304,28 +374,39 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 double bs = t.boxsize[t.level[sb]];
                 if (t.pwexp[sb] && sb == b) bs *= 0.5;
                 else if (t.level[sb] < t.level[b]) bs = t.boxsize[t.level[b]];
-                sr.push_back(t.src_begin[sb]);
-                sr.push_back(t.src_count[sb]);
-                bsv.push_back({(Real)(t.center[3 * b] - t.center[3 * sb]),
-                               (Real)(t.center[3 * b + 1] - t.center[3 * sb + 1]),
-                               (Real)(t.center[3 * b + 2] - t.center[3 * sb + 2]), (Real)bs});
+                if (t.src_count[sb] == 0) continue;
+                l1v.push_back({t.src_begin[sb], t.src_count[sb], cbase[sb], cf_index(bs)});
+                gv.push_back({(Real)(t.center[3 * b] - t.center[3 * sb]), (Real)(t.center[3 * b + 1] - t.center[3 * sb + 1]),
+                              (Real)(t.center[3 * b + 2] - t.center[3 * sb + 2]), (Real)bs});
                 cand += (double)t.trg_count[b] * t.src_count[sb];
+                src_pts += t.src_count[sb];
             }
-            ptr.push_back((int)bsv.size());
+            ptr.push_back((int)l1v.size());
+            for (int w = 0; w < t.trg_count[b]; w += 64) {
+                const int cnt = std::min(64, t.trg_count[b] - w);
+                itv.push_back(make_int4(t.trg_begin[b] + w, cnt, li, 0));
+                P4<Real> lo, hi;
+                box_of(&trgv[t.trg_begin[b] + w], cnt, lo, hi);
+                wb.push_back(lo);
+                wb.push_back(hi);
+            }
         }
         n_nf_leaves = (int)ptr.size() - 1;
+        n_nf_items = (int)itv.size();
         nf_trg.upload(tr);
         nf_ptr.upload(ptr);
-        nf_src.upload(sr);
-        nf_bs.upload(bsv);
-        std::vector<Real> cf(T.residual.begin(), T.residual.end());
-        coeffs.upload(cf);
+        nf_l1.upload(l1v);
+        nf_geom.upload(gv);
+        nf_items.upload(itv);
+        nf_wbox.upload(wb);
+        nf_cbox.upload(cb);
+        if (split) coeffs.upload(split_tab);
+        else coeffs.upload(std::vector<Real>(T.residual.begin(), T.residual.end()));
         work.nf_pairs_candidate = cand;
-        stats.n_list1 = (long long)bsv.size();
-        // near-field kernel HBM bytes: each list entry streams its sources once
-        double sb_bytes = 0;
-        for (size_t e = 0; e < bsv.size(); ++e) sb_bytes += (double)sr[2 * e + 1] * sizeof(P4<Real>);
-        work.nf_bytes = sb_bytes + (double)n * (sizeof(P4<Real>) + 2 * sizeof(Real));
+        stats.n_list1 = (long long)l1v.size();
+        // near-field algorithmic HBM bytes: every target read and written once, every
+        // list entry's sources read once per target leaf
+        work.nf_bytes = src_pts * sizeof(P4<Real>) + (double)n * (sizeof(P4<Real>) + 2 * sizeof(Real));
     }
 
     // ---------------- far field
@@ -380,16 +461,12 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
     }
 
-    // ---------------- scratch for the transform kernels
+    // ---------------- incoming plane-wave scratch: one level's boxes at a time, in batches
     {
-        size_t per = std::max({2 * P3 * sizeof(Real), (size_t)(p * p * H + p * n_pw * H) * 2 * sizeof(Real),
-                               (size_t)(NM + p * n_pw * H + p * p * H) * 2 * sizeof(Real)});
-        int items = std::max(n_up, 1);
+        int items = 1;
         for (auto &E : eval_levels) items = std::max(items, E.n_items);
-        for (auto &L : c2p_levels) items = std::max(items, L.n_items);
-        for (auto &L : p2c_levels) items = std::max(items, L.n_items);
-        scratch_blocks = std::min(kMaxScratchBlocks, items);
-        scratch.alloc(per * scratch_blocks);
+        pin_batch = std::max(1, std::min(items, (int)(kPinBytes / ((size_t)NM * 2 * sizeof(Real)))));
+        pin.alloc((size_t)pin_batch * NM * 2 * sizeof(Real));
     }
 
     // ---------------- statistics and exact near-field pair count
@@ -402,7 +479,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         cnt.alloc(sizeof(unsigned long long));
         check(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), stream), "hipMemsetAsync");
         hipLaunchKernelGGL(k_near_count<Real>, dim3(n_nf_leaves), dim3(256), 0, stream, src.as<P4<Real>>(),
-                           trg.as<P4<Real>>(), nf_trg.as<int>(), nf_ptr.as<int>(), nf_src.as<int>(), nf_bs.as<P4<Real>>(),
+                           trg.as<P4<Real>>(), nf_trg.as<int>(), nf_ptr.as<int>(), nf_l1.as<NfEntry>(), nf_geom.as<P4<Real>>(),
                            cnt.as<unsigned long long>());
         check(hipGetLastError(), "k_near_count launch");
         unsigned long long c = 0;
@@ -438,56 +515,53 @@ Plan<Real>::~Plan() {
 
 namespace {
 
-template <class Real, int P>
+template <class Real, int P, int NPW>
 void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     using C = Cx<Real>;
-    const int H = (pl.n_pw + 1) / 2;
+    const int NM = NPW * NPW * ((NPW + 1) / 2);
     if (pl.n_up > 0) {
-        const int bt = std::min(512, ((P * P + 63) / 64) * 64);
-        hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(bt), 0, s, pl.src.template as<P4<Real>>(),
+        hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(C2pCfg<P>::threads), 0, s, pl.src.template as<P4<Real>>(),
                            pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(), pl.up_rng.template as<int>(),
                            pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(), pl.rng_off.template as<Real>(),
                            pl.proxy_up.template as<Real>());
         check(hipGetLastError(), "k_charge2proxy");
     }
     for (auto &L : pl.c2p_levels) {
-        const int nb = std::min(L.n_items, pl.scratch_blocks);
-        hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(nb), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
+        hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(L.n_items), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
                            pl.proxy_up.template as<Real>(), L.out.template as<int>(), L.add.template as<int>(),
                            L.ptr.template as<int>(), L.in_slot.template as<int>(), L.in_code.template as<int>(),
-                           pl.mats.template as<Real>(), 1, L.n_items, pl.scratch.template as<Real>());
+                           pl.mats.template as<Real>(), 1, L.n_items);
         check(hipGetLastError(), "k_tensor3 c2p");
     }
     if (pl.n_up > 0) {
-        const int nb = std::min(pl.n_up, pl.scratch_blocks);
-        hipLaunchKernelGGL((k_proxy2pw<Real, P>), dim3(nb), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
-                           pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(), pl.n_pw, pl.n_up,
-                           pl.scratch.template as<C>());
+        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(pl.n_up), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
+                           pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(), pl.n_up);
         check(hipGetLastError(), "k_proxy2pw");
     }
     for (size_t l = 0; l < pl.eval_levels.size(); ++l) {
         auto &E = pl.eval_levels[l];
-        if (E.n_items > 0) {
-            const int nb = std::min(E.n_items, pl.scratch_blocks);
-            const Real lscale = (Real)std::ldexp(1.0, E.level);
-            hipLaunchKernelGGL((k_shift_pw2proxy<Real, P>), dim3(nb), dim3(256), 0, s, pl.pw_out.template as<C>(),
-                               pl.shift0.template as<C>(), lscale, E.down.template as<int>(), E.zero.template as<int>(),
-                               E.nbr_ptr.template as<int>(), E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(),
-                               pl.pw2poly.template as<C>(), pl.n_pw, E.n_items, pl.proxy_down.template as<Real>(),
-                               pl.scratch.template as<C>());
-            check(hipGetLastError(), "k_shift_pw2proxy");
+        const Real lscale = (Real)std::ldexp(1.0, E.level);
+        for (int i0 = 0; i0 < E.n_items; i0 += pl.pin_batch) {
+            const int nb = std::min(pl.pin_batch, E.n_items - i0);
+            hipLaunchKernelGGL(k_pw_shift<Real>, dim3((NM + 255) / 256, nb), dim3(256), 0, s, pl.pw_out.template as<C>(),
+                               pl.shift0.template as<C>(), lscale, E.nbr_ptr.template as<int>(),
+                               E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), NM, i0,
+                               pl.pin.template as<C>());
+            check(hipGetLastError(), "k_pw_shift");
+            hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb), dim3(256), 0, s, pl.pin.template as<C>(),
+                               pl.pw2poly.template as<C>(), E.down.template as<int>(), E.zero.template as<int>(), i0, nb,
+                               pl.proxy_down.template as<Real>());
+            check(hipGetLastError(), "k_pw2proxy");
         }
         auto &L = pl.p2c_levels[l];
         if (L.n_items > 0) {
-            const int nb = std::min(L.n_items, pl.scratch_blocks);
-            hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(nb), dim3(256), 0, s, pl.proxy_down.template as<Real>(),
+            hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(L.n_items), dim3(256), 0, s, pl.proxy_down.template as<Real>(),
                                pl.proxy_down.template as<Real>(), L.out.template as<int>(), L.add.template as<int>(),
                                L.ptr.template as<int>(), L.in_slot.template as<int>(), L.in_code.template as<int>(),
-                               pl.mats.template as<Real>(), 0, L.n_items, pl.scratch.template as<Real>());
+                               pl.mats.template as<Real>(), 0, L.n_items);
             check(hipGetLastError(), "k_tensor3 p2c");
         }
     }
-    (void)H;
     if (pl.n_leaf_eval > 0) {
         hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(256), 0, s,
                            pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
@@ -499,18 +573,21 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
 
 template <class Real, int NC, bool SPLIT>
 void launch_near(Plan<Real> &pl, hipStream_t s) {
-    if (pl.n_nf_leaves == 0) return;
-    hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3(pl.n_nf_leaves), dim3(256), 0, s,
-                       pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_trg.template as<int>(),
-                       pl.nf_ptr.template as<int>(), pl.nf_src.template as<int>(), pl.nf_bs.template as<P4<Real>>(),
-                       pl.coeffs.template as<Real>(), pl.pot_sorted.template as<Real>());
+    if (pl.n_nf_items == 0) return;
+    hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3((pl.n_nf_items + 3) / 4), dim3(256), 0, s,
+                       pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_items.template as<int4>(),
+                       pl.nf_wbox.template as<P4<Real>>(), pl.nf_ptr.template as<int>(), pl.nf_l1.template as<NfEntry>(),
+                       pl.nf_geom.template as<P4<Real>>(), pl.nf_cbox.template as<P4<Real>>(),
+                       pl.coeffs.template as<Real>(), pl.pot_sorted.template as<Real>(), pl.n_nf_items);
     check(hipGetLastError(), "k_near_field");
 }
 
 template <class Real>
 void launch_far(Plan<Real> &pl, const double *d_q, hipStream_t s) {
     if (pl.n_ff_modes == 0) {
-        check(hipMemsetAsync(pl.far.p, 0, sizeof(double) * pl.n, s), "hipMemsetAsync far");
+        hipLaunchKernelGGL(k_fill<double>, dim3((pl.n + 255) / 256), dim3(256), 0, s, pl.far.template as<double>(), 0.0,
+                           pl.n);
+        check(hipGetLastError(), "k_fill far");
         return;
     }
     const size_t lds = sizeof(double2) * FF_TP * (pl.ms + pl.mt + pl.mu);
@@ -553,10 +630,10 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     mark(2);
     if (stages & 1) {
         switch (p) {
-        case 9: launch_transforms<Real, 9>(*this, s); break;
-        case 18: launch_transforms<Real, 18>(*this, s); break;
-        case 28: launch_transforms<Real, 28>(*this, s); break;
-        case 38: launch_transforms<Real, 38>(*this, s); break;
+        case 9: launch_transforms<Real, 9, 13>(*this, s); break;
+        case 18: launch_transforms<Real, 18, 27>(*this, s); break;
+        case 28: launch_transforms<Real, 28, 39>(*this, s); break;
+        case 38: launch_transforms<Real, 38, 55>(*this, s); break;
         default: throw std::logic_error("unsupported proxy order");
         }
     }

@@ -76,14 +76,15 @@ struct Plan {
     DevBuf proxy_down;
     int n_leaf_eval = 0;
     DevBuf leaf_down, leaf_geom, leaf_trg;
-    // near field
-    int n_nf_leaves = 0;
-    DevBuf nf_trg, nf_ptr, nf_src, nf_bs, coeffs;
+    // near field: per target leaf (nf_trg, nf_ptr -> list1 entries nf_l1/nf_geom), per
+    // wavefront item (nf_items, nf_wbox), per source chunk (nf_cbox)
+    int n_nf_leaves = 0, n_nf_items = 0;
+    DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs;
     // far field
     DevBuf theta, ff_w, ff_chunk_ijk, ff_chunk_m0, ff_chunk_len, ff_line_jk, ff_line_i, ff_line_ptr, ff_Sp, ff_S;
-    // scratch for transform kernels
-    DevBuf scratch;
-    int scratch_blocks = 0;
+    // incoming plane waves of up to pin_batch boxes of one level
+    DevBuf pin;
+    int pin_batch = 0;
 
     Plan(const double *cell_colmajor, const double *pos, int N, int n_digits, int n_per_leaf, int nz);
     ~Plan();

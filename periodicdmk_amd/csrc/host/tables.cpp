@@ -170,12 +170,13 @@ DmkTables make_tables(int n_digits_user, double rc, int nlevels, const double *U
         }
         fk[i] = s / T.c0 * ws / (k * k);
     }
+    // mode layout m = c (h n) + a n + b: c = x index (n), a = z index (h), b = y index (n)
     T.radial0.resize(T.n_modes);
-    for (int a = 0, m = 0; a < T.h; ++a)
+    for (int a = 0; a < T.h; ++a)
         for (int b = 0; b < T.n_pw; ++b)
-            for (int c = 0; c < T.n_pw; ++c, ++m) {
+            for (int c = 0; c < T.n_pw; ++c) {
                 const int ja = a - half, jb = b - half, jc = c - half;
-                T.radial0[m] = fk[ja * ja + jb * jb + jc * jc];
+                T.radial0[mode_index(T, a, b, c)] = fk[ja * ja + jb * jb + jc * jc];
             }
 
     // Chebyshev nodes (ascending) and the closed-form inverse Vandermonde
@@ -204,9 +205,10 @@ DmkTables make_tables(int n_digits_user, double rc, int nlevels, const double *U
     T.shift0.assign(27 * (size_t)T.n_modes, 0);
     for (int t = 0; t < 27; ++t) {
         const int kz = t / 9 - 1, ky = (t / 3) % 3 - 1, kx = t % 3 - 1;
-        for (int a = 0, m = 0; a < T.h; ++a)
+        for (int a = 0; a < T.h; ++a)
             for (int b = 0; b < T.n_pw; ++b)
-                for (int c = 0; c < T.n_pw; ++c, ++m) {
+                for (int c = 0; c < T.n_pw; ++c) {
+                    const int m = mode_index(T, a, b, c);
                     const double ph = ((a - half) * kz + (b - half) * ky + (c - half) * kx) * sphase;
                     T.shift0[(size_t)t * T.n_modes + m] = std::polar(T.radial0[m], ph);
                 }

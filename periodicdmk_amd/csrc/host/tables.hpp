@@ -37,6 +37,8 @@ struct DmkTables {
     int nlevels = 0;
     std::vector<double> boxsize;              // nlevels + 1
     std::vector<double> w0;                   // nlevels + 1, indexed by depth + pwexp
+    // plane-wave modes (z index a < h, y index b < n_pw, x index c < n_pw) are stored at
+    // m = c (h n_pw) + a n_pw + b, see mode_index()
     std::vector<double> radial0;              // n_modes, level-0 difference kernel FT (level l: x 2^l)
     std::vector<std::complex<double>> poly2pw, pw2poly;  // [n_pw x p] column-major, same at every level
     std::vector<std::complex<double>> shift0;  // [27][n_modes]: translation phase x radial0
@@ -47,6 +49,8 @@ struct DmkTables {
     std::vector<int> mode_ijk;                // 3 per half-space mode with nonzero weight
     std::vector<double> mode_w;               // 2 * weight (conjugate mode folded in)
 };
+
+inline int mode_index(const DmkTables &T, int a, int b, int c) { return (c * T.h + a) * T.n_pw + b; }
 
 // n_digits in 1..12; rc and nlevels describe the tree; umat is the lattice frame.
 DmkTables make_tables(int n_digits, double rc, int nlevels, const double *umat);
