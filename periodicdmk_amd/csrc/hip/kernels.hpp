@@ -162,9 +162,9 @@ constexpr int kTfLdsBytes = 64 * 1024;
 
 template <int N> struct OddStride { static constexpr int v = (N % 2) ? N : N + 1; };
 
-// slices per pass so that `per_slice` bytes x slices stays within the LDS budget
-constexpr int tf_slices(int total, int per_slice) {
-    const int k = kTfLdsBytes / per_slice;
+// slices per pass so that `reserved` + `per_slice` bytes x slices stays within the LDS budget
+constexpr int tf_slices(int total, int per_slice, int reserved = 0) {
+    const int k = (kTfLdsBytes - reserved) / per_slice;
     return k < 1 ? 1 : (k > total ? total : k);
 }
 
@@ -172,93 +172,115 @@ constexpr int tf_slices(int total, int per_slice) {
 // (tensor layout [c][b][a], a fastest).  U_d is the minus/plus parent<->child matrix selected
 // by bit d of the child code, transposed for child -> parent.  Item i writes out slot
 // item_out[i] from the CSR range item_ptr[i]..item_ptr[i+1] of (in slot, child code);
-// item_add[i] = accumulate into the existing output.
-template <class R, int P>
-__global__ __launch_bounds__(256) void k_tensor3(const R *__restrict__ in_base, R *__restrict__ out_base,
-                                                  const int *__restrict__ item_out, const int *__restrict__ item_add,
-                                                  const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
-                                                  const int *__restrict__ in_code, const R *__restrict__ mats,
-                                                  int transpose, int n_items) {
-    constexpr int PP = P * P, TS = OddStride<P>::v;
-    constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R));
+// item_add[i] = accumulate into the existing output.  Workgroup (item, slice) computes the
+// output planes c' in [slice kc, slice kc + kc), kc <= T3Cfg::KC.
+template <class R, int P> struct T3Cfg {
+    static constexpr int PP = P * P, TS = OddStride<P>::v;
+    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * PP * (int)sizeof(R));
+    static constexpr int threads = PP <= 384 ? (PP + 63) / 64 * 64 : (P >= 38 ? 512 : 256);
+};
+
+template <class R, int P, bool TRANSPOSE>
+__global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
+    const R *__restrict__ in_base, R *__restrict__ out_base, const int *__restrict__ item_out,
+    const int *__restrict__ item_add, const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
+    const int *__restrict__ in_code, const R *__restrict__ mats, int n_items, int kc) {
+    constexpr int PP = P * P, TS = T3Cfg<R, P>::TS, KC = T3Cfg<R, P>::KC, NT = T3Cfg<R, P>::threads;
+    constexpr int XC = (KC * P + NT - 1) / NT;
     __shared__ R T1[KC * PP];       // [k][b][a]
     __shared__ R T2[KC * P * TS];   // [k][b'][a] rows of TS
+    __shared__ R Ms[2 * PP];        // minus / plus matrices
     const int it = blockIdx.x;
     if (it >= n_items) return;
+    const int k0 = blockIdx.y * kc;
+    if (k0 >= P) return;
+    for (int e = threadIdx.x; e < 2 * PP; e += NT) Ms[e] = mats[e];
+    const int nk = min(kc, P - k0);
     R *out = out_base + (size_t)item_out[it] * P * PP;
     const bool add = item_add[it] != 0;
     const int e0 = item_ptr[it], e1 = item_ptr[it + 1];
     // U_d(r, c) = M_s[r + P c] (parent -> child) or M_s[c + P r] (child -> parent)
-    const int sr = transpose ? P : 1, sc = transpose ? 1 : P;
-    for (int k0 = 0; k0 < P; k0 += KC) {
-        const int nk = min(KC, P - k0);
-        // x-step accumulators: thread column (k, b'), outputs a' -- persistent over children
-        constexpr int XC = (KC * P + 255) / 256;
-        R acc[XC][P];
+    constexpr int sr = TRANSPOSE ? P : 1, sc = TRANSPOSE ? 1 : P;
+    // x-step accumulators: thread column (k, b'), outputs a' -- persistent over children
+    R acc[XC][P];
 #pragma unroll
-        for (int q = 0; q < XC; ++q)
+    for (int q = 0; q < XC; ++q)
 #pragma unroll
-            for (int a = 0; a < P; ++a) acc[q][a] = R(0);
-        for (int e = e0; e < e1; ++e) {
-            const R *in = in_base + (size_t)in_slot[e] * P * PP;
-            const int code = in_code[e];
-            const R *M0 = mats + (code & 1) * PP, *M1 = mats + ((code >> 1) & 1) * PP, *M2 = mats + ((code >> 2) & 1) * PP;
-            __syncthreads();
-            // z: T1[k][b][a] = sum_c U2(k0 + k, c) in[c][b][a]
-            for (int j = threadIdx.x; j < PP; j += blockDim.x) {
-                R x[P];
+        for (int a = 0; a < P; ++a) acc[q][a] = R(0);
+    for (int e = e0; e < e1; ++e) {
+        const R *in = in_base + (size_t)in_slot[e] * P * PP;
+        const int code = in_code[e];
+        const R *M0 = Ms + (code & 1) * PP, *M1 = Ms + ((code >> 1) & 1) * PP, *M2 = Ms + ((code >> 2) & 1) * PP;
+        __syncthreads();
+        // z: T1[k][b][a] = sum_c U2(k0 + k, c) in[c][b][a]
+        for (int j = threadIdx.x; j < PP; j += NT) {
+            R x[P];
 #pragma unroll
-                for (int c = 0; c < P; ++c) x[c] = in[c * PP + j];
-                for (int k = 0; k < nk; ++k) {
-                    R s = 0;
+            for (int c = 0; c < P; ++c) x[c] = in[c * PP + j];
+            for (int k = 0; k < nk; ++k) {
+                R s = 0;
 #pragma unroll
-                    for (int c = 0; c < P; ++c) s = fma(M2[(k0 + k) * sr + c * sc], x[c], s);
-                    T1[k * PP + j] = s;
-                }
-            }
-            __syncthreads();
-            // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]
-            for (int w = threadIdx.x; w < nk * P; w += blockDim.x) {
-                const int k = w / P, a = w - k * P;
-                R x[P];
-#pragma unroll
-                for (int b = 0; b < P; ++b) x[b] = T1[k * PP + b * P + a];
-                for (int b2 = 0; b2 < P; ++b2) {
-                    R s = 0;
-#pragma unroll
-                    for (int b = 0; b < P; ++b) s = fma(M1[b2 * sr + b * sc], x[b], s);
-                    T2[(k * P + b2) * TS + a] = s;
-                }
-            }
-            __syncthreads();
-            // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a], column (k, b')
-#pragma unroll
-            for (int q = 0; q < XC; ++q) {
-                const int w = threadIdx.x + q * 256;
-                if (w < nk * P) {
-                    R x[P];
-#pragma unroll
-                    for (int a = 0; a < P; ++a) x[a] = T2[w * TS + a];
-#pragma unroll
-                    for (int a2 = 0; a2 < P; ++a2) {
-                        R s = acc[q][a2];
-#pragma unroll
-                        for (int a = 0; a < P; ++a) s = fma(M0[a2 * sr + a * sc], x[a], s);
-                        acc[q][a2] = s;
-                    }
-                }
+                for (int c = 0; c < P; ++c) s = fma(M2[(k0 + k) * sr + c * sc], x[c], s);
+                T1[k * PP + j] = s;
             }
         }
+        __syncthreads();
+        // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]
+        for (int w = threadIdx.x; w < nk * P; w += NT) {
+            const int k = w / P, a = w - k * P;
+            R x[P];
+#pragma unroll
+            for (int b = 0; b < P; ++b) x[b] = T1[k * PP + b * P + a];
+            for (int b2 = 0; b2 < P; ++b2) {
+                R s = 0;
+#pragma unroll
+                for (int b = 0; b < P; ++b) s = fma(M1[b2 * sr + b * sc], x[b], s);
+                T2[(k * P + b2) * TS + a] = s;
+            }
+        }
+        __syncthreads();
+        // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a], column (k, b')
 #pragma unroll
         for (int q = 0; q < XC; ++q) {
-            const int w = threadIdx.x + q * 256;
+            const int w = threadIdx.x + q * NT;
             if (w < nk * P) {
-                R *dst = out + (size_t)k0 * PP + (size_t)w * P;  // [c' = k0 + k][b'][a']
+                R x[P];
 #pragma unroll
-                for (int a2 = 0; a2 < P; ++a2) dst[a2] = add ? dst[a2] + acc[q][a2] : acc[q][a2];
+                for (int a = 0; a < P; ++a) x[a] = T2[w * TS + a];
+#pragma unroll
+                for (int a2 = 0; a2 < P; ++a2) {
+                    R s = acc[q][a2];
+#pragma unroll
+                    for (int a = 0; a < P; ++a) s = fma(M0[a2 * sr + a * sc], x[a], s);
+                    acc[q][a2] = s;
+                }
             }
         }
     }
+#pragma unroll
+    for (int q = 0; q < XC; ++q) {
+        const int w = threadIdx.x + q * NT;
+        if (w < nk * P) {
+            R *dst = out + (size_t)k0 * PP + (size_t)w * P;  // [c' = k0 + k][b'][a']
+#pragma unroll
+            for (int a2 = 0; a2 < P; ++a2) dst[a2] = add ? dst[a2] + acc[q][a2] : acc[q][a2];
+        }
+    }
+}
+
+// proxy[parent_slot[i]] += sum_{k in child_ptr[i] .. child_ptr[i+1]} scratch[k], element-wise, the
+// children added in order (deterministic).
+template <class R, int P>
+__global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, const R *__restrict__ scratch,
+                                                       const int *__restrict__ parent_slot, const int *__restrict__ child_ptr,
+                                                       int n_parents) {
+    constexpr int P3 = P * P * P;
+    const int it = blockIdx.x, e = blockIdx.y * blockDim.x + threadIdx.x;
+    if (it >= n_parents || e >= P3) return;
+    R *dst = proxy + (size_t)parent_slot[it] * P3 + e;
+    R v = *dst;
+    for (int k = child_ptr[it]; k < child_ptr[it + 1]; ++k) v += scratch[(size_t)k * P3 + e];
+    *dst = v;
 }
 
 // ------------------------------------------------------------------ plane waves
@@ -436,40 +458,87 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
     }
 }
 
-// pot_sorted[t] += sum C(a,b,c) T_a T_b T_c at the targets of one leaf per workgroup.
+// pot_sorted[t] += sum C(a,b,c) T_a T_b T_c at the targets of one leaf per workgroup.  The
+// coefficient planes C[c][.][.] are staged through LDS one at a time and read as broadcasts;
+// each thread evaluates EV_TPT targets, T_a sits in registers and T_b, T_c follow the
+// Chebyshev recurrence along their loops.
+constexpr int EV_THREADS = 128, EV_TPT = 2;
+
 template <class R, int P>
-__global__ __launch_bounds__(256) void k_eval_targets(const R *__restrict__ down_base, const int *__restrict__ leaf_down,
-                                                       const R *__restrict__ leaf_geom, const int *__restrict__ leaf_trg,
-                                                       const P4<R> *__restrict__ trg, R *__restrict__ pot_sorted) {
+__global__ __launch_bounds__(EV_THREADS) void k_eval_targets(const R *__restrict__ down_base, const int *__restrict__ leaf_down,
+                                                              const R *__restrict__ leaf_geom, const int *__restrict__ leaf_trg,
+                                                              const P4<R> *__restrict__ trg, R *__restrict__ pot_sorted) {
+    constexpr int PP = P * P;
+    __shared__ R plane[PP];
     const int lf = blockIdx.x;
-    const R *C = down_base + (size_t)leaf_down[lf] * P * P * P;
+    const R *C = down_base + (size_t)leaf_down[lf] * P * PP;
     const R cx = leaf_geom[4 * lf], cy = leaf_geom[4 * lf + 1], cz = leaf_geom[4 * lf + 2], sc = leaf_geom[4 * lf + 3];
     const int beg = leaf_trg[2 * lf], cnt = leaf_trg[2 * lf + 1];
-    for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-        const P4<R> pt = trg[beg + t];
-        R tx[P], ty[P], tz[P];
-        const R x = sc * (pt.x - cx), y = sc * (pt.y - cy), z = sc * (pt.z - cz);
-        tx[0] = ty[0] = tz[0] = R(1);
-        tx[1] = x; ty[1] = y; tz[1] = z;
+    for (int t0 = 0; t0 < cnt; t0 += EV_THREADS * EV_TPT) {
+        R tx[EV_TPT][P], y[EV_TPT], z[EV_TPT], s[EV_TPT], tcm1[EV_TPT], tc[EV_TPT];
 #pragma unroll
-        for (int k = 2; k < P; ++k) {
-            tx[k] = R(2) * x * tx[k - 1] - tx[k - 2];
-            ty[k] = R(2) * y * ty[k - 1] - ty[k - 2];
-            tz[k] = R(2) * z * tz[k - 1] - tz[k - 2];
+        for (int i = 0; i < EV_TPT; ++i) {
+            const int t = t0 + (int)threadIdx.x + EV_THREADS * i;
+            const P4<R> pt = t < cnt ? trg[beg + t] : P4<R>{cx, cy, cz, R(0)};
+            const R x = sc * (pt.x - cx);
+            y[i] = sc * (pt.y - cy);
+            z[i] = sc * (pt.z - cz);
+            tx[i][0] = R(1);
+            if (P > 1) tx[i][1] = x;
+#pragma unroll
+            for (int k = 2; k < P; ++k) tx[i][k] = R(2) * x * tx[i][k - 1] - tx[i][k - 2];
+            s[i] = R(0);
+            tcm1[i] = R(1);
+            tc[i] = R(1);
         }
-        R s = 0;
         for (int c = 0; c < P; ++c) {
-            R sb = 0;
-            for (int b = 0; b < P; ++b) {
-                const R *row = C + P * b + P * P * c;
-                R sa = 0;
+            __syncthreads();
+            for (int e = threadIdx.x; e < PP; e += EV_THREADS) plane[e] = C[(size_t)c * PP + e];
+            __syncthreads();
 #pragma unroll
-                for (int a = 0; a < P; ++a) sa = fma(row[a], tx[a], sa);
-                sb = fma(sa, ty[b], sb);
+            for (int i = 0; i < EV_TPT; ++i) {
+                if (c == 1) {
+                    tc[i] = z[i];
+                } else if (c > 1) {
+                    const R tn = R(2) * z[i] * tc[i] - tcm1[i];
+                    tcm1[i] = tc[i];
+                    tc[i] = tn;
+                }
             }
-            s = fma(sb, tz[c], s);
+            R sb[EV_TPT], tbm1[EV_TPT], tb[EV_TPT];
+#pragma unroll
+            for (int i = 0; i < EV_TPT; ++i) { sb[i] = R(0); tbm1[i] = R(1); tb[i] = R(1); }
+            for (int b = 0; b < P; ++b) {
+#pragma unroll
+                for (int i = 0; i < EV_TPT; ++i) {
+                    if (b == 1) {
+                        tb[i] = y[i];
+                    } else if (b > 1) {
+                        const R tn = R(2) * y[i] * tb[i] - tbm1[i];
+                        tbm1[i] = tb[i];
+                        tb[i] = tn;
+                    }
+                }
+                R sa[EV_TPT];
+#pragma unroll
+                for (int i = 0; i < EV_TPT; ++i) sa[i] = R(0);
+#pragma unroll
+                for (int a = 0; a < P; ++a) {
+                    const R cv = plane[b * P + a];
+#pragma unroll
+                    for (int i = 0; i < EV_TPT; ++i) sa[i] = fma(cv, tx[i][a], sa[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < EV_TPT; ++i) sb[i] = fma(sa[i], tb[i], sb[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < EV_TPT; ++i) s[i] = fma(sb[i], tc[i], s[i]);
         }
-        pot_sorted[beg + t] += s;
+#pragma unroll
+        for (int i = 0; i < EV_TPT; ++i) {
+            const int t = t0 + (int)threadIdx.x + EV_THREADS * i;
+            if (t < cnt) pot_sorted[beg + t] += s[i];
+        }
     }
 }
 
@@ -596,19 +665,26 @@ __global__ void k_fill(R *__restrict__ p, R v, int n) {
 // Half-space modes with nonzero weight are grouped in lines of constant (j, k) with i
 // contiguous; each line is cut into chunks of up to FF_CHUNK consecutive i.  Chunk c
 // covers modes chunk_m0[c] .. + chunk_len[c] - 1 with indices (chunk_i0[c] + r, j, k).
-// Structure factors per block of points:
-//   Sp[blk][m] = sum_{points of blk} q e^{-i(i th1 + j th2 + k th3)}.
-// Per-point axis factors e^{-i m th} are built in FP64 by recurrence in LDS.
-constexpr int FF_CHUNK = 4;
+// All far-field arithmetic is FP64.
+constexpr int FF_CHUNK = 8;
 constexpr int FF_TP = 16;  // points per LDS tile
+constexpr int FF_QUAD = 4;  // structure factors per scalar load in the evaluation
 
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+// Partial structure factors per workgroup of points:
+//   Sp[blk][m] = sum_{points of blk} q e^{-i(i th1 + j th2 + k th3)}.
+// Per point tile, q e^{-i m th} for every axis index m is tabulated in LDS (FP64 recurrence
+// from one sincos per axis).  A lane owns one chunk: per point it gathers its first mode's
+// factor from the three tables and walks the chunk by multiplying with e^{-i th1}.
 __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__ theta, const double *__restrict__ q, int n,
                                                        int ms, int mt, int mu, const int *__restrict__ chunk_ijk,
                                                        const int *__restrict__ chunk_m0, const int *__restrict__ chunk_len,
                                                        int n_chunks, int n_modes, double2 *__restrict__ Sp) {
     extern __shared__ double2 ff_lds[];
-    const int ni = ms, nj = mt, nk = mu;  // index ranges [-m/2, (m-1)/2]
-    double2 *E1 = ff_lds, *E2 = E1 + FF_TP * ni, *E3 = E2 + FF_TP * nj;
+    double2 *E1 = ff_lds, *E2 = E1 + FF_TP * ms, *E3 = E2 + FF_TP * mt, *Est = E3 + FF_TP * mu;
     const int c = blockIdx.y * blockDim.x + threadIdx.x;
     const bool active = c < n_chunks;
     const int ci = active ? chunk_ijk[3 * c] + ms / 2 : 0;
@@ -622,31 +698,33 @@ __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__
         __syncthreads();
         for (int t = threadIdx.x; t < FF_TP * 3; t += blockDim.x) {
             const int pp = t % FF_TP, ax = t / FF_TP, gp = p0 + pp;
-            const int cnt = ax == 0 ? ni : (ax == 1 ? nj : nk);
-            const int lo = -((ax == 0 ? ms : (ax == 1 ? mt : mu)) / 2);
+            const int cnt = ax == 0 ? ms : (ax == 1 ? mt : mu);
+            const int lo = -(cnt / 2);
             double2 *row = (ax == 0 ? E1 : (ax == 1 ? E2 : E3)) + pp * cnt;
             const double th = gp < n ? theta[3 * (size_t)gp + ax] : 0.0;
             const double qq = gp < n ? (ax == 2 ? q[gp] : 1.0) : 0.0;
             double s1, c1, sl, cl;
             sincos(th, &s1, &c1);
             sincos(-th * lo, &sl, &cl);
+            const double2 step = make_double2(c1, -s1);  // e^{-i th}
+            if (ax == 0) Est[pp] = step;
             double2 cur = make_double2(cl * qq, sl * qq);  // q e^{-i lo th}
             for (int m = 0; m < cnt; ++m) {
                 row[m] = cur;
-                cur = make_double2(cur.x * c1 + cur.y * s1, cur.y * c1 - cur.x * s1);  // x e^{-i th}
+                cur = cmul(cur, step);
             }
         }
         __syncthreads();
         if (active) {
             const int np = min(FF_TP, n - p0);
             for (int pp = 0; pp < np; ++pp) {
-                const double2 a = E2[pp * nj + cj], b = E3[pp * nk + ck];
-                const double2 base = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+                const double2 st = Est[pp];
+                double2 cur = cmul(cmul(E1[pp * ms + ci], E2[pp * mt + cj]), E3[pp * mu + ck]);
 #pragma unroll
                 for (int r = 0; r < FF_CHUNK; ++r) {
-                    const double2 e = E1[pp * ni + min(ci + r, ni - 1)];
-                    acc[r].x = fma(e.x, base.x, fma(-e.y, base.y, acc[r].x));
-                    acc[r].y = fma(e.x, base.y, fma(e.y, base.x, acc[r].y));
+                    acc[r].x += cur.x;
+                    acc[r].y += cur.y;
+                    cur = cmul(cur, st);
                 }
             }
         }
@@ -659,45 +737,89 @@ __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__
     }
 }
 
-// S[m] = w[m] * sum_blk Sp[blk][m]
-__global__ void k_ff_reduce(const double2 *__restrict__ Sp, int nblk, const double *__restrict__ w, int n_modes,
-                            double2 *__restrict__ S) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n_modes) return;
+// S[pidx[m]] = w[m] * sum_blk Sp[blk][m]: workgroup = 32 modes x 8 partial-sum lanes.
+__global__ __launch_bounds__(256) void k_ff_reduce(const double2 *__restrict__ Sp, int nblk, const double *__restrict__ w,
+                                                   const int *__restrict__ pidx, int n_modes, double2 *__restrict__ S) {
+    __shared__ double2 part[8][32];
+    const int ml = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int m = blockIdx.x * 32 + ml;
     double sr = 0, si = 0;
-    for (int b = 0; b < nblk; ++b) {
-        const double2 v = Sp[(size_t)b * n_modes + m];
-        sr += v.x;
-        si += v.y;
+    if (m < n_modes)
+        for (int b = g; b < nblk; b += 8) {
+            const double2 v = Sp[(size_t)b * n_modes + m];
+            sr += v.x;
+            si += v.y;
+        }
+    part[g][ml] = make_double2(sr, si);
+    __syncthreads();
+    if (g == 0 && m < n_modes) {
+        double tr = 0, ti = 0;
+        for (int k = 0; k < 8; ++k) {
+            tr += part[k][ml].x;
+            ti += part[k][ml].y;
+        }
+        S[pidx[m]] = make_double2(tr * w[m], ti * w[m]);
     }
-    S[m] = make_double2(sr * w[m], si * w[m]);
 }
 
-// far[u] = sum_m Re( S[m] e^{+i(i th1 + j th2 + k th3)} )
-__global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ theta, int n, const int *__restrict__ line_jk,
-                                                 const int *__restrict__ line_i, const int *__restrict__ line_ptr,
-                                                 int n_lines, const double2 *__restrict__ S, double *__restrict__ far) {
+// e *= z^d for a wave-uniform exponent d by square-and-multiply (|z| = 1, z^-1 = conj z).
+__device__ __forceinline__ void ff_advance(double2 &e, int d, double2 z) {
+    if (d < 0) {
+        z.y = -z.y;
+        d = -d;
+    }
+    while (d) {
+        if (d & 1) e = cmul(e, z);
+        d >>= 1;
+        if (d) z = cmul(z, z);
+    }
+}
+
+// far[u] = sum_m Re( S[m] e^{+i(i th1 + j th2 + k th3)} ), one point per lane.  Line L =
+// (j, k, ilo, nq) holds its weighted S at S[base[L] ..], padded to nq * FF_QUAD entries; its
+// sum over i is a Horner polynomial in z1 = e^{i th1} with wave-uniform coefficients (read
+// FF_QUAD at a time), multiplied by z1^ilo z2^j z3^k, which is advanced from the previous
+// line by (small) exponent steps.
+__global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ theta, int n, const int4 *__restrict__ lines,
+                                                 const int *__restrict__ base, int n_lines, const double2 *__restrict__ S,
+                                                 double *__restrict__ far) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
-    const double t1 = theta[3 * (size_t)u], t2 = theta[3 * (size_t)u + 1], t3 = theta[3 * (size_t)u + 2];
-    double s1, c1;
-    sincos(t1, &s1, &c1);
+    const int uc = u < n ? u : n - 1;
+    double2 z1, z2, z3;
+    {
+        double s, c;
+        sincos(theta[3 * (size_t)uc], &s, &c);
+        z1 = make_double2(c, s);
+        sincos(theta[3 * (size_t)uc + 1], &s, &c);
+        z2 = make_double2(c, s);
+        sincos(theta[3 * (size_t)uc + 2], &s, &c);
+        z3 = make_double2(c, s);
+    }
+    double2 e1 = make_double2(1, 0), e2 = make_double2(1, 0), e3 = make_double2(1, 0);
+    int ic = 0, jc = 0, kc = 0;
     double acc = 0;
     for (int L = 0; L < n_lines; ++L) {
-        const int j = line_jk[2 * L], k = line_jk[2 * L + 1], ilo = line_i[2 * L], ihi = line_i[2 * L + 1];
-        double sb, cb;
-        sincos(ilo * t1 + j * t2 + k * t3, &sb, &cb);
-        double br = cb, bi = sb;
-        const double2 *Sl = S + line_ptr[L];
-        for (int i = 0; i <= ihi - ilo; ++i) {
-            const double2 sv = Sl[i];
-            acc += sv.x * br - sv.y * bi;
-            const double nr = br * c1 - bi * s1;
-            bi = br * s1 + bi * c1;
-            br = nr;
+        const int4 d = lines[L];
+        const double2 *Sl = S + base[L];
+        ff_advance(e1, d.z - ic, z1);
+        ff_advance(e2, d.x - jc, z2);
+        ff_advance(e3, d.y - kc, z3);
+        ic = d.z;
+        jc = d.x;
+        kc = d.y;
+        double2 h = make_double2(0, 0);
+        for (int qd = d.w - 1; qd >= 0; --qd) {
+            double2 sv[FF_QUAD];
+#pragma unroll
+            for (int r = 0; r < FF_QUAD; ++r) sv[r] = Sl[FF_QUAD * qd + r];
+#pragma unroll
+            for (int r = FF_QUAD - 1; r >= 0; --r)
+                h = make_double2(fma(h.x, z1.x, fma(-h.y, z1.y, sv[r].x)), fma(h.x, z1.y, fma(h.y, z1.x, sv[r].y)));
         }
+        const double2 bs = cmul(cmul(e1, e2), e3);
+        acc = fma(h.x, bs.x, fma(-h.y, bs.y, acc));
     }
-    far[u] = acc;
+    if (u < n) far[u] = acc;
 }
 
 }  // namespace pdmk::gpu

@@ -215,20 +215,36 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         work.transform_flops += 6.0 * std::pow((double)p, 4) * (double)in.size();
         return L;
     };
+    // child -> parent: every child proxy is transformed on its own into a scratch slot, then
+    // each parent adds its children's slots (in child order) to the proxy it already holds
+    int c2p_max = 0;
     for (int l = nlevels - 1; l >= 0; --l) {
-        std::vector<int> out, add, ptr{0}, in, code;
+        std::vector<int> out, add, ptr{0}, in, code, pslot, cptr{0};
         for (int b : t.by_level[l]) {
             if (!t.proxy_from_children[b]) continue;
-            out.push_back(up_of[b]);
-            add.push_back(1);
+            pslot.push_back(up_of[b]);
             for (int k = 0; k < 8; ++k) {
                 const int ch = t.first_child[b] + k;
-                if (t.pwexp[ch] && t.src_count[ch] > 0) { in.push_back(up_of[ch]); code.push_back(k); }
+                if (t.pwexp[ch] && t.src_count[ch] > 0) {
+                    out.push_back((int)in.size());
+                    add.push_back(0);
+                    in.push_back(up_of[ch]);
+                    code.push_back(k);
+                    ptr.push_back((int)in.size());
+                }
             }
-            ptr.push_back((int)in.size());
+            cptr.push_back((int)in.size());
         }
-        if (!out.empty()) c2p_levels.push_back(make_t3(out, add, ptr, in, code));
+        if (pslot.empty()) continue;
+        C2pLevel C;
+        C.children = make_t3(out, add, ptr, in, code);
+        C.n_parents = (int)pslot.size();
+        C.parent_slot.upload(pslot);
+        C.child_ptr.upload(cptr);
+        c2p_max = std::max(c2p_max, (int)in.size());
+        c2p_levels.push_back(std::move(C));
     }
+    c2p_scratch.alloc(sizeof(Real) * P3 * (size_t)std::max(1, c2p_max));
     {
         std::vector<Real> m(2 * (size_t)p * p);
         for (int s = 0; s < 2; ++s)
@@ -451,12 +467,25 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         ff_chunk_ijk.upload(cijk);
         ff_chunk_m0.upload(cm0);
         ff_chunk_len.upload(clen);
-        ff_line_jk.upload(ljk);
-        ff_line_i.upload(li);
-        ff_line_ptr.upload(lptr);
-        ff_blocks = std::max(1, std::min(512, (n + FF_TP - 1) / FF_TP));
+        // weighted structure factors per line, padded at the high-i end to a multiple of
+        // FF_QUAD modes (zeros), so the evaluation reads them FF_QUAD at a time
+        std::vector<int4> lines;
+        std::vector<int> lbase, pidx(n_ff_modes);
+        int npad = 0;
+        for (size_t L = 0; L < lptr.size(); ++L) {
+            const int len = li[2 * L + 1] - li[2 * L] + 1, nq = (len + FF_QUAD - 1) / FF_QUAD;
+            lines.push_back(make_int4(ljk[2 * L], ljk[2 * L + 1], li[2 * L], nq));
+            lbase.push_back(npad);
+            for (int r = 0; r < len; ++r) pidx[lptr[L] + r] = npad + r;
+            npad += nq * FF_QUAD;
+        }
+        ff_lines.upload(lines);
+        ff_line_base.upload(lbase);
+        ff_pidx.upload(pidx);
+        ff_blocks = std::max(1, std::min(2048, (n + FF_TP - 1) / FF_TP));
         ff_Sp.alloc(sizeof(double2) * (size_t)ff_blocks * std::max(1, n_ff_modes));
-        ff_S.alloc(sizeof(double2) * (size_t)std::max(1, n_ff_modes));
+        ff_S.alloc(sizeof(double2) * (size_t)std::max(1, npad));
+        check(hipMemset(ff_S.p, 0, ff_S.bytes), "hipMemset");  // padding stays zero
         work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
     }
@@ -515,6 +544,21 @@ Plan<Real>::~Plan() {
 
 namespace {
 
+// c2p / p2c of one level: output planes are split into slices so that a level with few
+// boxes still spreads over the whole GPU (about 1024 workgroups or more)
+template <class Real, int P, bool TRANSPOSE>
+void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launch &L, const Real *mats, hipStream_t s) {
+    if (L.n_items == 0) return;
+    constexpr int KC = T3Cfg<Real, P>::KC;
+    int kc = KC;
+    while (kc > 1 && (long)L.n_items * ((P + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
+    const dim3 grid(L.n_items, (P + kc - 1) / kc);
+    hipLaunchKernelGGL((k_tensor3<Real, P, TRANSPOSE>), grid, dim3(T3Cfg<Real, P>::threads), 0, s, in, out,
+                       L.out.template as<int>(), L.add.template as<int>(), L.ptr.template as<int>(),
+                       L.in_slot.template as<int>(), L.in_code.template as<int>(), mats, L.n_items, kc);
+    check(hipGetLastError(), "k_tensor3");
+}
+
 template <class Real, int P, int NPW>
 void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     using C = Cx<Real>;
@@ -526,12 +570,13 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
                            pl.proxy_up.template as<Real>());
         check(hipGetLastError(), "k_charge2proxy");
     }
-    for (auto &L : pl.c2p_levels) {
-        hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(L.n_items), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
-                           pl.proxy_up.template as<Real>(), L.out.template as<int>(), L.add.template as<int>(),
-                           L.ptr.template as<int>(), L.in_slot.template as<int>(), L.in_code.template as<int>(),
-                           pl.mats.template as<Real>(), 1, L.n_items);
-        check(hipGetLastError(), "k_tensor3 c2p");
+    for (auto &C : pl.c2p_levels) {
+        launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(), C.children,
+                                      pl.mats.template as<Real>(), s);
+        hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256), dim3(256), 0, s,
+                           pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(),
+                           C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents);
+        check(hipGetLastError(), "k_sum_children");
     }
     if (pl.n_up > 0) {
         hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(pl.n_up), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
@@ -554,16 +599,11 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
             check(hipGetLastError(), "k_pw2proxy");
         }
         auto &L = pl.p2c_levels[l];
-        if (L.n_items > 0) {
-            hipLaunchKernelGGL((k_tensor3<Real, P>), dim3(L.n_items), dim3(256), 0, s, pl.proxy_down.template as<Real>(),
-                               pl.proxy_down.template as<Real>(), L.out.template as<int>(), L.add.template as<int>(),
-                               L.ptr.template as<int>(), L.in_slot.template as<int>(), L.in_code.template as<int>(),
-                               pl.mats.template as<Real>(), 0, L.n_items);
-            check(hipGetLastError(), "k_tensor3 p2c");
-        }
+        launch_tensor3<Real, P, false>(pl.proxy_down.template as<Real>(), pl.proxy_down.template as<Real>(), L,
+                                       pl.mats.template as<Real>(), s);
     }
     if (pl.n_leaf_eval > 0) {
-        hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(EV_THREADS), 0, s,
                            pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
                            pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(), pl.trg.template as<P4<Real>>(),
                            pl.pot_sorted.template as<Real>());
@@ -590,18 +630,19 @@ void launch_far(Plan<Real> &pl, const double *d_q, hipStream_t s) {
         check(hipGetLastError(), "k_fill far");
         return;
     }
-    const size_t lds = sizeof(double2) * FF_TP * (pl.ms + pl.mt + pl.mu);
+    const size_t lds = sizeof(double2) * FF_TP * (pl.ms + pl.mt + pl.mu + 1);
     dim3 grid(pl.ff_blocks, (pl.n_ff_chunks + 255) / 256);
     hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, pl.theta.template as<double>(), d_q, pl.n, pl.ms, pl.mt,
                        pl.mu, pl.ff_chunk_ijk.template as<int>(), pl.ff_chunk_m0.template as<int>(),
                        pl.ff_chunk_len.template as<int>(), pl.n_ff_chunks, pl.n_ff_modes, pl.ff_Sp.template as<double2>());
     check(hipGetLastError(), "k_ff_structure");
-    hipLaunchKernelGGL(k_ff_reduce, dim3((pl.n_ff_modes + 255) / 256), dim3(256), 0, s, pl.ff_Sp.template as<double2>(),
-                       pl.ff_blocks, pl.ff_w.template as<double>(), pl.n_ff_modes, pl.ff_S.template as<double2>());
+    hipLaunchKernelGGL(k_ff_reduce, dim3((pl.n_ff_modes + 31) / 32), dim3(256), 0, s, pl.ff_Sp.template as<double2>(),
+                       pl.ff_blocks, pl.ff_w.template as<double>(), pl.ff_pidx.template as<int>(), pl.n_ff_modes,
+                       pl.ff_S.template as<double2>());
     check(hipGetLastError(), "k_ff_reduce");
     hipLaunchKernelGGL(k_ff_eval, dim3((pl.n + 255) / 256), dim3(256), 0, s, pl.theta.template as<double>(), pl.n,
-                       pl.ff_line_jk.template as<int>(), pl.ff_line_i.template as<int>(), pl.ff_line_ptr.template as<int>(),
-                       pl.n_ff_lines, pl.ff_S.template as<double2>(), pl.far.template as<double>());
+                       pl.ff_lines.template as<int4>(), pl.ff_line_base.template as<int>(), pl.n_ff_lines,
+                       pl.ff_S.template as<double2>(), pl.far.template as<double>());
     check(hipGetLastError(), "k_ff_eval");
 }
 

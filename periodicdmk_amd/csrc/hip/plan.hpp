@@ -66,7 +66,11 @@ struct Plan {
     // upward
     DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, rng_off, proxy_up;
     struct Tensor3Launch { int n_items; DevBuf out, add, ptr, in_slot, in_code; };
-    std::vector<Tensor3Launch> c2p_levels;  // deepest first
+    // child -> parent per level (deepest first): per-child transforms into c2p_scratch, then
+    // parent_slot[i] += sum of scratch slots child_ptr[i] .. child_ptr[i+1]
+    struct C2pLevel { Tensor3Launch children; int n_parents = 0; DevBuf parent_slot, child_ptr; };
+    std::vector<C2pLevel> c2p_levels;
+    DevBuf c2p_scratch;
     DevBuf mats;                            // p2c minus / plus (p x p each)
     // plane waves
     DevBuf pw_out, poly2pw, pw2poly, shift0;
@@ -81,7 +85,7 @@ struct Plan {
     int n_nf_leaves = 0, n_nf_items = 0;
     DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs;
     // far field
-    DevBuf theta, ff_w, ff_chunk_ijk, ff_chunk_m0, ff_chunk_len, ff_line_jk, ff_line_i, ff_line_ptr, ff_Sp, ff_S;
+    DevBuf theta, ff_w, ff_chunk_ijk, ff_chunk_m0, ff_chunk_len, ff_lines, ff_line_base, ff_pidx, ff_Sp, ff_S;
     // incoming plane waves of up to pin_batch boxes of one level
     DevBuf pin;
     int pin_batch = 0;
