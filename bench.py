@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--nz", type=int, default=4)
     ap.add_argument("--paths", default="mixed6,fp64_6,fp64_9", help="comma list of mixed6, fp64_6, fp64_9")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="only run 4 device-to-device copies of MB MiB (counter calibration for the traffic figure)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -86,6 +88,17 @@ def main():
     if P.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        a = P.DeviceArray(nbytes // 8, np.float64)
+        b = P.DeviceArray(nbytes // 8, np.float64)
+        a.upload(np.ones(nbytes // 8))
+        for _ in range(4):
+            P.memcpy_d2d(b.ptr, a.ptr, nbytes)
+        P.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 4}))
+        return
 
     n = args.n
     cell = W.cubic_cell()

@@ -107,6 +107,12 @@ def synchronize() -> None:
         raise RuntimeError("hipDeviceSynchronize failed")
 
 
+def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device copy of nbytes between two device addresses."""
+    if lib().pdmk_gpu_memcpy(C.c_void_p(dst), C.c_void_p(src), int(nbytes), 3) != 0:
+        raise RuntimeError("device->device copy failed")
+
+
 class DeviceArray:
     """A 1-D device buffer allocated through libpdmk.so (one HIP runtime per process)."""
 

@@ -27,6 +27,15 @@ struct alignas(4 * sizeof(R)) P4 { R x, y, z, w; };
 template <class R>
 struct alignas(2 * sizeof(R)) Cx { R re, im; };
 
+// Workgroups are dispatched to the 8 XCDs round-robin; this bijection of [0, nb) gives each
+// XCD one contiguous range of workgroup indices, so work items that share inputs (items are
+// laid out in tree order, i.e. spatially) run under the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int X = 8;
+    const int x = b % X, i = b / X, q = nb / X, r = nb % X;
+    return x * q + min(x, r) + i;
+}
+
 template <class R> __device__ __forceinline__ R rsqrt_full(R x);
 template <> __device__ __forceinline__ float rsqrt_full<float>(float x) { return __frsqrt_rn(x); }
 template <> __device__ __forceinline__ double rsqrt_full<double>(double x) { return rsqrt(x); }
@@ -573,7 +582,7 @@ __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ sr
                                                      const P4<R> *__restrict__ geom, const P4<R> *__restrict__ cbox,
                                                      const R *__restrict__ coeffs, R *__restrict__ pot_sorted, int n_items) {
     // wave index made wave-uniform explicitly, so everything derived from it lives in SGPRs
-    const int item = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (item >= n_items) return;  // whole wavefront
     const int lane = threadIdx.x & 63;
     const int4 it = items[item];  // (first target, count, target-leaf list index, -)
