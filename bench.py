@@ -11,7 +11,8 @@ Three paths are timed in one run:
   * fp64_6  -- n_digits = 6, plain FP64 path;
   * fp64_9  -- n_digits = 9, plain FP64 path.
 The CPU baseline is the CPU oracle (a C++/OpenMP restatement of the reference
-evaluate()) timed on this host on the same workload.
+evaluate()) timed on this host on the same workload.  Errors are reported against that
+oracle (all N targets) and against the FP64 Ewald sum at 1000 sampled targets.
 
 Prints ONE JSON line.  ``--gpus`` above 1 is not supported (single-GPU build).
 """
@@ -30,6 +31,7 @@ METRIC = "evaluate() Mpts/s @N=1e6, 6 & 9 digits: tuned mixed vs plain FP64 GPU"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+EWALD_TARGETS = 1000
 
 
 def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, charges_dev, n_vec):
@@ -76,8 +78,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only, "
-              "see DESIGN.md)", file=sys.stderr)
+        print("bench.py: this solver runs on one GPU; --gpus must be 1 (multi-GPU sharding is not implemented)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -144,6 +146,20 @@ def main():
             d = pots[nm] - r
             results[nm]["err_vs_cpu_oracle_rel_l2"] = float(np.linalg.norm(d) / np.linalg.norm(r))
             results[nm]["err_vs_cpu_oracle_max_rel"] = float(np.max(np.abs(d)) / np.max(np.abs(r)))
+        # FP64 Ewald sum at EWALD_TARGETS sampled targets (splitting parameter at its lower
+        # bound, which keeps the reciprocal sum small; s = 5.5 truncates below 1e-12)
+        print(f"[bench] CPU Ewald oracle at {EWALD_TARGETS} sampled targets", file=sys.stderr, flush=True)
+        tg = np.sort(np.random.default_rng(5).choice(n, min(n, EWALD_TARGETS), replace=False)).astype(np.int32)
+        qi = (args.steps - 1) % 8
+        t0 = time.perf_counter()
+        ew = O.ewald_cells(cell, pos, qs[qi], s=5.5, targets=tg, alpha=1e-9)
+        ew_s = time.perf_counter() - t0
+        for nm in wanted:
+            d = pots[nm][tg] - ew
+            results[nm]["err_vs_ewald_rel_l2"] = float(np.linalg.norm(d) / np.linalg.norm(ew))
+            results[nm]["err_vs_ewald_max_rel"] = float(np.max(np.abs(d)) / np.max(np.abs(ew)))
+        if cpu is not None:
+            cpu["ewald_reference"] = f"FP64 Ewald (s=5.5) at {len(tg)} sampled targets, {ew_s:.1f} s"
 
     head = results.get("mixed6") or next(iter(results.values()))
     w = head["work"]

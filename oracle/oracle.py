@@ -55,7 +55,7 @@ def lib():
         L.oracle_beta.argtypes = [C.c_double]
         L.oracle_cheb_tables.argtypes = [C.c_int, dp, dp, dp]
         L.oracle_ewald_direct.argtypes = [dp, C.c_int, dp, dp, C.c_double, dp, dp]
-        L.oracle_ewald_cells.argtypes = [dp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp]
+        L.oracle_ewald_cells.argtypes = [dp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp, C.c_double]
         _lib = L
     return _lib
 
@@ -171,7 +171,9 @@ def ewald_direct(cell, positions, charges, s=4.8):
     return pot, float(e[0])
 
 
-def ewald_cells(cell, positions, charges, s=5.0, targets=None):
+def ewald_cells(cell, positions, charges, s=5.0, targets=None, alpha=0.0):
+    """FP64 Ewald potentials at `targets` (all points if None); alpha > 0 overrides the
+    splitting parameter (bounded below so that r_cut < h_min / 2)."""
     cellc = np.ascontiguousarray(np.asarray(cell, dtype=np.float64).reshape(3, 3).T)
     pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
     q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
@@ -181,6 +183,6 @@ def ewald_cells(cell, positions, charges, s=5.0, targets=None):
         tidx = np.ascontiguousarray(np.asarray(targets, dtype=np.int32))
     pot = np.zeros(tidx.shape[0])
     if lib().oracle_ewald_cells(_d(cellc), pos.shape[0], _d(pos), _d(q), float(s), tidx.shape[0], _i(tidx),
-                                _d(pot)) != 0:
+                                _d(pot), float(alpha)) != 0:
         raise RuntimeError("ewald_cells failed")
     return pot

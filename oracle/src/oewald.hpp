@@ -114,14 +114,16 @@ inline void ewald_direct(const double *cell, int N, const double *pos, const dou
     if (energy) *energy = 0.5 * e - M_PI * Q * Q / (2 * V * alpha * alpha);
 }
 
-// Potentials at targets tidx[0..M) (or all when tidx == nullptr and M == N).
+// Potentials at targets tidx[0..M) (or all when tidx == nullptr and M == N).  alpha_in > 0
+// overrides the splitting parameter (the sum does not depend on it; it moves work between
+// the real-space and reciprocal parts), still bounded so that r_cut < hmin/2.
 inline void ewald_cells(const double *cell, int N, const double *pos, const double *q, double s, int M,
-                        const int *tidx, double *pot) {
+                        const int *tidx, double *pot, double alpha_in = 0) {
     Cell C(cell);
     const double V = C.vol;
     const double hmin = std::min(C.h[0], std::min(C.h[1], C.h[2]));
     // density rule with ~400 neighbours, bounded so that r_cut < hmin/2
-    double alpha = s * std::cbrt(4 * M_PI * (N / V) / (3 * 400.0));
+    double alpha = alpha_in > 0 ? alpha_in : s * std::cbrt(4 * M_PI * (N / V) / (3 * 400.0));
     alpha = std::max(alpha, 2.05 * s / hmin);
     const double rcut = s / alpha, kmax = 2 * s * alpha;
     // fractional coordinates wrapped to [0,1)

@@ -166,8 +166,8 @@ int oracle_ewald_direct(const double *cell, int N, const double *pos, const doub
     catch (const std::exception &e) { std::fprintf(stderr, "oracle_ewald_direct: %s\n", e.what()); return -1; }
 }
 int oracle_ewald_cells(const double *cell, int N, const double *pos, const double *q, double s, int M,
-                       const int *tidx, double *pot) {
-    try { ewald_cells(cell, N, pos, q, s, M, tidx, pot); return 0; }
+                       const int *tidx, double *pot, double alpha) {
+    try { ewald_cells(cell, N, pos, q, s, M, tidx, pot, alpha); return 0; }
     catch (const std::exception &e) { std::fprintf(stderr, "oracle_ewald_cells: %s\n", e.what()); return -1; }
 }
 
