@@ -107,6 +107,22 @@ def synchronize() -> None:
         raise RuntimeError("hipDeviceSynchronize failed")
 
 
+def optimize_cubic_cover(cell, nz: int = 4) -> dict:
+    """Cubic cover of a cell (pdmk_optimize_cubic_cover): m, n, p, orthonormal frame e1, e2,
+    e3, cube side s and overhead cost.  Raises ValueError for nz <= 0 or a degenerate cell."""
+    cellc = np.ascontiguousarray(np.asarray(cell, dtype=np.float64).reshape(3, 3).T)  # column-major
+    m, n, p = C.c_int(), C.c_int(), C.c_int()
+    e = [np.zeros(3) for _ in range(3)]
+    s, cost = C.c_double(), C.c_double()
+    dp = C.POINTER(C.c_double)
+    rc = lib().pdmk_optimize_cubic_cover(cellc.ctypes.data_as(dp), int(nz), C.byref(m), C.byref(n), C.byref(p),
+                                         *(v.ctypes.data_as(dp) for v in e), C.byref(s), C.byref(cost))
+    if rc != 0:
+        raise ValueError(f"optimize_cubic_cover failed (code {rc})")
+    return {"m": m.value, "n": n.value, "p": p.value, "e1": e[0], "e2": e[1], "e3": e[2], "s": s.value,
+            "cost": cost.value}
+
+
 def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
     """Device-to-device copy of nbytes between two device addresses."""
     if lib().pdmk_gpu_memcpy(C.c_void_p(dst), C.c_void_p(src), int(nbytes), 3) != 0:

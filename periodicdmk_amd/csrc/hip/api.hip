@@ -9,6 +9,7 @@
 #include <pdmk/periodic_dmk.hpp>
 
 #include "plan.hpp"
+#include "../host/tree_build.hpp"
 
 #include <cstdio>
 #include <cstring>
@@ -179,11 +180,28 @@ int pdmk_ewald_nufft(const double *, int, const double *, const double *, double
     return -1;
 }
 
-int pdmk_optimize_cubic_cover(const double *, int nz, int *, int *, int *, double *, double *, double *, double *,
-                              double *) {
+int pdmk_optimize_cubic_cover(const double *cell, int nz, int *m, int *n, int *p, double *e1, double *e2, double *e3,
+                              double *s, double *cost) {
     if (nz <= 0) return 1;
-    std::fprintf(stderr, "pdmk_optimize_cubic_cover: not provided by this build\n");
-    return -1;
+    try {
+        if (!cell || !m || !n || !p || !e1 || !e2 || !e3 || !s || !cost)
+            throw std::invalid_argument("null argument");
+        const pdmk::host::CubicCover c = pdmk::host::cubic_cover(cell, nz);
+        *m = c.m;
+        *n = c.n;
+        *p = c.p;
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = c.e1[d];
+            e2[d] = c.e2[d];
+            e3[d] = c.e3[d];
+        }
+        *s = c.s;
+        *cost = c.cost;
+        return 0;
+    } catch (const std::exception &ex) {
+        std::fprintf(stderr, "pdmk_optimize_cubic_cover: %s\n", ex.what());
+        return -1;
+    }
 }
 
 void *pdmk_create_double(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz) {

@@ -385,4 +385,42 @@ Tree build_tree(const Geometry &g, int n_per_leaf) {
     return t;
 }
 
+CubicCover cubic_cover(const double *cell, int nz) {
+    if (nz <= 0) throw std::invalid_argument("cubic cover nz must be positive");
+    const double *a[3] = {cell, cell + 3, cell + 6};  // lattice vectors = columns
+    auto dot3 = [](const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[1][0] * (a[0][1] * a[2][2] - a[0][2] * a[2][1]) +
+                       a[2][0] * (a[0][1] * a[1][2] - a[0][2] * a[1][1]);
+    if (std::fabs(det) < 1e-30) throw std::invalid_argument("cubic cover: singular cell");
+    // longest and second-longest vectors; on equal norms the lower index ranks first
+    int ord[3] = {0, 1, 2};
+    double nrm[3];
+    for (int j = 0; j < 3; ++j) nrm[j] = std::sqrt(dot3(a[j], a[j]));
+    std::stable_sort(ord, ord + 3, [&](int x, int y) { return nrm[x] > nrm[y]; });
+    CubicCover c;
+    const double *b1 = a[ord[0]], *b2 = a[ord[1]];
+    if (!(nrm[ord[0]] > 0)) throw std::invalid_argument("cubic cover: degenerate lattice (zero-norm vector)");
+    for (int d = 0; d < 3; ++d) c.e1[d] = b1[d] / nrm[ord[0]];
+    const double p21 = dot3(b2, c.e1);
+    double v2[3];
+    for (int d = 0; d < 3; ++d) v2[d] = b2[d] - p21 * c.e1[d];
+    const double nv2 = std::sqrt(dot3(v2, v2));
+    if (!(nv2 > 0)) throw std::invalid_argument("cubic cover: degenerate lattice (parallel vectors)");
+    for (int d = 0; d < 3; ++d) c.e2[d] = v2[d] / nv2;
+    c.e3[0] = c.e1[1] * c.e2[2] - c.e1[2] * c.e2[1];
+    c.e3[1] = c.e1[2] * c.e2[0] - c.e1[0] * c.e2[2];
+    c.e3[2] = c.e1[0] * c.e2[1] - c.e1[1] * c.e2[0];
+    // widths of the cell's bounding box along the frame axes
+    double w[3] = {0, 0, 0};
+    const double *e[3] = {c.e1, c.e2, c.e3};
+    for (int d = 0; d < 3; ++d)
+        for (int j = 0; j < 3; ++j) w[d] += std::fabs(dot3(a[j], e[d]));
+    c.s = w[2] / nz;
+    c.m = std::max(1, (int)std::ceil(w[0] / c.s));
+    c.n = std::max(1, (int)std::ceil(w[1] / c.s));
+    c.p = nz;
+    c.cost = (double)((c.m + 2) * (c.n + 2) * (c.p + 2)) * c.s * c.s * c.s / std::fabs(det);
+    return c;
+}
+
 }  // namespace pdmk::host

@@ -38,6 +38,18 @@ struct Geometry {
 // Lattice frame, cutoff cube size rc = min diag(umat)/nz, rotated positions and images.
 Geometry make_geometry(const double *cell_colmajor, const double *pos, int n, int nz);
 
+// Cover of the cell's bounding box by (m+2) x (n+2) x (p+2) cubes of side s in the frame
+// (e1, e2, e3): e1 along the longest lattice vector, e2 in the plane of the two longest,
+// p = nz layers across the e3 width (optimize_cubic_cover,
+// cpp/src/cubic_cover_optimizer.cpp:36-89).  Throws std::invalid_argument for nz <= 0 or a
+// degenerate cell.
+struct CubicCover {
+    int m = 0, n = 0, p = 0;
+    double e1[3] = {0, 0, 0}, e2[3] = {0, 0, 0}, e3[3] = {0, 0, 0};
+    double s = 0, cost = 0;
+};
+CubicCover cubic_cover(const double *cell_colmajor, int nz);
+
 struct Tree {
     int nboxes = 0, nlevels = 0;
     int mle[3] = {0, 0, 0};

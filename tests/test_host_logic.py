@@ -137,3 +137,52 @@ def test_workload_generator_is_deterministic():
     assert abs(q.sum()) < 1e-12
     qpm = W.pm1_charges(1000, 3)
     assert qpm.sum() == 0 and set(np.unique(qpm)) == {-1.0, 1.0}
+
+
+def _frame_ok(r):
+    E = np.array([r["e1"], r["e2"], r["e3"]])
+    np.testing.assert_allclose(E @ E.T, np.eye(3), atol=1e-12)
+    assert np.linalg.det(E) > 0  # right-handed
+
+
+@pytest.mark.parametrize("cell,nz,expect", [
+    (np.eye(3), 4, (4, 4, 4, 0.25)),                      # cpp/tests/test_cubic_cover_optimizer.cpp:8-30
+    (np.diag([4.0, 2.0, 1.0]), 4, (16, 8, 4, 0.25)),      # :32-45, julia runtests.jl:44-50
+    (np.diag([16.0, 2.0, 2.0]), 4, (32, 4, 4, 0.5)),      # :47-59, runtests.jl:52-58
+    (np.eye(3) * 2.0, 1, (1, 1, 1, 2.0)),                  # :61-72
+])
+def test_optimize_cubic_cover_known_answers(pdmk, cell, nz, expect):
+    r = pdmk.optimize_cubic_cover(cell, nz)
+    assert (r["m"], r["n"], r["p"]) == expect[:3]
+    assert abs(r["s"] - expect[3]) < 1e-12
+    _frame_ok(r)
+    # cost = (m+2)(n+2)(p+2) s^3 / V
+    V = abs(np.linalg.det(cell))
+    assert abs(r["cost"] - (r["m"] + 2) * (r["n"] + 2) * (r["p"] + 2) * r["s"] ** 3 / V) < 1e-12 * r["cost"]
+
+
+def test_optimize_cubic_cover_triclinic_frame(pdmk):
+    # rows are lattice vectors in the reference test (cpp/tests/test_cubic_cover_optimizer.cpp:84-104)
+    cell = np.array([[5.0, 0.0, 0.0], [1.0, 4.5, 0.0], [0.5, 0.8, 4.0]]).T
+    r = pdmk.optimize_cubic_cover(cell, 4)
+    _frame_ok(r)
+    assert r["p"] == 4 and r["m"] >= 1 and r["n"] >= 1 and r["s"] > 0 and np.isfinite(r["cost"])
+    # e1 along the longest lattice vector, e2 in the plane of the two longest
+    a = cell.T
+    order = np.argsort(-np.linalg.norm(a, axis=1), kind="stable")
+    np.testing.assert_allclose(r["e1"], a[order[0]] / np.linalg.norm(a[order[0]]), atol=1e-14)
+    assert abs(np.dot(np.cross(a[order[0]], a[order[1]]), r["e2"])) < 1e-12
+    # p cubes of side s span the e3 width of the cell's bounding box; m, n the smallest covers
+    width = [np.sum(np.abs(a @ r[k])) for k in ("e1", "e2", "e3")]
+    assert abs(r["s"] * 4 - width[2]) < 1e-12
+    assert (r["m"] - 1) * r["s"] < width[0] <= r["m"] * r["s"] + 1e-12
+    assert (r["n"] - 1) * r["s"] < width[1] <= r["n"] * r["s"] + 1e-12
+
+
+def test_optimize_cubic_cover_errors(pdmk):
+    cell = np.array([[1.0, 0.0, 0.0], [0.3, 1.0, 0.0], [0.1, 0.2, 0.8]]).T
+    for nz in (0, -1):  # the C ABI returns 1 (pdmk_capi.cpp:114)
+        with pytest.raises(ValueError, match="code 1"):
+            pdmk.optimize_cubic_cover(cell, nz)
+    with pytest.raises(ValueError, match="code -1"):
+        pdmk.optimize_cubic_cover(np.zeros((3, 3)), 4)
