@@ -23,7 +23,9 @@
 extern "C" {
 #endif
 
-/// FP64 Ewald reference potentials (and energy when energy_out != NULL).
+/// FP64 Ewald reference potentials (and energy when energy_out != NULL), computed on the
+/// GPU: exact truncated real-space and reciprocal sums (r_cut = s/alpha, k_max = 2 s alpha),
+/// with the reference's splitting-parameter convention for non-neutral systems.
 /// Replaces pdmk_ewald_nufft, cpp/include/pdmk/pdmk_capi.h:22-28.
 int pdmk_ewald_nufft(const double* cell_3x3, int N,
                      const double* positions_3xN,

@@ -123,6 +123,20 @@ def optimize_cubic_cover(cell, nz: int = 4) -> dict:
             "cost": cost.value}
 
 
+def ewald(cell, positions, charges, s: float = 5.0):
+    """FP64 Ewald potentials and energy on the GPU (pdmk_ewald_nufft); returns (pot, energy)."""
+    cellc = _cell_colmajor(cell)
+    pos = _positions(positions)
+    q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+    if q.shape != (pos.shape[0],):
+        raise ValueError("charges must have shape (N,)")
+    pot = np.zeros(pos.shape[0])
+    e = C.c_double()
+    if lib().pdmk_ewald_nufft(_d(cellc), pos.shape[0], _d(pos), _d(q), float(s), _d(pot), C.byref(e)) != 0:
+        raise RuntimeError("pdmk_ewald_nufft failed")
+    return pot, e.value
+
+
 def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
     """Device-to-device copy of nbytes between two device addresses."""
     if lib().pdmk_gpu_memcpy(C.c_void_p(dst), C.c_void_p(src), int(nbytes), 3) != 0:

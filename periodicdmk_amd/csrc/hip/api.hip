@@ -8,6 +8,7 @@
 #include <pdmk/pdmk_gpu.h>
 #include <pdmk/periodic_dmk.hpp>
 
+#include "ewald.hpp"
 #include "plan.hpp"
 #include "../host/tree_build.hpp"
 
@@ -175,9 +176,16 @@ int guarded(const char *what, F &&f) {
 
 extern "C" {
 
-int pdmk_ewald_nufft(const double *, int, const double *, const double *, double, double *, double *) {
-    std::fprintf(stderr, "pdmk_ewald_nufft: the FP64 Ewald reference is not provided by this build\n");
-    return -1;
+int pdmk_ewald_nufft(const double *cell, int N, const double *pos, const double *charges, double s, double *pot_out,
+                     double *energy_out) {
+    try {
+        if (!cell || !pos || !charges || !pot_out) throw std::invalid_argument("null argument");
+        pdmk::gpu::ewald_sum(cell, N, pos, charges, s, pot_out, energy_out);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_ewald_nufft: %s\n", e.what());
+        return -1;
+    }
 }
 
 int pdmk_optimize_cubic_cover(const double *cell, int nz, int *m, int *n, int *p, double *e1, double *e2, double *e3,

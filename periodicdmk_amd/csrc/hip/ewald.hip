@@ -1,0 +1,254 @@
+// FP64 Ewald sum on the GPU: the computation behind pdmk_ewald_nufft
+// (ewald_nufft, cpp/src/ewald_nufft.cpp:445-654; CellList, cpp/include/pdmk/cell_list.hpp).
+//
+//   phi_i = sum_{j, images n, 0 < r < r_cut} q_j erfc(alpha r)/r            (real space)
+//         + sum_{k != 0, |k| <= k_max} (4 pi/V) e^{-k^2/4 alpha^2}/k^2 Re(S(k) e^{i k.x_i})
+//         - 2 alpha/sqrt(pi) q_i,
+//   r_cut = s/alpha, k_max = 2 s alpha, S(k) = sum_j q_j e^{-i k.x_j},
+//   energy = 1/2 sum_i q_i phi_i - pi Q^2/(2 V alpha^2).
+//
+// The sum does not depend on alpha at a given s except, for a system with net charge Q, by
+// the uniform shift pi Q/(V alpha^2) that the omitted k = 0 term leaves in the real-space
+// part.  The GPU picks the alpha that balances its real-space and reciprocal work and then
+// adds pi Q/V (1/alpha_ref^2 - 1/alpha^2), alpha_ref being the reference's choose_alpha
+// (cpp/src/ewald_nufft.cpp:40-56), so potentials and energy are the reference's for any Q.
+// The reciprocal part is the exact truncated sum (direct structure factors, FP64), which the
+// reference approximates with a NUFFT at tolerance 1e-12.  Positions are wrapped into the
+// cell first (the periodic sum is invariant under that).
+#include "ewald.hpp"
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <stdexcept>
+#include <vector>
+
+namespace pdmk::gpu {
+
+namespace {
+
+constexpr int EW_THREADS = 256;
+
+// Real-space sum for the targets of one bin (workgroup (bin, chunk of EW_THREADS targets)).
+// Neighbour list of bin b: entries nb_ptr[b] .. nb_ptr[b+1] of (source bin, image shift).
+// Sources of a neighbour bin are read with wave-uniform addresses.
+__global__ __launch_bounds__(EW_THREADS) void k_ewald_real(const double4 *__restrict__ pts, const int *__restrict__ bin_start,
+                                                           const int *__restrict__ nb_ptr, const int *__restrict__ nb_bin,
+                                                           const double4 *__restrict__ nb_shift, double alpha, double rc2,
+                                                           double *__restrict__ out) {
+    const int b = blockIdx.x;
+    const int t = bin_start[b] + blockIdx.y * EW_THREADS + (int)threadIdx.x;
+    const int tend = bin_start[b + 1];
+    if (bin_start[b] + (int)blockIdx.y * EW_THREADS >= tend) return;  // whole workgroup
+    const bool active = t < tend;
+    const double4 me = active ? pts[t] : make_double4(0, 0, 0, 0);
+    double acc = 0;
+    for (int e = nb_ptr[b]; e < nb_ptr[b + 1]; ++e) {
+        const int sb = nb_bin[e];
+        const double4 sh = nb_shift[e];
+        const double tx = me.x - sh.x, ty = me.y - sh.y, tz = me.z - sh.z;  // = r_t - shift
+        for (int j = bin_start[sb]; j < bin_start[sb + 1]; ++j) {
+            const double4 q = pts[j];
+            const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (active && r2 < rc2 && r2 > 1e-30) {
+                const double r = sqrt(r2);
+                acc += q.w * erfc(alpha * r) / r;
+            }
+        }
+    }
+    if (active) out[t] = acc;
+}
+
+// phi[u] = real[sorted(u)] + far[u] - 2 alpha/sqrt(pi) q_u + shift
+__global__ void k_ewald_finalize(const double *__restrict__ real_sorted, const int *__restrict__ sorted_of,
+                                 const double *__restrict__ far, const double *__restrict__ q, double self, double shift,
+                                 double *__restrict__ phi, int n) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    phi[u] = real_sorted[sorted_of[u]] + far[u] - self * q[u] + shift;
+}
+
+struct Lat {
+    double a[3][3];    // a[j] = lattice vector j
+    double inv[3][3];  // fractional f = inv x: f_d = sum_k inv[d][k] x_k
+    double vol, h[3];  // volume and heights
+};
+
+Lat make_lat(const double *cell) {
+    Lat L;
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) L.a[j][i] = cell[i + 3 * j];
+    auto cross = [](const double *u, const double *v, double *w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    double c[3][3];
+    for (int d = 0; d < 3; ++d) cross(L.a[(d + 1) % 3], L.a[(d + 2) % 3], c[d]);
+    const double det = L.a[0][0] * c[0][0] + L.a[0][1] * c[0][1] + L.a[0][2] * c[0][2];
+    if (std::fabs(det) < 1e-30) throw std::invalid_argument("cell matrix is singular");
+    L.vol = std::fabs(det);
+    for (int d = 0; d < 3; ++d) {
+        for (int k = 0; k < 3; ++k) L.inv[d][k] = c[d][k] / det;  // rows of A^{-1} = (a_{d+1} x a_{d+2}) / det
+        L.h[d] = L.vol / std::sqrt(c[d][0] * c[d][0] + c[d][1] * c[d][1] + c[d][2] * c[d][2]);
+    }
+    return L;
+}
+
+}  // namespace
+
+double ewald_reference_alpha(int n, double vol, double s) {
+    const double balanced = std::pow(n * M_PI * M_PI * M_PI / (vol * vol), 1.0 / 6.0);
+    double target_nbr = 5000.0;
+    if (const char *e = std::getenv("DMK_EWALD_TARGET_NBR")) target_nbr = std::atof(e);
+    const double density = s * std::cbrt(4.0 * M_PI * (n / vol) / (3.0 * target_nbr));
+    return std::max(balanced, density);
+}
+
+double ewald_gpu_alpha(int n, double vol) {
+    // real-space work ~ n rho (s/alpha)^3, reciprocal work ~ n V (s alpha)^3: balanced at
+    // alpha^6 = c n / V^2, with c ~ 500 from the FP64 operation counts per pair and per mode
+    return std::pow(500.0 * n / (vol * vol), 1.0 / 6.0);
+}
+
+void ewald_sum(const double *cell, int n, const double *pos, const double *q, double s, double *pot, double *energy) {
+    if (n <= 0) throw std::invalid_argument("N must be positive");
+    if (!(s > 0)) throw std::invalid_argument("s must be positive");
+    const Lat L = make_lat(cell);
+    const double alpha_ref = ewald_reference_alpha(n, L.vol, s);
+    const double alpha = ewald_gpu_alpha(n, L.vol);
+    const double rcut = s / alpha, kmax = 2 * s * alpha;
+
+    // wrapped fractional and Cartesian coordinates
+    std::vector<double> frac(3 * (size_t)n), theta(3 * (size_t)n);
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            double f = L.inv[d][0] * pos[3 * (size_t)i] + L.inv[d][1] * pos[3 * (size_t)i + 1] +
+                       L.inv[d][2] * pos[3 * (size_t)i + 2];
+            f -= std::floor(f);
+            if (f >= 1.0) f = 0.0;
+            frac[3 * (size_t)i + d] = f;
+            theta[3 * (size_t)i + d] = 2 * M_PI * f;
+        }
+
+    // ---- real space: bins of side >= r_cut along each height, counting sort
+    int nb[3], nimg[3];
+    for (int d = 0; d < 3; ++d) {
+        nb[d] = std::max(1, (int)std::floor(L.h[d] / rcut));
+        nimg[d] = nb[d] == 1 ? (int)std::ceil(rcut / L.h[d]) : 1;
+    }
+    const int nbins = nb[0] * nb[1] * nb[2];
+    std::vector<int> bin_of(n), bstart(nbins + 1, 0), sorted_of(n);
+    for (int i = 0; i < n; ++i) {
+        int c[3];
+        for (int d = 0; d < 3; ++d) c[d] = std::min(nb[d] - 1, (int)(frac[3 * (size_t)i + d] * nb[d]));
+        bin_of[i] = (c[2] * nb[1] + c[1]) * nb[0] + c[0];
+        bstart[bin_of[i] + 1]++;
+    }
+    for (int b = 0; b < nbins; ++b) bstart[b + 1] += bstart[b];
+    std::vector<double4> pts(n);
+    {
+        std::vector<int> fill(bstart.begin(), bstart.end() - 1);
+        for (int i = 0; i < n; ++i) {
+            const int k = fill[bin_of[i]]++;
+            sorted_of[i] = k;
+            double x[3];
+            for (int dd = 0; dd < 3; ++dd)
+                x[dd] = L.a[0][dd] * frac[3 * (size_t)i] + L.a[1][dd] * frac[3 * (size_t)i + 1] +
+                        L.a[2][dd] * frac[3 * (size_t)i + 2];
+            pts[k] = make_double4(x[0], x[1], x[2], q[i]);
+        }
+    }
+    // neighbour (bin, image shift) entries: offsets -nimg..nimg per axis, wrapped
+    std::vector<int> nptr{0}, nbin;
+    std::vector<double4> nshift;
+    int max_occ = 0;
+    for (int b = 0; b < nbins; ++b) {
+        max_occ = std::max(max_occ, bstart[b + 1] - bstart[b]);
+        const int c[3] = {b % nb[0], (b / nb[0]) % nb[1], b / (nb[0] * nb[1])};
+        for (int oz = -nimg[2]; oz <= nimg[2]; ++oz)
+            for (int oy = -nimg[1]; oy <= nimg[1]; ++oy)
+                for (int ox = -nimg[0]; ox <= nimg[0]; ++ox) {
+                    const int o[3] = {ox, oy, oz};
+                    int cc[3], sh[3];
+                    for (int d = 0; d < 3; ++d) {
+                        cc[d] = c[d] + o[d];
+                        sh[d] = 0;
+                        while (cc[d] < 0) { cc[d] += nb[d]; --sh[d]; }
+                        while (cc[d] >= nb[d]) { cc[d] -= nb[d]; ++sh[d]; }
+                    }
+                    const int sb = (cc[2] * nb[1] + cc[1]) * nb[0] + cc[0];
+                    if (bstart[sb + 1] == bstart[sb]) continue;
+                    double v[3];
+                    for (int d = 0; d < 3; ++d) v[d] = sh[0] * L.a[0][d] + sh[1] * L.a[1][d] + sh[2] * L.a[2][d];
+                    nbin.push_back(sb);
+                    nshift.push_back(make_double4(v[0], v[1], v[2], 0));
+                }
+        nptr.push_back((int)nbin.size());
+    }
+
+    // ---- reciprocal: k = 2 pi A^{-T} m, |k| <= k_max, m != 0; half space with weight 2 w
+    int mmax[3];
+    for (int d = 0; d < 3; ++d) mmax[d] = (int)std::ceil(kmax * L.h[d] / (2 * M_PI));
+    std::vector<int> mijk;
+    std::vector<double> mw;
+    const double pref = 4 * M_PI / L.vol, inv4a2 = 1 / (4 * alpha * alpha), kmax2 = kmax * kmax;
+    for (int k = 0; k <= mmax[2]; ++k)
+        for (int j = (k == 0 ? 0 : -mmax[1]); j <= mmax[1]; ++j)
+            for (int i = ((k == 0 && j == 0) ? 1 : -mmax[0]); i <= mmax[0]; ++i) {
+                double kv[3];
+                for (int d = 0; d < 3; ++d) kv[d] = 2 * M_PI * (i * L.inv[0][d] + j * L.inv[1][d] + k * L.inv[2][d]);
+                const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                if (k2 > kmax2) continue;
+                mijk.push_back(i);
+                mijk.push_back(j);
+                mijk.push_back(k);
+                mw.push_back(2 * pref * std::exp(-k2 * inv4a2) / k2);
+            }
+
+    // ---- device work
+    hipStream_t st = nullptr;
+    check(hipStreamCreate(&st), "hipStreamCreate");
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() {
+            if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+        }
+    } guard{st};
+    DevBuf d_pts, d_bstart, d_nptr, d_nbin, d_nshift, d_real, d_far, d_q, d_sorted, d_phi;
+    d_pts.upload(pts);
+    d_bstart.upload(bstart);
+    d_nptr.upload(nptr);
+    d_nbin.upload(nbin);
+    d_nshift.upload(nshift);
+    d_sorted.upload(sorted_of);
+    d_q.upload(std::vector<double>(q, q + n));
+    d_real.alloc(sizeof(double) * (size_t)n);
+    d_far.alloc(sizeof(double) * (size_t)n);
+    d_phi.alloc(sizeof(double) * (size_t)n);
+    FourierSum fs;
+    fs.setup(theta, n, 2 * mmax[0] + 1, 2 * mmax[1] + 1, 2 * mmax[2] + 1, mijk, mw);
+    fs.run(d_q.as<double>(), d_far.as<double>(), st);
+    const dim3 grid(nbins, (max_occ + EW_THREADS - 1) / EW_THREADS);
+    hipLaunchKernelGGL(k_ewald_real, grid, dim3(EW_THREADS), 0, st, d_pts.as<double4>(), d_bstart.as<int>(),
+                       d_nptr.as<int>(), d_nbin.as<int>(), d_nshift.as<double4>(), alpha, rcut * rcut,
+                       d_real.as<double>());
+    check(hipGetLastError(), "k_ewald_real");
+    double Q = 0;
+    for (int i = 0; i < n; ++i) Q += q[i];
+    const double shift = M_PI * Q / L.vol * (1 / (alpha_ref * alpha_ref) - 1 / (alpha * alpha));
+    hipLaunchKernelGGL(k_ewald_finalize, dim3((n + 255) / 256), dim3(256), 0, st, d_real.as<double>(), d_sorted.as<int>(),
+                       d_far.as<double>(), d_q.as<double>(), 2 * alpha / std::sqrt(M_PI), shift, d_phi.as<double>(), n);
+    check(hipGetLastError(), "k_ewald_finalize");
+    check(hipMemcpyAsync(pot, d_phi.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (energy) {
+        double e = 0;
+        for (int i = 0; i < n; ++i) e += q[i] * pot[i];
+        *energy = 0.5 * e - M_PI * Q * Q / (2 * L.vol * alpha_ref * alpha_ref);
+    }
+}
+
+}  // namespace pdmk::gpu

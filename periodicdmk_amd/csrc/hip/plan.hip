@@ -438,54 +438,8 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 th[3 * (size_t)u + d] = 2.0 * M_PI * f[d];
             }
         }
-        theta.upload(th);
         n_ff_modes = (int)T.mode_w.size();
-        ff_w.upload(T.mode_w);
-        std::vector<int> cijk, cm0, clen, ljk, li, lptr;
-        const auto &M = T.mode_ijk;
-        for (int m = 0; m < n_ff_modes;) {
-            int e = m + 1;
-            while (e < n_ff_modes && M[3 * e + 1] == M[3 * m + 1] && M[3 * e + 2] == M[3 * m + 2] &&
-                   M[3 * e] == M[3 * (e - 1)] + 1)
-                ++e;
-            ljk.push_back(M[3 * m + 1]);
-            ljk.push_back(M[3 * m + 2]);
-            li.push_back(M[3 * m]);
-            li.push_back(M[3 * (e - 1)]);
-            lptr.push_back(m);
-            for (int c = m; c < e; c += FF_CHUNK) {
-                cijk.push_back(M[3 * c]);
-                cijk.push_back(M[3 * c + 1]);
-                cijk.push_back(M[3 * c + 2]);
-                cm0.push_back(c);
-                clen.push_back(std::min(FF_CHUNK, e - c));
-            }
-            m = e;
-        }
-        n_ff_lines = (int)lptr.size();
-        n_ff_chunks = (int)cm0.size();
-        ff_chunk_ijk.upload(cijk);
-        ff_chunk_m0.upload(cm0);
-        ff_chunk_len.upload(clen);
-        // weighted structure factors per line, padded at the high-i end to a multiple of
-        // FF_QUAD modes (zeros), so the evaluation reads them FF_QUAD at a time
-        std::vector<int4> lines;
-        std::vector<int> lbase, pidx(n_ff_modes);
-        int npad = 0;
-        for (size_t L = 0; L < lptr.size(); ++L) {
-            const int len = li[2 * L + 1] - li[2 * L] + 1, nq = (len + FF_QUAD - 1) / FF_QUAD;
-            lines.push_back(make_int4(ljk[2 * L], ljk[2 * L + 1], li[2 * L], nq));
-            lbase.push_back(npad);
-            for (int r = 0; r < len; ++r) pidx[lptr[L] + r] = npad + r;
-            npad += nq * FF_QUAD;
-        }
-        ff_lines.upload(lines);
-        ff_line_base.upload(lbase);
-        ff_pidx.upload(pidx);
-        ff_blocks = std::max(1, std::min(2048, (n + FF_TP - 1) / FF_TP));
-        ff_Sp.alloc(sizeof(double2) * (size_t)ff_blocks * std::max(1, n_ff_modes));
-        ff_S.alloc(sizeof(double2) * (size_t)std::max(1, npad));
-        check(hipMemset(ff_S.p, 0, ff_S.bytes), "hipMemset");  // padding stays zero
+        ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w);
         work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
     }
@@ -622,30 +576,6 @@ void launch_near(Plan<Real> &pl, hipStream_t s) {
     check(hipGetLastError(), "k_near_field");
 }
 
-template <class Real>
-void launch_far(Plan<Real> &pl, const double *d_q, hipStream_t s) {
-    if (pl.n_ff_modes == 0) {
-        hipLaunchKernelGGL(k_fill<double>, dim3((pl.n + 255) / 256), dim3(256), 0, s, pl.far.template as<double>(), 0.0,
-                           pl.n);
-        check(hipGetLastError(), "k_fill far");
-        return;
-    }
-    const size_t lds = sizeof(double2) * FF_TP * (pl.ms + pl.mt + pl.mu + 1);
-    dim3 grid(pl.ff_blocks, (pl.n_ff_chunks + 255) / 256);
-    hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, pl.theta.template as<double>(), d_q, pl.n, pl.ms, pl.mt,
-                       pl.mu, pl.ff_chunk_ijk.template as<int>(), pl.ff_chunk_m0.template as<int>(),
-                       pl.ff_chunk_len.template as<int>(), pl.n_ff_chunks, pl.n_ff_modes, pl.ff_Sp.template as<double2>());
-    check(hipGetLastError(), "k_ff_structure");
-    hipLaunchKernelGGL(k_ff_reduce, dim3((pl.n_ff_modes + 31) / 32), dim3(256), 0, s, pl.ff_Sp.template as<double2>(),
-                       pl.ff_blocks, pl.ff_w.template as<double>(), pl.ff_pidx.template as<int>(), pl.n_ff_modes,
-                       pl.ff_S.template as<double2>());
-    check(hipGetLastError(), "k_ff_reduce");
-    hipLaunchKernelGGL(k_ff_eval, dim3((pl.n + 255) / 256), dim3(256), 0, s, pl.theta.template as<double>(), pl.n,
-                       pl.ff_lines.template as<int4>(), pl.ff_line_base.template as<int>(), pl.n_ff_lines,
-                       pl.ff_S.template as<double2>(), pl.far.template as<double>());
-    check(hipGetLastError(), "k_ff_eval");
-}
-
 }  // namespace
 
 template <typename Real>
@@ -659,7 +589,7 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     if (times_ms) check(hipEventRecord(ev[0], s), "hipEventRecord");
     mark(0);
     // far field first: it depends only on the user-order charges
-    if (stages & 8) launch_far<Real>(*this, d_q, s);
+    if (stages & 8) ff.run(d_q, far.as<double>(), s);
     mark(1);
     if (times_ms) check(hipEventRecord(ev[1], s), "hipEventRecord");
     {
@@ -745,6 +675,82 @@ int Plan<Real>::kernel_times(double *avg_ms, int *n_calls) {
         }
     if (n_calls) *n_calls = nc;
     return nc;
+}
+
+void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
+                       const std::vector<int> &M, const std::vector<double> &mode_w) {
+    n = n_points;
+    ms = ms_;
+    mt = mt_;
+    mu = mu_;
+    n_modes = (int)mode_w.size();
+    theta.upload(theta_host);
+    w.upload(mode_w);
+    // lines of constant (j, k) and consecutive i, cut into chunks of FF_CHUNK modes
+    std::vector<int> cijk, cm0, clen, ljk, li, lptr;
+    for (int m = 0; m < n_modes;) {
+        int e = m + 1;
+        while (e < n_modes && M[3 * e + 1] == M[3 * m + 1] && M[3 * e + 2] == M[3 * m + 2] && M[3 * e] == M[3 * (e - 1)] + 1)
+            ++e;
+        ljk.push_back(M[3 * m + 1]);
+        ljk.push_back(M[3 * m + 2]);
+        li.push_back(M[3 * m]);
+        li.push_back(M[3 * (e - 1)]);
+        lptr.push_back(m);
+        for (int c = m; c < e; c += FF_CHUNK) {
+            cijk.push_back(M[3 * c]);
+            cijk.push_back(M[3 * c + 1]);
+            cijk.push_back(M[3 * c + 2]);
+            cm0.push_back(c);
+            clen.push_back(std::min(FF_CHUNK, e - c));
+        }
+        m = e;
+    }
+    n_lines = (int)lptr.size();
+    n_chunks = (int)cm0.size();
+    chunk_ijk.upload(cijk);
+    chunk_m0.upload(cm0);
+    chunk_len.upload(clen);
+    // weighted structure factors per line, padded at the high-i end to a multiple of
+    // FF_QUAD modes (zeros), so the evaluation reads them FF_QUAD at a time
+    std::vector<int4> lv;
+    std::vector<int> lbase, pix(n_modes);
+    int npad = 0;
+    for (size_t L = 0; L < lptr.size(); ++L) {
+        const int len = li[2 * L + 1] - li[2 * L] + 1, nq = (len + FF_QUAD - 1) / FF_QUAD;
+        lv.push_back(make_int4(ljk[2 * L], ljk[2 * L + 1], li[2 * L], nq));
+        lbase.push_back(npad);
+        for (int r = 0; r < len; ++r) pix[lptr[L] + r] = npad + r;
+        npad += nq * FF_QUAD;
+    }
+    lines.upload(lv);
+    line_base.upload(lbase);
+    pidx.upload(pix);
+    blocks = std::max(1, std::min(2048, (n + FF_TP - 1) / FF_TP));
+    Sp.alloc(sizeof(double2) * (size_t)blocks * std::max(1, n_modes));
+    S.alloc(sizeof(double2) * (size_t)std::max(1, npad));
+    check(hipMemset(S.p, 0, S.bytes), "hipMemset");  // padding stays zero
+}
+
+void FourierSum::run(const double *d_q, double *d_out, hipStream_t s) const {
+    if (n == 0) return;
+    if (n_modes == 0) {
+        hipLaunchKernelGGL(k_fill<double>, dim3((n + 255) / 256), dim3(256), 0, s, d_out, 0.0, n);
+        check(hipGetLastError(), "k_fill far");
+        return;
+    }
+    const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1);
+    dim3 grid(blocks, (n_chunks + 255) / 256);
+    hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, theta.as<double>(), d_q, n, ms, mt, mu,
+                       chunk_ijk.as<int>(), chunk_m0.as<int>(), chunk_len.as<int>(), n_chunks, n_modes,
+                       Sp.as<double2>());
+    check(hipGetLastError(), "k_ff_structure");
+    hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32), dim3(256), 0, s, Sp.as<double2>(), blocks, w.as<double>(),
+                       pidx.as<int>(), n_modes, S.as<double2>());
+    check(hipGetLastError(), "k_ff_reduce");
+    hipLaunchKernelGGL(k_ff_eval, dim3((n + 255) / 256), dim3(256), 0, s, theta.as<double>(), n, lines.as<int4>(),
+                       line_base.as<int>(), n_lines, S.as<double2>(), d_out);
+    check(hipGetLastError(), "k_ff_eval");
 }
 
 template struct Plan<float>;

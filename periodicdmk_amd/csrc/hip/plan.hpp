@@ -49,13 +49,25 @@ struct Work {
     double ff_mode_points = 0, ff_flops = 0, on_bytes = 0, nf_bytes = 0;
 };
 
+// Truncated lattice Fourier sum in FP64:
+//   out[u] = sum_{m in the half space} Re( w_m S(m) e^{+i m.theta_u} ),  S(m) = sum_u q_u e^{-i m.theta_u},
+// theta = 2 pi x (fractional coordinates, wrapped to [0, 1)).  Modes (i, j, k) come in lines
+// of constant (j, k) with consecutive i; index ranges are [-m/2, (m-1)/2] for m = ms, mt, mu.
+struct FourierSum {
+    int n = 0, ms = 0, mt = 0, mu = 0, n_modes = 0, n_chunks = 0, n_lines = 0, blocks = 0;
+    DevBuf theta, w, chunk_ijk, chunk_m0, chunk_len, lines, line_base, pidx, Sp, S;
+    void setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
+               const std::vector<int> &mode_ijk, const std::vector<double> &mode_w);
+    void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
+};
+
 template <typename Real>
 struct Plan {
     // sizes
     int n = 0, nsall = 0, nboxes = 0, nlevels = 0, p = 0, n_pw = 0, nc = 0, n_digits = 0;
     double rc = 0;
     int n_up = 0, n_down = 0;     // proxy slots (upward == outgoing plane-wave slots)
-    int ms = 0, mt = 0, mu = 0, n_ff_modes = 0, n_ff_chunks = 0, n_ff_lines = 0, ff_blocks = 0;
+    int ms = 0, mt = 0, mu = 0, n_ff_modes = 0;
     Stats stats;
     Work work;
     hipStream_t stream = nullptr;
@@ -85,7 +97,7 @@ struct Plan {
     int n_nf_leaves = 0, n_nf_items = 0;
     DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs;
     // far field
-    DevBuf theta, ff_w, ff_chunk_ijk, ff_chunk_m0, ff_chunk_len, ff_lines, ff_line_base, ff_pidx, ff_Sp, ff_S;
+    FourierSum ff;
     // incoming plane waves of up to pin_batch boxes of one level
     DevBuf pin;
     int pin_batch = 0;
