@@ -28,6 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     ("host/tree_build.cpp", False),
     ("host/tables.cpp", False),
+    ("host/nufft.cpp", False),
     ("host/host_capi.cpp", False),
     ("hip/plan.hip", True),
     ("hip/ewald.hip", True),

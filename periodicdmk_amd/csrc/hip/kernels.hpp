@@ -831,4 +831,152 @@ __global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ thet
     if (u < n) far[u] = acc;
 }
 
+// ------------------------------------------------------------------ far field by NUFFT
+// Spreading, grid DFTs, deconvolution and interpolation of the non-uniform FFT evaluation of
+// the same truncated Fourier sum (kernel values and point order prepared on the host).
+// Grid layout [g2][g1][g0] (axis 0 fastest); mode box [k2][k1][k0], k = m + M/2.
+
+// One wavefront per work unit (a range of points of one tile): the points are spread one
+// after the other onto the tile's subgrid of E^3 cells in LDS, lane l owning the kernel
+// offsets (dy, dz) = (l % W, l / W) (+ 64 per pass), so no two lanes touch the same cell and
+// the sum order is fixed.  The subgrid is written to scratch[unit].
+template <int W>
+__global__ __launch_bounds__(64) void k_nufft_spread(const int4 *__restrict__ pinfo, const double *__restrict__ kval,
+                                                     const int *__restrict__ unit_ptr, const double *__restrict__ q, int E,
+                                                     double *__restrict__ scratch) {
+    extern __shared__ double nu_lds[];
+    const int unit = blockIdx.x;
+    const int E2 = E * E, E3 = E2 * E;
+    for (int e = threadIdx.x; e < E3; e += 64) nu_lds[e] = 0.0;
+    __syncthreads();
+    constexpr int NP = (W * W + 63) / 64;
+    int oy[NP], oz[NP];
+    bool ok[NP];
+#pragma unroll
+    for (int ps = 0; ps < NP; ++ps) {
+        const int l = (int)threadIdx.x + 64 * ps;
+        ok[ps] = l < W * W;
+        oy[ps] = l % W;
+        oz[ps] = l / W;
+    }
+    for (int p = unit_ptr[unit]; p < unit_ptr[unit + 1]; ++p) {
+        const int4 pi = pinfo[p];
+        const double *kv = kval + (size_t)p * 3 * W;
+        const double qq = q[pi.w];
+        double kx[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) kx[k] = kv[k];
+#pragma unroll
+        for (int ps = 0; ps < NP; ++ps) {
+            if (ok[ps]) {
+                const double c = qq * kv[W + oy[ps]] * kv[2 * W + oz[ps]];
+                double *row = nu_lds + (pi.z + oz[ps]) * E2 + (pi.y + oy[ps]) * E + pi.x;
+#pragma unroll
+                for (int dx = 0; dx < W; ++dx) row[dx] = fma(c, kx[dx], row[dx]);
+            }
+        }
+    }
+    __syncthreads();
+    double *dst = scratch + (size_t)unit * E3;
+    for (int e = threadIdx.x; e < E3; e += 64) dst[e] = nu_lds[e];
+}
+
+// grid[g] = sum over the work units of the tiles whose subgrid covers g (fixed order).
+__global__ __launch_bounds__(256) void k_nufft_gather(const double *__restrict__ scratch, const int *__restrict__ tile_units,
+                                                      const int *__restrict__ gp0, const int *__restrict__ gt0,
+                                                      const int *__restrict__ gl0, const int *__restrict__ gp1,
+                                                      const int *__restrict__ gt1, const int *__restrict__ gl1,
+                                                      const int *__restrict__ gp2, const int *__restrict__ gt2,
+                                                      const int *__restrict__ gl2, int n0, int n1, int n2, int nt0, int nt1,
+                                                      int E, double *__restrict__ grid) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n0 * n1 * n2) return;
+    const int g0 = g % n0, g1 = (g / n0) % n1, g2 = g / (n0 * n1);
+    const size_t E3 = (size_t)E * E * E;
+    double v = 0;
+    for (int a = gp2[g2]; a < gp2[g2 + 1]; ++a)
+        for (int b = gp1[g1]; b < gp1[g1 + 1]; ++b)
+            for (int c = gp0[g0]; c < gp0[g0 + 1]; ++c) {
+                const int tile = (gt2[a] * nt1 + gt1[b]) * nt0 + gt0[c];
+                const int u0 = tile_units[2 * tile], cnt = tile_units[2 * tile + 1];
+                const size_t off = ((size_t)gl2[a] * E + gl1[b]) * E + gl0[c];
+                for (int k = 0; k < cnt; ++k) v += scratch[(size_t)(u0 + k) * E3 + off];
+            }
+    grid[g] = v;
+}
+
+// out[a][k][b] = sum_l in[a][l][b] mat[k L + l] (complex matrix; real or complex input and
+// output, the real output keeping the real part).
+template <bool RIN, bool ROUT>
+__global__ __launch_bounds__(256) void k_dft_axis(const void *__restrict__ in, void *__restrict__ out,
+                                                  const double2 *__restrict__ mat, int A, int L, int K, int B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)A * K * B) return;
+    const int b = (int)(idx % B), k = (int)((idx / B) % K), a = (int)(idx / ((long)B * K));
+    double sr = 0, si = 0;
+    for (int l = 0; l < L; ++l) {
+        const double2 m = mat[(size_t)k * L + l];
+        const size_t ii = ((size_t)a * L + l) * B + b;
+        if (RIN) {
+            const double x = static_cast<const double *>(in)[ii];
+            sr = fma(x, m.x, sr);
+            si = fma(x, m.y, si);
+        } else {
+            const double2 x = static_cast<const double2 *>(in)[ii];
+            sr = fma(x.x, m.x, fma(-x.y, m.y, sr));
+            si = fma(x.x, m.y, fma(x.y, m.x, si));
+        }
+    }
+    if (ROUT) static_cast<double *>(out)[idx] = sr;
+    else static_cast<double2 *>(out)[idx] = make_double2(sr, si);
+}
+
+// C[pos[i]] = cw[i] B[pos[i]] for the listed modes (the other entries of C stay zero).
+__global__ void k_nufft_coef(const double2 *__restrict__ Bbox, const int *__restrict__ pos, const double *__restrict__ cw,
+                             int n_modes, double2 *__restrict__ Cbox) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_modes) return;
+    const double2 v = Bbox[pos[i]];
+    Cbox[pos[i]] = make_double2(cw[i] * v.x, cw[i] * v.y);
+}
+
+// One workgroup per tile with points: the tile's subgrid (periodic wrap) is staged in LDS and
+// each thread interpolates one point: far[user] = sum phi_x phi_y phi_z grid.
+template <int W>
+__global__ __launch_bounds__(256) void k_nufft_interp(const double *__restrict__ grid, const int4 *__restrict__ pinfo,
+                                                      const double *__restrict__ kval, const int4 *__restrict__ tiles,
+                                                      int n0, int n1, int n2, int E, double *__restrict__ far) {
+    extern __shared__ double nu_lds[];
+    const int4 tl = tiles[blockIdx.x];  // (tile x, tile y, tile z, -), points [tbeg[.], tend[.])
+    const int E2 = E * E, E3 = E2 * E;
+    const int ox = 4 * tl.x - W / 2 - 1, oy = 4 * tl.y - W / 2 - 1, oz = 4 * tl.z - W / 2 - 1;
+    for (int e = threadIdx.x; e < E3; e += blockDim.x) {
+        const int lx = e % E, ly = (e / E) % E, lz = e / E2;
+        const int gx = ((ox + lx) % n0 + n0) % n0, gy = ((oy + ly) % n1 + n1) % n1, gz = ((oz + lz) % n2 + n2) % n2;
+        nu_lds[e] = grid[((size_t)gz * n1 + gy) * n0 + gx];
+    }
+    __syncthreads();
+    const int pbeg = tiles[gridDim.x + blockIdx.x].x, pend = tiles[gridDim.x + blockIdx.x].y;
+    for (int p = pbeg + (int)threadIdx.x; p < pend; p += blockDim.x) {
+        const int4 pi = pinfo[p];
+        const double *kv = kval + (size_t)p * 3 * W;
+        double kx[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) kx[k] = kv[k];
+        double s = 0;
+        for (int dz = 0; dz < W; ++dz) {
+            double sz = 0;
+            for (int dy = 0; dy < W; ++dy) {
+                const double *row = nu_lds + (pi.z + dz) * E2 + (pi.y + dy) * E + pi.x;
+                double sy = 0;
+#pragma unroll
+                for (int dx = 0; dx < W; ++dx) sy = fma(kx[dx], row[dx], sy);
+                sz = fma(kv[W + dy], sy, sz);
+            }
+            s = fma(kv[2 * W + dz], sz, s);
+        }
+        far[pi.w] = s;
+    }
+}
+
 }  // namespace pdmk::gpu

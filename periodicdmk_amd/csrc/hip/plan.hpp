@@ -61,6 +61,19 @@ struct FourierSum {
     void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
 };
 
+// The same Fourier sum by a non-uniform FFT (spreading with an exponential-of-semicircle
+// kernel of width W onto a 2x oversampled grid, grid DFTs, interpolation), accurate to about
+// 10^(2-W) relative; deterministic (no atomics).
+struct NufftFarField {
+    int n = 0, n_modes = 0, W = 0, E = 0, n_units = 0, n_tiles = 0;
+    int g[3] = {0, 0, 0}, M[3] = {0, 0, 0}, nt[3] = {0, 0, 0};
+    DevBuf pinfo, kval, unit_ptr, tile_units, gp[3], gt[3], gl[3], tiles, scratch, grid, grid2;
+    DevBuf fwd[3], inv[3], t1, t2, bbox, cbox, pos, cw;
+    void setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
+               const std::vector<int> &mode_ijk, const std::vector<double> &mode_w, int width);
+    void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
+};
+
 template <typename Real>
 struct Plan {
     // sizes
@@ -96,8 +109,10 @@ struct Plan {
     // wavefront item (nf_items, nf_wbox), per source chunk (nf_cbox)
     int n_nf_leaves = 0, n_nf_items = 0;
     DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs;
-    // far field
+    // far field: exact Fourier sum, or the NUFFT for large N x modes
+    int far_nufft = 0;
     FourierSum ff;
+    NufftFarField nf;
     // incoming plane waves of up to pin_batch boxes of one level
     DevBuf pin;
     int pin_batch = 0;

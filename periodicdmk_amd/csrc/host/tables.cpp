@@ -12,6 +12,27 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+
+double cheb(int k, double x) {
+    double a = 1, b = x;
+    if (k == 0) return 1;
+    for (int i = 1; i < k; ++i) { const double c = 2 * x * b - a; a = b; b = c; }
+    return b;
+}
+
+struct Params { double beta; int n_pw, p; };
+
+Params params_for(double eps) {
+    // bracket tables for 3/6/9/12 digits (util.cpp:22-27, tree.hpp:40-44)
+    if (eps >= 1e-3) return {7.2462000846862793, 13, 9};
+    if (eps >= 1e-6) return {13.739999771118164, 27, 18};
+    if (eps >= 1e-9) return {20.736000061035156, 39, 28};
+    if (eps >= 1e-12) return {27.870000839233398, 55, 38};
+    throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
+}
+
+}  // namespace
+
 // Gauss-Legendre nodes (ascending) and weights on [-1, 1].
 void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
     x.resize(n);
@@ -35,26 +56,6 @@ void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
         w[i] = w[n - 1 - i] = 2.0 / ((1 - z * z) * dp * dp);
     }
 }
-
-double cheb(int k, double x) {
-    double a = 1, b = x;
-    if (k == 0) return 1;
-    for (int i = 1; i < k; ++i) { const double c = 2 * x * b - a; a = b; b = c; }
-    return b;
-}
-
-struct Params { double beta; int n_pw, p; };
-
-Params params_for(double eps) {
-    // bracket tables for 3/6/9/12 digits (util.cpp:22-27, tree.hpp:40-44)
-    if (eps >= 1e-3) return {7.2462000846862793, 13, 9};
-    if (eps >= 1e-6) return {13.739999771118164, 27, 18};
-    if (eps >= 1e-9) return {20.736000061035156, 39, 28};
-    if (eps >= 1e-12) return {27.870000839233398, 55, 38};
-    throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
-}
-
-}  // namespace
 
 // The prolate operator -(1-x^2) d^2/dx^2 + 2x d/dx + c^2 x^2 restricted to even
 // normalised Legendre functions is symmetric tridiagonal; psi_0 is the eigenvector of

@@ -50,6 +50,9 @@ struct DmkTables {
     std::vector<double> mode_w;               // 2 * weight (conjugate mode folded in)
 };
 
+// Gauss-Legendre nodes (ascending) and weights on [-1, 1].
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w);
+
 inline int mode_index(const DmkTables &T, int a, int b, int c) { return (c * T.h + a) * T.n_pw + b; }
 
 // n_digits in 1..12; rc and nlevels describe the tree; umat is the lattice frame.

@@ -172,7 +172,9 @@ def test_wrong_charge_count_raises(pdmk, cuda):
 
 
 def test_full_size_workload_parity_and_linearity(pdmk, oracle_mod, cuda):
-    """N = 1e6 cubic workload: FP64 path equals the oracle; the FP32 path is linear in q."""
+    """N = 1e6 cubic workload: the FP64 path equals the oracle stage by stage (the far field,
+    evaluated by the NUFFT at this size, to the reference's NUFFT tolerance eps/4 of the exact
+    sum); the FP32 path is linear in q and agrees with FP64."""
     n = 1_000_000
     cell = W.cubic_cell()
     pos = W.uniform_positions(n, 11, cell)
@@ -180,8 +182,11 @@ def test_full_size_workload_parity_and_linearity(pdmk, oracle_mod, cuda):
     q2 = W.neutral_uniform_charges(n, 101)
     s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=300, precision="double")
     o = oracle_mod.OracleTree(cell, pos, 6, 300, 4)
+    assert rel(gpu_stages(s, q1, 7), o.evaluate(q1, 7)) < 1e-12
+    eps4 = 0.25e-6  # the reference's far-field NUFFT tolerance at 6 digits
+    assert rel(gpu_stages(s, q1, 8), o.evaluate(q1, 8)) < eps4
     g = s.evaluate(q1)
-    assert rel(g, o.evaluate(q1)) < 1e-12
+    assert rel(g, o.evaluate(q1)) < eps4
     s.close()
     o.close()
     f = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=300, precision="float")
@@ -189,3 +194,20 @@ def test_full_size_workload_parity_and_linearity(pdmk, oracle_mod, cuda):
     ab = f.evaluate(0.5 * q1 - 2.0 * q2).astype(np.float64)
     assert rel(ab, 0.5 * a - 2.0 * b) < 1e-5
     assert rel(a, g) < 1e-5
+
+
+@pytest.mark.parametrize("name,digits", [("cubic", 3), ("cubic", 6), ("cubic", 9), ("cubic", 12), ("triclinic", 6),
+                                         ("triclinic", 9), ("clustered", 6), ("long_cell", 6)])
+def test_nufft_far_field(pdmk, oracle_mod, cuda, monkeypatch, name, digits):
+    """Far field by the NUFFT (forced) against the oracle's exact truncated Fourier sum, to
+    the reference's NUFFT tolerance eps/4 (cpp/src/dmk/pbc.cpp:440)."""
+    cell, pos, q, npl = case(name)
+    monkeypatch.setenv("PDMK_FAR_FIELD", "nufft")
+    s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=npl, nz=4, precision="double")
+    monkeypatch.delenv("PDMK_FAR_FIELD")
+    o = oracle_mod.OracleTree(cell, pos, digits, npl, 4, precision="double")
+    far_g, far_o = gpu_stages(s, q, 8), o.evaluate(q, 8)
+    assert rel(far_g, far_o) < 0.25 * 10.0 ** -digits, rel(far_g, far_o)
+    # the NUFFT path is deterministic
+    np.testing.assert_array_equal(gpu_stages(s, q, 8), far_g)
+    s.close()
