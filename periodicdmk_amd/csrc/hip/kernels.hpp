@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dpp_fma.hpp"
+
 namespace pdmk::gpu {
 
 template <class R>
@@ -52,14 +54,14 @@ __global__ void k_gather_charges(const double *__restrict__ q, const int *__rest
 // pot[u] = pot_sorted[i] - w0_i q_u + far[u], u = trg_user[i]
 template <class R>
 __global__ void k_finalize(const R *__restrict__ pot_sorted, const double *__restrict__ w0, const int *__restrict__ trg_user,
-                           const double *__restrict__ q, const double *__restrict__ far, R *__restrict__ pot, int n,
-                           int use_self, int use_far) {
+                           const double *__restrict__ q, const double *__restrict__ far, const int *__restrict__ far_idx,
+                           R *__restrict__ pot, int n, int use_self, int use_far) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int u = trg_user[i];
     double v = (double)pot_sorted[i];
     if (use_self) v -= w0[i] * q[u];
-    if (use_far) v += far[u];
+    if (use_far) v += far[far_idx[i]];
     pot[u] = (R)v;
 }
 
@@ -836,72 +838,168 @@ __global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ thet
 // the same truncated Fourier sum (kernel values and point order prepared on the host).
 // Grid layout [g2][g1][g0] (axis 0 fastest); mode box [k2][k1][k0], k = m + M/2.
 
-// One wavefront per work unit (a range of points of one tile): the points are spread one
-// after the other onto the tile's subgrid of E^3 cells in LDS, lane l owning the kernel
-// offsets (dy, dz) = (l % W, l / W) (+ 64 per pass), so no two lanes touch the same cell and
-// the sum order is fixed.  The subgrid is written to scratch[unit].
-template <int W>
-__global__ __launch_bounds__(64) void k_nufft_spread(const int4 *__restrict__ pinfo, const double *__restrict__ kval,
-                                                     const int *__restrict__ unit_ptr, const double *__restrict__ q, int E,
-                                                     double *__restrict__ scratch) {
-    extern __shared__ double nu_lds[];
-    const int unit = blockIdx.x;
-    const int E2 = E * E, E3 = E2 * E;
-    for (int e = threadIdx.x; e < E3; e += 64) nu_lds[e] = 0.0;
-    __syncthreads();
-    constexpr int NP = (W * W + 63) / 64;
-    int oy[NP], oz[NP];
-    bool ok[NP];
+// Lane l of a wavefront owns the kernel offsets (dy, dz) = (j % W, j / W), j = l + 64 ps
+// (NP passes when W^2 > 64), and keeps its subgrid row (y, z) = (by + dy, bz + dz), the E
+// cells along x, in registers.  A point's kernel along x is stored already placed in the row
+// (phi_x at bx .. bx + W - 1, zeros elsewhere), held by lanes 0..E-1 of every 16-lane row and
+// broadcast with DPP, so every point costs E FMAs per pass whatever its base bx; ky[dy],
+// kz[dz], the charge and the row id are per-lane loads, issued D points ahead.
+template <int W, int E> struct NufftCfg {
+    static constexpr int NP = (W * W + 63) / 64;
+    static constexpr int KS = E + 2 * W;  // kernel values per point
+    static constexpr int D = NP == 1 ? 8 : 4;
+};
+
+template <int NP> struct NuPoint {
+    double ka, kb, q, y[NP], z[NP];
+    int pb;
+};
+
+// A zero the compiler cannot see through: indexing a uniform load with it keeps the load on
+// the vector path, whose in-order completion lets the loads of later points stay in flight
+// (a scalar load would need a full wait on every use).
+__device__ __forceinline__ int nu_vzero() {
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+// Loads point p (kernel row, ky[dy], kz[dz], charge if Q, packed base); vz = nu_vzero().
+template <int W, int E, bool Q, int NP>
+__device__ __forceinline__ void nu_load(NuPoint<NP> &s, const double *__restrict__ kval, const double *__restrict__ qs,
+                                        const int *__restrict__ pbase, int p, int rl, int vz, const int (&oy)[NP],
+                                        const int (&oz)[NP]) {
+    constexpr int KS = NufftCfg<W, E>::KS;
+    const double *kv = kval + (size_t)p * KS;
+    s.ka = kv[min(rl, KS - 1)];
+    s.kb = E > 16 ? kv[min(16 + rl, KS - 1)] : 0.0;
 #pragma unroll
     for (int ps = 0; ps < NP; ++ps) {
-        const int l = (int)threadIdx.x + 64 * ps;
-        ok[ps] = l < W * W;
-        oy[ps] = l % W;
-        oz[ps] = l / W;
+        s.y[ps] = kv[E + oy[ps]];
+        s.z[ps] = kv[E + W + oz[ps]];
     }
-    for (int p = unit_ptr[unit]; p < unit_ptr[unit + 1]; ++p) {
-        const int4 pi = pinfo[p];
-        const double *kv = kval + (size_t)p * 3 * W;
-        const double qq = q[pi.w];
-        double kx[W];
+    s.q = Q ? qs[p + vz] : 0.0;
+    s.pb = pbase[p + vz];
+}
+
+// qs[i] = q[order[i]]: the charges in plan order.
+__global__ __launch_bounds__(256) void k_nufft_permute(const double *__restrict__ q, const int *__restrict__ order, int n,
+                                                       double *__restrict__ qs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) qs[i] = q[order[i]];
+}
+
+// One wavefront per work unit (the points of one tile, or a part of them).  The lanes sum
+// q phi_x phi_y phi_z over the points of a subgrid row (consecutive points with one (by, bz))
+// in registers and add the row into the tile subgrid in LDS when the row changes; the
+// subgrid (E^3, compact) goes to scratch[slot].  The sum order is fixed by the plan, so the
+// result is deterministic.
+template <int W, int E>
+__global__ __launch_bounds__(64) void k_nufft_spread(const int *__restrict__ unit_ptr, const int *__restrict__ unit_slot,
+                                                     const double *__restrict__ kval, const double *__restrict__ qs,
+                                                     const int *__restrict__ pbase, int SY, int SZ,
+                                                     double *__restrict__ scratch) {
+    using C = NufftCfg<W, E>;
+    constexpr int NP = C::NP, D = C::D;
+    extern __shared__ double nu_lds[];
+    double *sub = nu_lds;
+    const int unit = blockIdx.x, lane = threadIdx.x;
+    for (int e = lane; e < E * SZ; e += 64) sub[e] = 0.0;
+    const int rl = lane & 15, vz = nu_vzero();
+    int off[NP], oy[NP], oz[NP];
+    bool ok[NP];
+    double okf[NP];
 #pragma unroll
-        for (int k = 0; k < W; ++k) kx[k] = kv[k];
+    for (int ps = 0; ps < NP; ++ps) {
+        const int j = lane + 64 * ps;
+        ok[ps] = j < W * W;
+        okf[ps] = ok[ps] ? 1.0 : 0.0;
+        oy[ps] = ok[ps] ? j % W : 0;
+        oz[ps] = ok[ps] ? j / W : 0;
+        off[ps] = oz[ps] * SZ + oy[ps] * SY;
+    }
+    const int pbeg = unit_ptr[unit], pend = unit_ptr[unit + 1];
+    double acc[NP][E];
 #pragma unroll
-        for (int ps = 0; ps < NP; ++ps) {
+    for (int ps = 0; ps < NP; ++ps)
+#pragma unroll
+        for (int k = 0; k < E; ++k) acc[ps][k] = 0.0;
+    // adds the register rows into the subgrid row (by, bz) = (rw & 255, rw >> 8)
+    auto flush = [&](int rw) {
+        asm volatile("" ::: "memory");
+        double *blk = sub + (rw >> 8) * SZ + (rw & 255) * SY;
+#pragma unroll
+        for (int ps = 0; ps < NP; ++ps)
             if (ok[ps]) {
-                const double c = qq * kv[W + oy[ps]] * kv[2 * W + oz[ps]];
-                double *row = nu_lds + (pi.z + oz[ps]) * E2 + (pi.y + oy[ps]) * E + pi.x;
+                double *r = blk + off[ps];
 #pragma unroll
-                for (int dx = 0; dx < W; ++dx) row[dx] = fma(c, kx[dx], row[dx]);
+                for (int k = 0; k < E; ++k) r[k] += acc[ps][k];
+            }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int ps = 0; ps < NP; ++ps)
+#pragma unroll
+            for (int k = 0; k < E; ++k) acc[ps][k] = 0.0;
+    };
+    NuPoint<NP> ring[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) nu_load<W, E, true>(ring[j], kval, qs, pbase, min(pbeg + j, pend - 1), rl, vz, oy, oz);
+    int row = -1;
+    for (int p0 = pbeg; p0 < pend; p0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            if (p0 + j < pend) {
+                const NuPoint<NP> cur = ring[j];
+                nu_load<W, E, true>(ring[j], kval, qs, pbase, min(p0 + j + D, pend - 1), rl, vz, oy, oz);
+                const int rw = __builtin_amdgcn_readfirstlane(cur.pb) >> 8;
+                if (rw != row) {
+                    if (row >= 0) flush(row);
+                    row = rw;
+                }
+#pragma unroll
+                for (int ps = 0; ps < NP; ++ps)
+                    dpp_row_axpy<E>(acc[ps], cur.ka, cur.kb, cur.q * cur.y[ps] * cur.z[ps] * okf[ps]);
             }
         }
     }
+    if (row >= 0) flush(row);
     __syncthreads();
-    double *dst = scratch + (size_t)unit * E3;
-    for (int e = threadIdx.x; e < E3; e += 64) dst[e] = nu_lds[e];
+    constexpr int E2 = E * E, E3 = E2 * E;
+    double *dst = scratch + (size_t)unit_slot[unit] * E3;
+    for (int e = lane; e < E3; e += 64) dst[e] = sub[(e / E2) * SZ + ((e / E) % E) * SY + e % E];
 }
 
-// grid[g] = sum over the work units of the tiles whose subgrid covers g (fixed order).
-__global__ __launch_bounds__(256) void k_nufft_gather(const double *__restrict__ scratch, const int *__restrict__ tile_units,
-                                                      const int *__restrict__ gp0, const int *__restrict__ gt0,
-                                                      const int *__restrict__ gl0, const int *__restrict__ gp1,
-                                                      const int *__restrict__ gt1, const int *__restrict__ gl1,
-                                                      const int *__restrict__ gp2, const int *__restrict__ gt2,
-                                                      const int *__restrict__ gl2, int n0, int n1, int n2, int nt0, int nt1,
-                                                      int E, double *__restrict__ grid) {
+// scratch[tile] += scratch[extra slots of the tile] (in unit order), for tiles split into
+// several units.  Workgroup (chunk of 256 cells, listed tile).
+__global__ __launch_bounds__(256) void k_nufft_extra_reduce(double *__restrict__ scratch, const int *__restrict__ extra_tile,
+                                                            const int *__restrict__ extra_ptr, int n_tiles, int E3) {
+    const int i = blockIdx.y, e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E3) return;
+    double v = scratch[(size_t)extra_tile[i] * E3 + e];
+    for (int k = extra_ptr[i]; k < extra_ptr[i + 1]; ++k) v += scratch[(size_t)(n_tiles + k) * E3 + e];
+    scratch[(size_t)extra_tile[i] * E3 + e] = v;
+}
+
+// grid[g] = sum over the tiles whose subgrid covers g (fixed order); tiles without points
+// hold zeros.
+__global__ __launch_bounds__(64) void k_nufft_gather(const double *__restrict__ scratch, const int *__restrict__ gp0,
+                                                     const int *__restrict__ gt0, const int *__restrict__ gl0,
+                                                     const int *__restrict__ gp1, const int *__restrict__ gt1,
+                                                     const int *__restrict__ gl1, const int *__restrict__ gp2,
+                                                     const int *__restrict__ gt2, const int *__restrict__ gl2, int n0, int n1,
+                                                     int n2, int nt0, int nt1, int E, double *__restrict__ grid) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n0 * n1 * n2) return;
     const int g0 = g % n0, g1 = (g / n0) % n1, g2 = g / (n0 * n1);
     const size_t E3 = (size_t)E * E * E;
     double v = 0;
     for (int a = gp2[g2]; a < gp2[g2 + 1]; ++a)
-        for (int b = gp1[g1]; b < gp1[g1 + 1]; ++b)
-            for (int c = gp0[g0]; c < gp0[g0 + 1]; ++c) {
-                const int tile = (gt2[a] * nt1 + gt1[b]) * nt0 + gt0[c];
-                const int u0 = tile_units[2 * tile], cnt = tile_units[2 * tile + 1];
-                const size_t off = ((size_t)gl2[a] * E + gl1[b]) * E + gl0[c];
-                for (int k = 0; k < cnt; ++k) v += scratch[(size_t)(u0 + k) * E3 + off];
-            }
+        for (int b = gp1[g1]; b < gp1[g1 + 1]; ++b) {
+            const int tzy = gt2[a] * nt1 + gt1[b];
+            const int lzy = gl2[a] * E + gl1[b];
+            for (int c = gp0[g0]; c < gp0[g0 + 1]; ++c)
+                v += scratch[(size_t)(tzy * nt0 + gt0[c]) * E3 + (size_t)lzy * E + gl0[c]];
+        }
     grid[g] = v;
 }
 
@@ -940,43 +1038,110 @@ __global__ void k_nufft_coef(const double2 *__restrict__ Bbox, const int *__rest
     Cbox[pos[i]] = make_double2(cw[i] * v.x, cw[i] * v.y);
 }
 
-// One workgroup per tile with points: the tile's subgrid (periodic wrap) is staged in LDS and
-// each thread interpolates one point: far[user] = sum phi_x phi_y phi_z grid.
-template <int W>
-__global__ __launch_bounds__(256) void k_nufft_interp(const double *__restrict__ grid, const int4 *__restrict__ pinfo,
-                                                      const double *__restrict__ kval, const int4 *__restrict__ tiles,
-                                                      int n0, int n1, int n2, int E, double *__restrict__ far) {
+// Row stride of the interpolation kernel's partial sums (odd, so that lane i reading row i
+// spreads over the banks), and points per partial-sum batch.
+constexpr int NU_PS = 65, NU_IB = 32;
+
+// One wavefront per work unit: the tile subgrid (periodic wrap of the grid) is staged in
+// LDS; for each subgrid row (by, bz) the lanes load their rows into registers and, per point,
+// form sum_x phi_x row[x] phi_y phi_z over their offsets (dy, dz); the 64 lane partials of
+// each point are added in lane order.  far[i] = the interpolated value at the point in plan
+// position i.
+template <int W, int E>
+__global__ __launch_bounds__(64) void k_nufft_interp(const double *__restrict__ grid, const int *__restrict__ unit_ptr,
+                                                     const int *__restrict__ unit_tile, const double *__restrict__ kval,
+                                                     const int *__restrict__ pbase, int n0, int n1, int n2, int nt0, int nt1,
+                                                     int T, int lo, int SY, int SZ, double *__restrict__ far) {
+    using C = NufftCfg<W, E>;
+    constexpr int NP = C::NP, D = C::D;
     extern __shared__ double nu_lds[];
-    const int4 tl = tiles[blockIdx.x];  // (tile x, tile y, tile z, -), points [tbeg[.], tend[.])
-    const int E2 = E * E, E3 = E2 * E;
-    const int ox = 4 * tl.x - W / 2 - 1, oy = 4 * tl.y - W / 2 - 1, oz = 4 * tl.z - W / 2 - 1;
-    for (int e = threadIdx.x; e < E3; e += blockDim.x) {
-        const int lx = e % E, ly = (e / E) % E, lz = e / E2;
-        const int gx = ((ox + lx) % n0 + n0) % n0, gy = ((oy + ly) % n1 + n1) % n1, gz = ((oz + lz) % n2 + n2) % n2;
-        nu_lds[e] = grid[((size_t)gz * n1 + gy) * n0 + gx];
+    const int unit = blockIdx.x, lane = threadIdx.x;
+    double *sub = nu_lds, *part = nu_lds + E * SZ;
+    int *wrap = reinterpret_cast<int *>(part + NU_IB * NU_PS);  // 3 E grid indices
+    const int rl = lane & 15, vz = nu_vzero();
+    int off[NP], oy[NP], oz[NP];
+    double okf[NP];
+#pragma unroll
+    for (int ps = 0; ps < NP; ++ps) {
+        const int j = lane + 64 * ps;
+        const bool ok = j < W * W;
+        okf[ps] = ok ? 1.0 : 0.0;
+        oy[ps] = ok ? j % W : 0;
+        oz[ps] = ok ? j / W : 0;
+        off[ps] = oz[ps] * SZ + oy[ps] * SY;
+    }
+    const int pbeg = unit_ptr[unit], pend = unit_ptr[unit + 1];
+    NuPoint<NP> ring[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) nu_load<W, E, false>(ring[j], kval, nullptr, pbase, min(pbeg + j, pend - 1), rl, vz, oy, oz);
+    const int tile = unit_tile[unit];
+    const int org[3] = {T * (tile % nt0) + lo, T * ((tile / nt0) % nt1) + lo, T * (tile / (nt0 * nt1)) + lo};
+    const int nn[3] = {n0, n1, n2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (lane < E) wrap[d * E + lane] = ((org[d] + lane) % nn[d] + nn[d]) % nn[d];
+    __syncthreads();
+    {
+        constexpr int E2 = E * E, E3 = E2 * E;
+        for (int e0 = 0; e0 < E3; e0 += 256) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = min(e0 + 64 * u + lane, E3 - 1);
+                v[u] = grid[((size_t)wrap[2 * E + e / E2] * n1 + wrap[E + (e / E) % E]) * n0 + wrap[e % E]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + 64 * u + lane;
+                if (e < E3) sub[(e / E2) * SZ + ((e / E) % E) * SY + e % E] = v[u];
+            }
+        }
     }
     __syncthreads();
-    const int pbeg = tiles[gridDim.x + blockIdx.x].x, pend = tiles[gridDim.x + blockIdx.x].y;
-    for (int p = pbeg + (int)threadIdx.x; p < pend; p += blockDim.x) {
-        const int4 pi = pinfo[p];
-        const double *kv = kval + (size_t)p * 3 * W;
-        double kx[W];
-#pragma unroll
-        for (int k = 0; k < W; ++k) kx[k] = kv[k];
-        double s = 0;
-        for (int dz = 0; dz < W; ++dz) {
-            double sz = 0;
-            for (int dy = 0; dy < W; ++dy) {
-                const double *row = nu_lds + (pi.z + dz) * E2 + (pi.y + dy) * E + pi.x;
-                double sy = 0;
-#pragma unroll
-                for (int dx = 0; dx < W; ++dx) sy = fma(kx[dx], row[dx], sy);
-                sz = fma(kv[W + dy], sy, sz);
-            }
-            s = fma(kv[2 * W + dz], sz, s);
+    double col[NP][E];
+    int row = -1, bstart = pbeg, nb = 0;
+    // far for the nb points from bstart: lane i adds row i of the partial sums
+    auto reduce = [&](int n_pts) {
+        asm volatile("" ::: "memory");
+        if (lane < n_pts) {
+            double s = 0;
+            for (int l = 0; l < 64; ++l) s += part[lane * NU_PS + l];
+            far[bstart + lane] = s;
         }
-        far[pi.w] = s;
+        asm volatile("" ::: "memory");
+    };
+    for (int p0 = pbeg; p0 < pend; p0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            if (p0 + j < pend) {
+                const NuPoint<NP> cur = ring[j];
+                nu_load<W, E, false>(ring[j], kval, nullptr, pbase, min(p0 + j + D, pend - 1), rl, vz, oy, oz);
+                const int rw = __builtin_amdgcn_readfirstlane(cur.pb) >> 8;
+                if (rw != row) {
+                    row = rw;
+                    const double *blk = sub + (row >> 8) * SZ + (row & 255) * SY;
+#pragma unroll
+                    for (int ps = 0; ps < NP; ++ps)
+#pragma unroll
+                        for (int k = 0; k < E; ++k) col[ps][k] = blk[off[ps] + k];
+                }
+                double acc = 0;
+#pragma unroll
+                for (int ps = 0; ps < NP; ++ps) {
+                    double s0 = 0, s1 = 0;
+                    dpp_row_dot<E>(s0, s1, cur.ka, cur.kb, col[ps]);
+                    acc = fma(s0 + s1, cur.y[ps] * cur.z[ps] * okf[ps], acc);
+                }
+                part[nb * NU_PS + lane] = acc;
+                if (++nb == NU_IB) {
+                    reduce(NU_IB);
+                    bstart += NU_IB;
+                    nb = 0;
+                }
+            }
+        }
     }
+    reduce(nb);
 }
 
 }  // namespace pdmk::gpu

@@ -450,8 +450,16 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             if (std::string(e) == "exact") far_nufft = 0;
             else if (std::string(e) == "nufft") far_nufft = 1;
         }
-        if (far_nufft) nf.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w, std::max(4, std::min(14, n_digits + 2)));
-        else ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w);
+        // far[] is in user order for the direct sum and in the NUFFT's point order otherwise;
+        // far_idx[i] = where the far field of sorted target i sits
+        std::vector<int> fidx(tu);
+        if (far_nufft) {
+            nf.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w, std::max(4, std::min(14, n_digits + 2)));
+            for (int i = 0; i < n; ++i) fidx[i] = nf.position[tu[i]];
+        } else {
+            ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w);
+        }
+        far_idx.upload(fidx);
         work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
     }
@@ -643,7 +651,8 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     }
     mark(4);
     hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(), trg_w0.as<double>(),
-                       trg_user.as<int>(), d_q, far.as<double>(), d_pot, n, (stages & 4) ? 1 : 0, (stages & 8) ? 1 : 0);
+                       trg_user.as<int>(), d_q, far.as<double>(), far_idx.as<int>(), d_pot, n, (stages & 4) ? 1 : 0,
+                       (stages & 8) ? 1 : 0);
     check(hipGetLastError(), "k_finalize");
     mark(5);
     if (times_ms) {
@@ -775,31 +784,29 @@ void NufftFarField::setup(const std::vector<double> &theta_host, int n_points, i
     n = n_points;
     n_modes = (int)mode_w.size();
     W = P.W;
+    T = P.T;
+    lo = P.lo;
     E = P.E;
+    SY = P.SY;
+    SZ = P.SZ;
     for (int d = 0; d < 3; ++d) {
         g[d] = P.n[d];
         M[d] = P.M[d];
         nt[d] = P.ntile[d];
     }
-    std::vector<int4> pi(n);
-    for (int i = 0; i < n; ++i)
-        pi[i] = make_int4(P.pinfo[4 * (size_t)i], P.pinfo[4 * (size_t)i + 1], P.pinfo[4 * (size_t)i + 2], P.pinfo[4 * (size_t)i + 3]);
-    pinfo.upload(pi);
+    order.upload(P.order);
+    position.assign(n, 0);
+    for (int i = 0; i < n; ++i) position[P.order[i]] = i;
     kval.upload(P.kval);
+    pbase.upload(P.pbase);
     unit_ptr.upload(P.unit_ptr);
-    tile_units.upload(P.tile_units);
+    unit_tile.upload(P.unit_tile);
+    unit_slot.upload(P.unit_slot);
+    extra_tile.upload(P.extra_tile);
+    extra_ptr.upload(P.extra_ptr);
     n_units = (int)P.unit_tile.size();
-    // tiles with points: (tile coordinates), then (first point, end)
-    std::vector<int4> tl, tr;
-    for (int t = 0; t < nt[0] * nt[1] * nt[2]; ++t) {
-        const int u0 = P.tile_units[2 * t], cnt = P.tile_units[2 * t + 1];
-        if (cnt == 0) continue;
-        tl.push_back(make_int4(t % nt[0], (t / nt[0]) % nt[1], t / (nt[0] * nt[1]), 0));
-        tr.push_back(make_int4(P.unit_ptr[u0], P.unit_ptr[u0 + cnt], 0, 0));
-    }
-    n_tiles = (int)tl.size();
-    tl.insert(tl.end(), tr.begin(), tr.end());
-    tiles.upload(tl);
+    n_tiles = nt[0] * nt[1] * nt[2];
+    n_extra_tiles = (int)P.extra_tile.size();
     for (int d = 0; d < 3; ++d) {
         gp[d].upload(P.gptr[d]);
         gt[d].upload(P.gtile[d]);
@@ -822,7 +829,11 @@ void NufftFarField::setup(const std::vector<double> &theta_host, int n_points, i
     pos.upload(ps);
     cw.upload(c);
     const size_t E3 = (size_t)E * E * E, G3 = (size_t)g[0] * g[1] * g[2], M3 = (size_t)M[0] * M[1] * M[2];
-    scratch.alloc(sizeof(double) * E3 * std::max(1, n_units));
+    const int n_extra = P.extra_ptr.empty() ? 0 : P.extra_ptr.back();
+    qs.alloc(sizeof(double) * std::max(1, n));
+    // tiles without points keep the zeros written here: the spreading writes only unit slots
+    scratch.alloc(sizeof(double) * E3 * ((size_t)n_tiles + n_extra));
+    check(hipMemset(scratch.p, 0, scratch.bytes), "hipMemset");
     grid.alloc(sizeof(double) * G3);
     grid2.alloc(sizeof(double) * G3);
     t1.alloc(sizeof(double2) * std::max((size_t)g[2] * g[1] * M[0], (size_t)1));
@@ -830,37 +841,51 @@ void NufftFarField::setup(const std::vector<double> &theta_host, int n_points, i
     bbox.alloc(sizeof(double2) * M3);
     cbox.alloc(sizeof(double2) * M3);
     check(hipMemset(cbox.p, 0, cbox.bytes), "hipMemset");  // unlisted modes stay zero
+    check(hipDeviceSynchronize(), "NUFFT setup");
 }
 
 namespace {
 
-template <int W>
-void nufft_spread_interp(const NufftFarField &F, const double *d_q, double *d_out, hipStream_t s, bool spread) {
-    const size_t lds = sizeof(double) * (size_t)F.E * F.E * F.E;
+template <int W, int E>
+void nufft_spread_interp(const NufftFarField &F, double *d_out, hipStream_t s, bool spread) {
+    if (F.E != E) throw std::logic_error("NUFFT subgrid extent mismatch");
+    const size_t G = (size_t)E * F.SZ;
     if (spread) {
-        hipLaunchKernelGGL(k_nufft_spread<W>, dim3(F.n_units), dim3(64), lds, s, F.pinfo.as<int4>(), F.kval.as<double>(),
-                           F.unit_ptr.as<int>(), d_q, F.E, F.scratch.as<double>());
+        hipLaunchKernelGGL((k_nufft_spread<W, E>), dim3(F.n_units), dim3(64), sizeof(double) * G, s, F.unit_ptr.as<int>(),
+                           F.unit_slot.as<int>(), F.kval.as<double>(), F.qs.as<double>(), F.pbase.as<int>(), F.SY, F.SZ,
+                           F.scratch.as<double>());
         check(hipGetLastError(), "k_nufft_spread");
     } else {
-        hipLaunchKernelGGL(k_nufft_interp<W>, dim3(F.n_tiles), dim3(256), lds, s, F.grid2.as<double>(), F.pinfo.as<int4>(),
-                           F.kval.as<double>(), F.tiles.as<int4>(), F.g[0], F.g[1], F.g[2], F.E, d_out);
+        const size_t lds = sizeof(double) * (G + (size_t)NU_IB * NU_PS) + sizeof(int) * 3 * E;
+        hipLaunchKernelGGL((k_nufft_interp<W, E>), dim3(F.n_units), dim3(64), lds, s, F.grid2.as<double>(),
+                           F.unit_ptr.as<int>(), F.unit_tile.as<int>(), F.kval.as<double>(), F.pbase.as<int>(), F.g[0], F.g[1],
+                           F.g[2], F.nt[0], F.nt[1], F.T, F.lo, F.SY, F.SZ, d_out);
         check(hipGetLastError(), "k_nufft_interp");
     }
 }
 
-void nufft_dispatch(const NufftFarField &F, const double *d_q, double *d_out, hipStream_t s, bool spread) {
+// Subgrid extent E = T - 1 + W (+1 for odd W) for tile edge T = 2 or 4.
+template <int W>
+void nufft_width(const NufftFarField &F, double *d_out, hipStream_t s, bool spread) {
+    constexpr int EW = W + (W & 1);
+    if (F.T == 2) nufft_spread_interp<W, EW + 1>(F, d_out, s, spread);
+    else if (F.T == 4) nufft_spread_interp<W, EW + 3>(F, d_out, s, spread);
+    else throw std::logic_error("unsupported NUFFT tile edge");
+}
+
+void nufft_dispatch(const NufftFarField &F, double *d_out, hipStream_t s, bool spread) {
     switch (F.W) {
-    case 4: nufft_spread_interp<4>(F, d_q, d_out, s, spread); break;
-    case 5: nufft_spread_interp<5>(F, d_q, d_out, s, spread); break;
-    case 6: nufft_spread_interp<6>(F, d_q, d_out, s, spread); break;
-    case 7: nufft_spread_interp<7>(F, d_q, d_out, s, spread); break;
-    case 8: nufft_spread_interp<8>(F, d_q, d_out, s, spread); break;
-    case 9: nufft_spread_interp<9>(F, d_q, d_out, s, spread); break;
-    case 10: nufft_spread_interp<10>(F, d_q, d_out, s, spread); break;
-    case 11: nufft_spread_interp<11>(F, d_q, d_out, s, spread); break;
-    case 12: nufft_spread_interp<12>(F, d_q, d_out, s, spread); break;
-    case 13: nufft_spread_interp<13>(F, d_q, d_out, s, spread); break;
-    case 14: nufft_spread_interp<14>(F, d_q, d_out, s, spread); break;
+    case 4: nufft_width<4>(F, d_out, s, spread); break;
+    case 5: nufft_width<5>(F, d_out, s, spread); break;
+    case 6: nufft_width<6>(F, d_out, s, spread); break;
+    case 7: nufft_width<7>(F, d_out, s, spread); break;
+    case 8: nufft_width<8>(F, d_out, s, spread); break;
+    case 9: nufft_width<9>(F, d_out, s, spread); break;
+    case 10: nufft_width<10>(F, d_out, s, spread); break;
+    case 11: nufft_width<11>(F, d_out, s, spread); break;
+    case 12: nufft_width<12>(F, d_out, s, spread); break;
+    case 13: nufft_width<13>(F, d_out, s, spread); break;
+    case 14: nufft_width<14>(F, d_out, s, spread); break;
     default: throw std::logic_error("unsupported NUFFT kernel width");
     }
 }
@@ -882,12 +907,18 @@ void NufftFarField::run(const double *d_q, double *d_out, hipStream_t s) const {
         check(hipGetLastError(), "k_fill far");
         return;
     }
-    nufft_dispatch(*this, d_q, d_out, s, true);
-    const int G3 = g[0] * g[1] * g[2];
-    hipLaunchKernelGGL(k_nufft_gather, dim3((G3 + 255) / 256), dim3(256), 0, s, scratch.as<double>(), tile_units.as<int>(),
-                       gp[0].as<int>(), gt[0].as<int>(), gl[0].as<int>(), gp[1].as<int>(), gt[1].as<int>(), gl[1].as<int>(),
-                       gp[2].as<int>(), gt[2].as<int>(), gl[2].as<int>(), g[0], g[1], g[2], nt[0], nt[1], E,
-                       grid.as<double>());
+    hipLaunchKernelGGL(k_nufft_permute, dim3((n + 255) / 256), dim3(256), 0, s, d_q, order.as<int>(), n, qs.as<double>());
+    check(hipGetLastError(), "k_nufft_permute");
+    nufft_dispatch(*this, d_out, s, true);
+    const int G3 = g[0] * g[1] * g[2], E3 = E * E * E;
+    if (n_extra_tiles > 0) {
+        hipLaunchKernelGGL(k_nufft_extra_reduce, dim3((E3 + 255) / 256, n_extra_tiles), dim3(256), 0, s,
+                           scratch.as<double>(), extra_tile.as<int>(), extra_ptr.as<int>(), n_tiles, E3);
+        check(hipGetLastError(), "k_nufft_extra_reduce");
+    }
+    hipLaunchKernelGGL(k_nufft_gather, dim3((G3 + 63) / 64), dim3(64), 0, s, scratch.as<double>(), gp[0].as<int>(),
+                       gt[0].as<int>(), gl[0].as<int>(), gp[1].as<int>(), gt[1].as<int>(), gl[1].as<int>(), gp[2].as<int>(),
+                       gt[2].as<int>(), gl[2].as<int>(), g[0], g[1], g[2], nt[0], nt[1], E, grid.as<double>());
     check(hipGetLastError(), "k_nufft_gather");
     // grid [g2][g1][g0] -> mode box [k2][k1][k0]
     dft_axis<true, false>(grid.p, t1.p, fwd[0], g[2] * g[1], g[0], M[0], 1, s);
@@ -900,7 +931,7 @@ void NufftFarField::run(const double *d_q, double *d_out, hipStream_t s) const {
     dft_axis<false, false>(cbox.p, t2.p, inv[2], 1, M[2], g[2], M[1] * M[0], s);
     dft_axis<false, false>(t2.p, t1.p, inv[1], g[2], M[1], g[1], M[0], s);
     dft_axis<false, true>(t1.p, grid2.p, inv[0], g[2] * g[1], M[0], g[0], 1, s);
-    nufft_dispatch(*this, d_q, d_out, s, false);
+    nufft_dispatch(*this, d_out, s, false);
 }
 
 template struct Plan<float>;

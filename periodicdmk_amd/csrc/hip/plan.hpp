@@ -65,10 +65,11 @@ struct FourierSum {
 // kernel of width W onto a 2x oversampled grid, grid DFTs, interpolation), accurate to about
 // 10^(2-W) relative; deterministic (no atomics).
 struct NufftFarField {
-    int n = 0, n_modes = 0, W = 0, E = 0, n_units = 0, n_tiles = 0;
+    int n = 0, n_modes = 0, W = 0, T = 0, lo = 0, E = 0, SY = 0, SZ = 0, n_units = 0, n_tiles = 0, n_extra_tiles = 0;
     int g[3] = {0, 0, 0}, M[3] = {0, 0, 0}, nt[3] = {0, 0, 0};
-    DevBuf pinfo, kval, unit_ptr, tile_units, gp[3], gt[3], gl[3], tiles, scratch, grid, grid2;
-    DevBuf fwd[3], inv[3], t1, t2, bbox, cbox, pos, cw;
+    DevBuf order, kval, pbase, unit_ptr, unit_tile, unit_slot, extra_tile, extra_ptr, gp[3], gt[3], gl[3];
+    DevBuf qs, scratch, grid, grid2, fwd[3], inv[3], t1, t2, bbox, cbox, pos, cw;
+    std::vector<int> position;  // plan position of each user point (run() writes out[position[u]])
     void setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
                const std::vector<int> &mode_ijk, const std::vector<double> &mode_w, int width);
     void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
@@ -87,7 +88,7 @@ struct Plan {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
     // O(N) data
-    DevBuf src, src_user, trg, trg_user, trg_w0, pot_sorted, q_dev, pot_dev, far;
+    DevBuf src, src_user, trg, trg_user, trg_w0, pot_sorted, q_dev, pot_dev, far, far_idx;
     // upward
     DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, rng_off, proxy_up;
     struct Tensor3Launch { int n_items; DevBuf out, add, ptr, in_slot, in_code; };

@@ -18,30 +18,63 @@ double es_kernel(double z, double beta) {
     return std::exp(beta * (std::sqrt(1.0 - z * z) - 1.0));
 }
 
+// Subgrid strides: the smallest layout (SY >= E, SZ >= E SY) for which the 8-byte cells
+// z SZ + y SY of the lanes l = 0..31 and 32..63, (y, z) = (l % W, l / W), fall in distinct
+// double-banks (address mod 32 in 8-byte units).
+void subgrid_strides(int E, int W, int &SY, int &SZ) {
+    int best_w = 1 << 30, best_size = 1 << 30;
+    for (int sy = E; sy < E + 9; ++sy)
+        for (int sz = E * sy; sz < E * sy + 64; ++sz) {
+            int worst = 0;
+            for (int half = 0; half < 2; ++half) {
+                int cnt[32] = {0};
+                for (int l = 32 * half; l < 32 * half + 32 && l < W * W; ++l)
+                    worst = std::max(worst, ++cnt[((l / W) * sz + (l % W) * sy) % 32]);
+            }
+            const int size = E * sz;
+            if (worst < best_w || (worst == best_w && size < best_size)) {
+                best_w = worst;
+                best_size = size;
+                SY = sy;
+                SZ = sz;
+            }
+        }
+}
+
 }  // namespace
 
+int nufft_tile_edge(const int M[3], int W) {
+    (void)W;
+    long tiles = 1;
+    for (int d = 0; d < 3; ++d) tiles *= (2L * M[d] + 3) / 4;
+    return tiles >= 2048 ? 4 : 2;
+}
+
 NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const int Mm[3], int W) {
-    if (W < 2 || W > 14) throw std::invalid_argument("NUFFT kernel width out of range");
+    if (W < 4 || W > 14) throw std::invalid_argument("NUFFT kernel width out of range");
     NufftPlanHost P;
-    constexpr int T = NufftPlanHost::kTile;
+    const int T = nufft_tile_edge(Mm, W);
+    P.T = T;
     P.W = W;
+    P.B = (W & 1) ? T + 1 : T;
+    P.lo = 1 - (W + 1) / 2;
+    P.E = P.B - 1 + W;
     P.beta = 2.30 * W;  // exponential-of-semicircle shape for an upsampling factor of 2
-    P.E = T + W + 2;
+    subgrid_strides(P.E, W, P.SY, P.SZ);
     for (int d = 0; d < 3; ++d) {
         P.M[d] = Mm[d];
-        int n = 2 * Mm[d];
-        n = (n + T - 1) / T * T;
+        const int n = (2 * Mm[d] + T - 1) / T * T;
         P.n[d] = n;
         P.h[d] = kTwoPi / n;
         P.ntile[d] = n / T;
     }
-    const int half = W / 2;
-    // ---- per point: tile, subgrid base and kernel values
+    const int B = P.B;
+    // ---- per point: tile, base and kernel values
     const int ntiles = P.ntile[0] * P.ntile[1] * P.ntile[2];
-    std::vector<int> tile_of(np), base(3 * (size_t)np);
+    std::vector<int> key(np);  // tile * B^3 + base index
     std::vector<double> kv(3 * (size_t)W * np);
     for (int u = 0; u < np; ++u) {
-        int tc[3];
+        int tc[3], b[3];
         for (int d = 0; d < 3; ++d) {
             const int n = P.n[d];
             double x = theta[3 * (size_t)u + d] / P.h[d];  // grid coordinate in [0, n)
@@ -50,43 +83,62 @@ NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const in
             const int c = std::min(n - 1, (int)std::floor(x));
             tc[d] = c / T;
             const int g0 = (int)std::floor(x - 0.5 * W) + 1;  // first grid point inside the support
-            const int org = T * tc[d] - half - 1;
-            const int b = g0 - org;
-            if (b < 0 || b + W > P.E) throw std::logic_error("NUFFT tile subgrid too small");
-            base[3 * (size_t)u + d] = b;
+            b[d] = g0 - T * tc[d] - P.lo;
+            if (b[d] < 0 || b[d] >= B) throw std::logic_error("NUFFT base outside the tile subgrid");
             for (int k = 0; k < W; ++k)
                 kv[((size_t)u * 3 + d) * W + k] = es_kernel((x - (g0 + k)) / (0.5 * W), P.beta);
         }
-        tile_of[u] = (tc[2] * P.ntile[1] + tc[1]) * P.ntile[0] + tc[0];
+        const int tile = (tc[2] * P.ntile[1] + tc[1]) * P.ntile[0] + tc[0];
+        key[u] = tile * B * B * B + (b[2] * B + b[1]) * B + b[0];
     }
-    // ---- counting sort by tile; tiles split into parts of at most `part` points so that
-    // there are enough work units for the GPU
-    std::vector<int> tstart(ntiles + 1, 0);
-    for (int u = 0; u < np; ++u) tstart[tile_of[u] + 1]++;
-    for (int t = 0; t < ntiles; ++t) tstart[t + 1] += tstart[t];
-    std::vector<int> order(np);
+    // ---- counting sort by (tile, base), stable in the user order
+    const long nkeys = (long)ntiles * B * B * B;
+    std::vector<int> kstart(nkeys + 1, 0);
+    for (int u = 0; u < np; ++u) kstart[key[u] + 1]++;
+    for (long k = 0; k < nkeys; ++k) kstart[k + 1] += kstart[k];
+    P.order.resize(np);
     {
-        std::vector<int> fill(tstart.begin(), tstart.end() - 1);
-        for (int u = 0; u < np; ++u) order[fill[tile_of[u]]++] = u;
+        std::vector<int> fill(kstart.begin(), kstart.end() - 1);
+        for (int u = 0; u < np; ++u) P.order[fill[key[u]]++] = u;
     }
-    P.pinfo.resize(4 * (size_t)np);
-    P.kval.resize(3 * (size_t)W * np);
+    P.KS = P.E + 2 * W;
+    P.kval.assign((size_t)P.KS * np, 0.0);
+    P.pbase.resize(np);
     for (int i = 0; i < np; ++i) {
-        const int u = order[i];
-        for (int d = 0; d < 3; ++d) P.pinfo[4 * (size_t)i + d] = base[3 * (size_t)u + d];
-        P.pinfo[4 * (size_t)i + 3] = u;
-        std::copy(kv.begin() + 3 * (size_t)W * u, kv.begin() + 3 * (size_t)W * (u + 1), P.kval.begin() + 3 * (size_t)W * i);
-    }
-    const int part = std::max(64, np / 4096 + 1);
-    P.unit_ptr.push_back(0);
-    P.tile_units.assign(2 * (size_t)ntiles, 0);
-    for (int t = 0; t < ntiles; ++t) {
-        P.tile_units[2 * t] = (int)P.unit_tile.size();
-        for (int a = tstart[t]; a < tstart[t + 1]; a += part) {
-            P.unit_tile.push_back(t);
-            P.unit_ptr.push_back(std::min(tstart[t + 1], a + part));
+        const int u = P.order[i];
+        const int bi = key[u] % (B * B * B);
+        const int bx = bi % B;
+        P.pbase[i] = bx | (((bi / B) % B) << 8) | ((bi / (B * B)) << 16);
+        double *dst = P.kval.data() + (size_t)P.KS * i;
+        const double *src = kv.data() + 3 * (size_t)W * u;
+        for (int k = 0; k < W; ++k) {
+            dst[bx + k] = src[k];
+            dst[P.E + k] = src[W + k];
+            dst[P.E + W + k] = src[2 * W + k];
         }
-        P.tile_units[2 * t + 1] = (int)P.unit_tile.size() - P.tile_units[2 * t];
+    }
+    // ---- units: the points of a tile, split into equal parts of at most `part` points when a
+    // tile holds many more than the average
+    const int B3 = B * B * B;
+    int nonempty = 0;
+    for (int t = 0; t < ntiles; ++t) nonempty += kstart[(long)(t + 1) * B3] > kstart[(long)t * B3];
+    const int part = std::max(256, 4 * ((np + std::max(1, nonempty) - 1) / std::max(1, nonempty)));
+    P.unit_ptr.push_back(0);
+    int n_extra = 0;
+    P.extra_ptr.push_back(0);
+    for (int t = 0; t < ntiles; ++t) {
+        const int a = kstart[(long)t * B3], e = kstart[(long)(t + 1) * B3];
+        if (e == a) continue;
+        const int nparts = (e - a + part - 1) / part;
+        for (int k = 0; k < nparts; ++k) {
+            P.unit_ptr.push_back(a + (int)((long)(e - a) * (k + 1) / nparts));
+            P.unit_tile.push_back(t);
+            P.unit_slot.push_back(k == 0 ? t : ntiles + n_extra++);
+        }
+        if (nparts > 1) {
+            P.extra_tile.push_back(t);
+            P.extra_ptr.push_back(n_extra);
+        }
     }
     // ---- per axis: tiles whose subgrid covers grid index g, with the local offset
     for (int d = 0; d < 3; ++d) {
@@ -94,7 +146,7 @@ NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const in
         P.gptr[d].assign(1, 0);
         for (int g = 0; g < n; ++g) {
             for (int t = 0; t < P.ntile[d]; ++t) {
-                const int org = T * t - half - 1;
+                const int org = T * t + P.lo;
                 int loc = ((g - org) % n + n) % n;
                 for (; loc < P.E; loc += n) {
                     P.gtile[d].push_back(t);
