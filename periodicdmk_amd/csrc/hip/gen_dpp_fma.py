@@ -6,7 +6,9 @@ import os
 
 lines=[]
 A=lines.append
-A('''// Row-broadcast FP64 FMAs for gfx950 (CDNA4).  The 64 lanes of a wavefront form four rows of
+A('''// Written by gen_dpp_fma.py.
+//
+// Row-broadcast FP64 FMAs for gfx950 (CDNA4).  The 64 lanes of a wavefront form four rows of
 // 16; a DP-ALU DPP operand with row_newbcast:k reads lane k of the lane's own row.  A row
 // whose lanes hold the E values x[0..E-1] (lanes 0..15 of register ka, then lanes 0..E-17 of
 // kb when E > 16) feeds every lane's E FMAs with no scalar or LDS traffic.  Defined for the
