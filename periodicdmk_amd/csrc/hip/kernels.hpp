@@ -160,6 +160,105 @@ __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
     }
 }
 
+// The same proxy charges on the FP64 matrix cores, for 28 <= P <= 32.  Per destination box
+//   C[(c, b), a] = sum_s A[(c, b), s] B[s, a],  A = q_s T_c(z_s) T_b(y_s),  B = T_a(x_s),
+// is a GEMM of P^2 rows by 32 columns (a padded) over the box's sources, computed with
+// v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// D[l / 16 + 4 r][l % 16] in accumulator element r (layout read off gfx950 hardware).  Wave w of the workgroup owns the row
+// tiles w, w + NW, ...  and both column tiles.
+typedef double c2p_d4 __attribute__((ext_vector_type(4)));
+template <int P> struct C2pMfmaCfg {
+    static constexpr int RT = (P * P + 15) / 16;      // row tiles
+    static constexpr int NW = 8;                      // waves per workgroup (two per SIMD)
+    static constexpr int TPW = (RT + NW - 1) / NW;    // row tiles per wave
+    static constexpr int TS = 64;                     // sources per LDS table
+    static constexpr int PA = 32;                     // a, padded
+};
+
+template <int P>
+__global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
+    const P4<double> *__restrict__ src, const int *__restrict__ grp_slot, const double *__restrict__ grp_geom,
+    const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
+    const double *__restrict__ rng_off, double *__restrict__ proxy) {
+    using C = C2pMfmaCfg<P>;
+    constexpr int TS = C::TS, PA = C::PA, RT = C::RT, NW = C::NW, TPW = C::TPW;
+    static_assert(P <= PA, "order above the padded column count");
+    __shared__ double Tx[TS * PA], Ty[TS * P], Tz[TS * P];
+    const int g = blockIdx.x, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int kk = lane >> 4, jj = lane & 15;
+    const double sc = grp_geom[4 * g + 3];
+    int ci[TPW], bi[TPW];
+    double rv[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int row = (w + NW * t) * 16 + jj;
+        const bool ok = row < P * P;
+        rv[t] = ok ? 1.0 : 0.0;
+        ci[t] = ok ? row / P : 0;
+        bi[t] = ok ? row % P : 0;
+    }
+    c2p_d4 acc[TPW][2];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t][0] = acc[t][1] = c2p_d4{0.0, 0.0, 0.0, 0.0};
+    for (int r = grp_rng[g]; r < grp_rng[g + 1]; ++r) {
+        const int beg = rng_begin[r], cnt = rng_count[r];
+        for (int s0 = 0; s0 < cnt; s0 += TS) {
+            const int ns = min(TS, cnt - s0);
+            __syncthreads();
+            for (int t = threadIdx.x; t < 3 * TS; t += blockDim.x) {
+                const int s = t % TS, d = t / TS;
+                double *T = d == 0 ? Tx + s * PA : (d == 1 ? Ty + s * P : Tz + s * P);
+                const int len = d == 0 ? PA : P;
+                if (s < ns) {
+                    const P4<double> pt = src[beg + s0 + s];
+                    const double x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) + rng_off[4 * r + d]);
+                    const double m = (d == 2) ? pt.w : 1.0;
+                    double t0 = m, t1 = x * m;
+                    T[0] = t0;
+                    T[1] = t1;
+                    for (int k = 2; k < P; ++k) {
+                        const double t2 = 2.0 * x * t1 - t0;
+                        T[k] = t2;
+                        t0 = t1;
+                        t1 = t2;
+                    }
+                    for (int k = P; k < len; ++k) T[k] = 0.0;
+                } else {
+                    for (int k = 0; k < len; ++k) T[k] = 0.0;
+                }
+            }
+            __syncthreads();
+            for (int k0 = 0; k0 < ns; k0 += 4) {
+                const int s = k0 + kk;
+                const double b0 = Tx[s * PA + jj], b1 = Tx[s * PA + 16 + jj];
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    if (w + NW * t < RT) {
+                        const double a = Tz[s * P + ci[t]] * Ty[s * P + bi[t]] * rv[t];
+                        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[t][1], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+    double *out = proxy + (size_t)grp_slot[g] * (P * P * P);
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        if (w + NW * t >= RT) continue;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int a = ct * 16 + jj;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = (w + NW * t) * 16 + kk + 4 * e;
+                if (row < P * P && a < P) out[(size_t)row * P + a] = acc[t][ct][e];
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ tensor transforms
 // All transforms are sequences of mode products of a small 3-D tensor with small dense
 // matrices.  One workgroup owns one output box; a thread owns one column of the mode being
@@ -549,6 +648,127 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval_targets(const R *__restrict
         for (int i = 0; i < EV_TPT; ++i) {
             const int t = t0 + (int)threadIdx.x + EV_THREADS * i;
             if (t < cnt) pot_sorted[beg + t] += s[i];
+        }
+    }
+}
+
+// The same evaluation on the FP64 matrix cores, for 28 <= P <= 32.  Per leaf and stage of
+// 64 targets
+//   R[t, (c, b)] = sum_a T_a(x_t) C[(c, b), a]          (GEMM, a padded to 32)
+//   pot_t += sum_(c,b) R[t, (c, b)] T_b(y_t) T_c(z_t)
+// with v_mfma_f64_16x16x4f64 (operand layout as in k_charge2proxy_mfma: A[l % 16][l / 16],
+// B[l / 16][l % 16], D[l / 16 + 4 r][l % 16]).  Wave w takes the (c, b) tiles w, w + NW, ...;
+// for each it loads the tile's 8 B operands once and runs the 4 target tiles of the stage
+// as independent accumulators.  Lane partials are reduced over each 16-lane row, then over
+// the waves in a fixed order.
+template <int P> struct EvMfmaCfg {
+    static constexpr int RT = (P * P + 15) / 16;
+    static constexpr int NW = 8;
+    static constexpr int TT = 4;          // target tiles per stage
+    static constexpr int NT = 16 * TT;    // targets per stage
+    static constexpr int PA = 32;
+};
+
+template <int P>
+__global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
+    const double *__restrict__ down_base, const int *__restrict__ leaf_down, const double *__restrict__ leaf_geom,
+    const int *__restrict__ leaf_trg, const P4<double> *__restrict__ trg, double *__restrict__ pot_sorted) {
+    using C = EvMfmaCfg<P>;
+    constexpr int RT = C::RT, NW = C::NW, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
+    static_assert(P <= PA, "order above the padded a count");
+    __shared__ double Tx[NT * PA], Ty[NT * P], Tz[NT * P], red[NW * NT];
+    const int lf = blockIdx.x, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int kk = lane >> 4, jj = lane & 15;
+    const double *Cm = down_base + (size_t)leaf_down[lf] * P * PP;
+    const double cx = leaf_geom[4 * lf], cy = leaf_geom[4 * lf + 1], cz = leaf_geom[4 * lf + 2], sc = leaf_geom[4 * lf + 3];
+    const int beg = leaf_trg[2 * lf], cnt = leaf_trg[2 * lf + 1];
+    for (int t0 = 0; t0 < cnt; t0 += NT) {
+        __syncthreads();
+        for (int task = threadIdx.x; task < 3 * NT; task += blockDim.x) {
+            const int t = task % NT, d = task / NT;
+            double *T = d == 0 ? Tx + t * PA : (d == 1 ? Ty + t * P : Tz + t * P);
+            const int len = d == 0 ? PA : P;
+            if (t0 + t < cnt) {
+                const P4<double> pt = trg[beg + t0 + t];
+                const double x = sc * (d == 0 ? pt.x - cx : (d == 1 ? pt.y - cy : pt.z - cz));
+                double ta = 1.0, tb = x;
+                T[0] = 1.0;
+                T[1] = x;
+                for (int k = 2; k < P; ++k) {
+                    const double tn = 2.0 * x * tb - ta;
+                    T[k] = tn;
+                    ta = tb;
+                    tb = tn;
+                }
+                for (int k = P; k < len; ++k) T[k] = 0.0;
+            } else {
+                for (int k = 0; k < len; ++k) T[k] = 0.0;
+            }
+        }
+        __syncthreads();
+        double A[TT][PA / 4];
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+            for (int ks = 0; ks < PA / 4; ++ks) A[tt][ks] = Tx[(tt * 16 + jj) * PA + 4 * ks + kk];
+        double part[TT][4];
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) part[tt][e] = 0.0;
+        // B operands of (c, b) tile rt: lane (kk, jj) supplies C[rt 16 + jj][4 ks + kk]
+        auto load_b = [&](double (&B)[PA / 4], int rt) {
+            const int row = rt * 16 + jj;
+#pragma unroll
+            for (int ks = 0; ks < PA / 4; ++ks) {
+                const int a = 4 * ks + kk;
+                B[ks] = (row < PP && a < P) ? Cm[(size_t)row * P + a] : 0.0;
+            }
+        };
+        double Bn[PA / 4];
+        load_b(Bn, w);
+        for (int rt = w; rt < RT; rt += NW) {
+            const int row = rt * 16 + jj;  // (c, b) of the B column this lane supplies
+            const bool rok = row < PP;
+            double B[PA / 4];
+#pragma unroll
+            for (int ks = 0; ks < PA / 4; ++ks) B[ks] = Bn[ks];
+            if (rt + NW < RT) load_b(Bn, rt + NW);  // next tile's operands, in flight meanwhile
+            c2p_d4 D[TT];
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt) D[tt] = c2p_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < PA / 4; ++ks)
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt) D[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[tt][ks], B[ks], D[tt], 0, 0, 0);
+            // D[tt][e] = R[target tt 16 + kk + 4 e, (c, b) = rt 16 + jj]
+            const int cb = rok ? row : 0, c = cb / P, b = cb % P;
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int t = tt * 16 + kk + 4 * e;
+                    part[tt][e] = fma(D[tt][e], Ty[t * P + b] * Tz[t * P + c], part[tt][e]);
+                }
+        }
+        // sum over the 16 lanes of each row (the (c, b) columns), then over the waves
+#pragma unroll
+        for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                double v = part[tt][e];
+#pragma unroll
+                for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
+                if (jj == 0) red[w * NT + tt * 16 + kk + 4 * e] = v;
+            }
+        __syncthreads();
+        for (int t = threadIdx.x; t < NT; t += blockDim.x) {
+            if (t0 + t < cnt) {
+                double v = 0;
+                for (int q = 0; q < NW; ++q) v += red[q * NT + t];
+                pot_sorted[beg + t0 + t] += v;
+            }
         }
     }
 }

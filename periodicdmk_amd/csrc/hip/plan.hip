@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 #include <string>
 #include <cmath>
 #include <cstdio>
@@ -538,11 +539,21 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     using C = Cx<Real>;
     const int NM = NPW * NPW * ((NPW + 1) / 2);
     if (pl.n_up > 0) {
-        hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(C2pCfg<P>::threads), 0, s, pl.src.template as<P4<Real>>(),
+        // FP64 at P = 28 (9 digits): matrix cores (a padded to 32 costs 1/8); smaller orders pad
+        // too much and stay on the vector ALUs
+        if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
+            hipLaunchKernelGGL((k_charge2proxy_mfma<P>), dim3(pl.n_up), dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
+                               pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
+                               pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
+                               pl.rng_off.template as<Real>(), pl.proxy_up.template as<Real>());
+            check(hipGetLastError(), "k_charge2proxy_mfma");
+        } else {
+            hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(C2pCfg<P>::threads), 0, s, pl.src.template as<P4<Real>>(),
                            pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(), pl.up_rng.template as<int>(),
                            pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(), pl.rng_off.template as<Real>(),
                            pl.proxy_up.template as<Real>());
-        check(hipGetLastError(), "k_charge2proxy");
+            check(hipGetLastError(), "k_charge2proxy");
+        }
     }
     for (auto &C : pl.c2p_levels) {
         launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(), C.children,
@@ -577,11 +588,19 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
                                        pl.mats.template as<Real>(), s);
     }
     if (pl.n_leaf_eval > 0) {
-        hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(EV_THREADS), 0, s,
-                           pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
-                           pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(), pl.trg.template as<P4<Real>>(),
-                           pl.pot_sorted.template as<Real>());
-        check(hipGetLastError(), "k_eval_targets");
+        if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
+            hipLaunchKernelGGL((k_eval_targets_mfma<P>), dim3(pl.n_leaf_eval), dim3(64 * EvMfmaCfg<P>::NW), 0, s,
+                               pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
+                               pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(),
+                               pl.trg.template as<P4<Real>>(), pl.pot_sorted.template as<Real>());
+            check(hipGetLastError(), "k_eval_targets_mfma");
+        } else {
+            hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(EV_THREADS), 0, s,
+                               pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
+                               pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(),
+                               pl.trg.template as<P4<Real>>(), pl.pot_sorted.template as<Real>());
+            check(hipGetLastError(), "k_eval_targets");
+        }
     }
 }
 
