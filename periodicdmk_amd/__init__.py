@@ -68,6 +68,12 @@ def lib():
         "pdmk_gpu_kernel_times": (C.c_int, [vp, dp]),
         "pdmk_host_tree": (C.c_int, [dp, C.c_int, dp, C.c_int, C.c_int, lp, ip, dp, ip, ip, dp]),
         "pdmk_host_tables": (C.c_int, [C.c_int, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "pdmk_campaign_sha256": (C.c_int, [vp, C.c_size_t, vp]),
+        "pdmk_campaign_write": (C.c_int, [C.c_char_p, C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_ulonglong, dp,
+                                          C.c_uint, C.POINTER(C.c_uint), dp, dp, dp]),
+        "pdmk_campaign_read_header": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint), ip, ip,
+                                                C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]),
+        "pdmk_campaign_read": (C.c_int, [C.c_char_p, dp, C.POINTER(C.c_uint), dp, dp, dp]),
         "pdmk_host_pswf": (C.c_double, [C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():

@@ -30,6 +30,7 @@ SOURCES = [
     ("host/tables.cpp", False),
     ("host/nufft.cpp", False),
     ("host/host_capi.cpp", False),
+    ("host/campaign_capi.cpp", False),
     ("hip/plan.hip", True),
     ("hip/ewald.hip", True),
     ("hip/api.hip", True),

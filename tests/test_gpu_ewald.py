@@ -66,3 +66,25 @@ def test_errors(pdmk, cuda):
         pdmk.ewald(np.zeros((3, 3)), pos, q, 5.0)
     with pytest.raises(RuntimeError):
         pdmk.ewald(np.eye(3), pos, q, -1.0)
+
+
+def test_reference_data_file_from_gpu_ewald(pdmk, cuda, tmp_path):
+    """Reference-data files carry oracle potentials between runs: the FP64 Ewald sum of the
+    N=1e4 cubic problem at M=1000 sampled targets goes into a PDMC file, is read back and
+    verified, and the 6-digit evaluate() agrees with it to 1e-5 (the reference's float
+    end-to-end bound is 1e-4)."""
+    from periodicdmk_amd import campaign as K
+    cell = W.cubic_cell()
+    pos = W.uniform_positions(10_000, 1, cell)
+    q = W.neutral_uniform_charges(10_000, 2)
+    idx = np.random.default_rng(7).choice(10_000, K.DEFAULT_M, replace=False).astype(np.uint32)
+    u_ewald, _ = pdmk.ewald(cell, pos, q, 5.5)
+    path = tmp_path / "cfg1_d6.pdmc"
+    K.write_ref(path, K.RefData(cell=cell, positions=pos, charges=q, ref_idx=idx, ref_u=u_ewald[idx],
+                                config_id=1, digits=6, prec=1, seed=1))
+    r = K.read_ref(path)
+    assert (r.config_id, r.N, r.M, r.digits) == (1, 10_000, K.DEFAULT_M, 6)
+    s = pdmk.PDMKSolver(r.cell, r.positions, n_digits=r.digits, n_per_leaf=300, nz=4, precision="double")
+    pot = s.evaluate(r.charges)
+    s.close()
+    assert rel(pot[r.ref_idx], r.ref_u) < 1e-5
