@@ -56,7 +56,8 @@ def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, char
     out.free()
     res = {
         "precision": precision, "n_digits": n_digits, "ms_per_step": ms, "mpts_per_s": n / (ms / 1000.0) / 1e6,
-        "build_s": build_s, "kernel_ms": ktimes, "stats": s.stats(), "work": s.work(), "last_charge_index": last_q,
+        "build_s": build_s, "build": s.build_info(), "kernel_ms": ktimes, "stats": s.stats(), "work": s.work(),
+        "last_charge_index": last_q,
     }
     s.close()
     return res, pot
@@ -119,6 +120,18 @@ def main():
         print(f"[bench] {name}: {results[name]['ms_per_step']:.3f} ms/step "
               f"({results[name]['mpts_per_s']:.1f} Mpts/s) kernels {results[name]['kernel_ms']}",
               file=sys.stderr, flush=True)
+
+    # tree construction: per-point work on the GPU vs the same construction on the host
+    tree_build = None
+    if wanted:
+        t0 = time.time()
+        P.host_tree(cell, pos, args.npl, args.nz)
+        host_ms = (time.time() - t0) * 1e3
+        b = results[wanted[0]]["build"]
+        tree_build = {"tree_ms_gpu_points": round(b["tree_ms"], 1), "tree_ms_host_points": round(host_ms, 1),
+                      "tree_on_gpu": b["tree_on_gpu"], "build_ms_total": round(b["build_ms"], 1),
+                      "far_plan_ms": round(b["far_plan_ms"], 1)}
+        print(f"[bench] tree build {tree_build}", file=sys.stderr, flush=True)
 
     # CPU oracle on the same workload: timed baseline and error reference
     cpu = None
@@ -195,6 +208,7 @@ def main():
                      "flops_per_launch": w["nf_flops"], "pairs_in_support": w["nf_pairs_support"],
                      "pairs_candidate": w["nf_pairs_candidate"], "kernel_ms": round(nf_ms, 4)},
         "cpu_baseline": cpu,
+        "tree_build": tree_build,
         "paths": {k: {kk: vv for kk, vv in v.items() if kk not in ("stats",)} for k, v in results.items()},
     }
     if "mixed6" in results and "fp64_6" in results:

@@ -50,6 +50,11 @@ int pdmk_gpu_evaluate_stages(void* h, const double* d_charges, void* d_pot, int 
 /// is_float, n_pair_tiles}.
 int pdmk_gpu_stats(void* h, long long* out16);
 
+/// Construction of the handle: out[0..3] = {total ms, tree ms (geometry, images, tree),
+/// far-field plan ms, 1 if the tree's per-point work (images, binning, distribution, sort)
+/// ran on the GPU else 0}.  No reference counterpart (measurement hook).
+int pdmk_gpu_build_info(void* h, double* out4);
+
 /// Algorithmic work per evaluate(): out[0..7] = {near-field pairs in support,
 /// near-field candidate pairs, near-field flops (in-support pairs x (2*NC+12)),
 /// proxy/PW transform flops, far-field mode-point products, far-field flops,

@@ -68,6 +68,7 @@ def lib():
         "pdmk_gpu_kernel_times": (C.c_int, [vp, dp]),
         "pdmk_host_tree": (C.c_int, [dp, C.c_int, dp, C.c_int, C.c_int, lp, ip, dp, ip, ip, dp]),
         "pdmk_host_tables": (C.c_int, [C.c_int, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "pdmk_gpu_build_info": (C.c_int, [vp, dp]),
         "pdmk_campaign_sha256": (C.c_int, [vp, C.c_size_t, vp]),
         "pdmk_campaign_write": (C.c_int, [C.c_char_p, C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_ulonglong, dp,
                                           C.c_uint, C.POINTER(C.c_uint), dp, dp, dp]),
@@ -259,6 +260,12 @@ class PDMKSolver:
         o = np.zeros(16, dtype=np.int64)
         lib().pdmk_gpu_stats(self._h, o.ctypes.data_as(C.POINTER(C.c_longlong)))
         return dict(zip(STATS_KEYS, (int(v) for v in o)))
+
+    def build_info(self):
+        """Construction times (ms) and whether the tree's per-point work ran on the GPU."""
+        o = np.zeros(4)
+        lib().pdmk_gpu_build_info(self._h, _d(o))
+        return {"build_ms": float(o[0]), "tree_ms": float(o[1]), "far_plan_ms": float(o[2]), "tree_on_gpu": bool(o[3])}
 
     def work(self):
         o = np.zeros(8)

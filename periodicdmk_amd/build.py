@@ -33,6 +33,7 @@ SOURCES = [
     ("host/campaign_capi.cpp", False),
     ("hip/plan.hip", True),
     ("hip/ewald.hip", True),
+    ("hip/tree_gpu.hip", True),
     ("hip/api.hip", True),
 ]
 

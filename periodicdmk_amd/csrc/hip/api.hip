@@ -317,6 +317,16 @@ int pdmk_gpu_stats(void *h, long long *o) {
     return 0;
 }
 
+int pdmk_gpu_build_info(void *h, double *o) {
+    if (!h || !o) return -1;
+    const bool f = static_cast<Handle *>(h)->is_float;
+    const double *ms = f ? tree_of<float>(h)->plan().build_ms : tree_of<double>(h)->plan().build_ms;
+    const bool gpu = f ? tree_of<float>(h)->plan().tree_on_gpu : tree_of<double>(h)->plan().tree_on_gpu;
+    const double v[4] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0};
+    std::memcpy(o, v, sizeof(v));
+    return 0;
+}
+
 int pdmk_gpu_work(void *h, double *o) {
     if (!h || !o) return -1;
     const pdmk::gpu::Work &w =

@@ -5,8 +5,10 @@
 #include "../host/tables.hpp"
 #include "../host/tree_build.hpp"
 #include "kernels.hpp"
+#include "tree_gpu.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <type_traits>
 #include <string>
@@ -70,8 +72,27 @@ __global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restr
 template <typename Real>
 Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user, int n_per_leaf, int nz) {
     if (n_digits_user < 1 || n_digits_user > 12) throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
-    const host::Geometry g = host::make_geometry(cell, pos, N, nz);
-    const host::Tree t = host::build_tree(g, n_per_leaf);
+    check(hipStreamCreate(&stream), "hipStreamCreate");
+    for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    using clk = std::chrono::steady_clock;
+    const auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+    const auto t_start = clk::now();
+    host::Geometry g;
+    host::Tree t;
+    try {
+        // per-point tree construction on the GPU; a tree deeper than its path keys hold is
+        // built with the points on the host (same result)
+        tree_on_gpu = build_tree_gpu(cell, pos, N, nz, n_per_leaf, stream, g, t);
+        if (!tree_on_gpu) {
+            g = host::make_geometry(cell, pos, N, nz);
+            t = host::build_tree(g, n_per_leaf);
+        }
+        build_ms[1] = ms_since(t_start);
+    } catch (...) {
+        for (auto &e : ev) hipEventDestroy(e);
+        hipStreamDestroy(stream);
+        throw;
+    }
     const host::DmkTables T = host::make_tables(n_digits_user, g.rc, t.nlevels, g.umat);
     n = N;
     rc = g.rc;
@@ -83,9 +104,6 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     n_digits = T.n_digits;
     nc = (int)T.residual.size();
     if (nc != nc_for_digits(n_digits)) throw std::logic_error("PeriodicDMK: residual table mismatch");
-
-    check(hipStreamCreate(&stream), "hipStreamCreate");
-    for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
 
     // ---------------- sources and targets (sorted order), coordinates relative to their
     // leaf centre: offsets between boxes are formed in FP64 on the host, so FP32 kernels
@@ -454,6 +472,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         // far[] is in user order for the direct sum and in the NUFFT's point order otherwise;
         // far_idx[i] = where the far field of sorted target i sits
         std::vector<int> fidx(tu);
+        const auto t_far = clk::now();
         if (far_nufft) {
             nf.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w, std::max(4, std::min(14, n_digits + 2)));
             for (int i = 0; i < n; ++i) fidx[i] = nf.position[tu[i]];
@@ -461,6 +480,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w);
         }
         far_idx.upload(fidx);
+        build_ms[2] = ms_since(t_far);
         work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
     }
@@ -506,6 +526,8 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     }
     src_perm_host = t.src_order;
     trg_perm_host = t.trg_order;
+    check(hipStreamSynchronize(stream), "plan build");
+    build_ms[0] = ms_since(t_start);
 }
 
 template <typename Real>

@@ -142,6 +142,11 @@ struct Plan {
     int kslots = 0;
     long kcall = 0;
 
+    // construction: total, tree (geometry, images, tree) and far-field plan milliseconds;
+    // whether the tree's per-point work ran on the GPU
+    double build_ms[3] = {0, 0, 0};
+    bool tree_on_gpu = false;
+
     // topology export for parity tests
     std::vector<int> box7;
     std::vector<double> box_centers;

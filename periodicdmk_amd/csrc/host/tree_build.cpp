@@ -7,7 +7,7 @@
 
 namespace pdmk::host {
 
-Geometry make_geometry(const double *cell, const double *pos, int n, int nz) {
+Geometry make_frame(const double *cell, const double *pos, int n, int nz) {
     if (n < 1) throw std::invalid_argument("PeriodicDMK: need at least one position");
     if (nz <= 0) throw std::invalid_argument("PeriodicDMK: nz must be positive");
     Geometry g;
@@ -62,9 +62,12 @@ Geometry make_geometry(const double *cell, const double *pos, int n, int nz) {
             g.r_src[3 * (size_t)p + j] = s;
         }
 
-    // periodic images inside the extended grid [xgrid0 - rc, xgrid0 + (mgrid+1) rc]
-    // (image_layout.hpp:14-88: same shell order, so images carry the same indices)
-    double elo[3], ehi[3], clo[3], chi[3];
+    return g;
+}
+
+std::vector<std::array<double, 3>> image_shifts(const Geometry &g, double elo[3], double ehi[3]) {
+    const double *U = g.umat;
+    double clo[3], chi[3];
     for (int i = 0; i < 3; ++i) {
         elo[i] = g.xgrid0[i] - g.rc;
         ehi[i] = g.xgrid0[i] + (g.mgrid[i] + 1) * g.rc;
@@ -73,6 +76,7 @@ Geometry make_geometry(const double *cell, const double *pos, int n, int nz) {
         for (int j = i + 1; j < 3; ++j) w += std::abs(U[i + 3 * j]);
         chi[i] = g.xgrid0[i] + w;
     }
+    std::vector<std::array<double, 3>> out;
     const double rc = g.rc;
     const int n3 = (int)((g.mgrid[2] + 1) * rc / U[8]) + 1;
     for (int k3 = -n3; k3 <= n3; ++k3) {
@@ -88,18 +92,28 @@ Geometry make_geometry(const double *cell, const double *pos, int n, int nz) {
                 if (!k1 && !k2 && !k3) continue;
                 const double t1 = U[0] * k1 + o12;
                 if (clo[0] + t1 > ehi[0] || chi[0] + t1 < elo[0]) continue;
-                for (int p = 0; p < n; ++p) {
-                    const double x = g.r_src[3 * (size_t)p] + t1, y = g.r_src[3 * (size_t)p + 1] + t2,
-                                 z = g.r_src[3 * (size_t)p + 2] + t3;
-                    if (x < elo[0] || x > ehi[0] || y < elo[1] || y > ehi[1] || z < elo[2] || z > ehi[2]) continue;
-                    g.img_pos.push_back(x);
-                    g.img_pos.push_back(y);
-                    g.img_pos.push_back(z);
-                    g.img_src.push_back(p);
-                }
+                out.push_back({t1, t2, t3});
             }
         }
     }
+    return out;
+}
+
+Geometry make_geometry(const double *cell, const double *pos, int n, int nz) {
+    Geometry g = make_frame(cell, pos, n, nz);
+    // periodic images inside the extended grid [xgrid0 - rc, xgrid0 + (mgrid+1) rc]
+    // (image_layout.hpp:14-88: same shell order, so images carry the same indices)
+    double elo[3], ehi[3];
+    for (const auto &t : image_shifts(g, elo, ehi))
+        for (int p = 0; p < n; ++p) {
+            const double x = g.r_src[3 * (size_t)p] + t[0], y = g.r_src[3 * (size_t)p + 1] + t[1],
+                         z = g.r_src[3 * (size_t)p + 2] + t[2];
+            if (x < elo[0] || x > ehi[0] || y < elo[1] || y > ehi[1] || z < elo[2] || z > ehi[2]) continue;
+            g.img_pos.push_back(x);
+            g.img_pos.push_back(y);
+            g.img_pos.push_back(z);
+            g.img_src.push_back(p);
+        }
     return g;
 }
 
@@ -129,8 +143,6 @@ bool touching(int la, const std::array<int, 3> &a, int lb, const std::array<int,
 
 struct Builder {
     Tree &t;
-    const Geometry &g;
-    std::vector<double> &spos, &tpos;  // sorted positions
 
     int new_box(int lvl, int par, std::array<int, 3> c) {
         const int id = t.nboxes++;
@@ -150,9 +162,62 @@ struct Builder {
         return id;
     }
 
-    // stable distribution of a parent's points into its 8 children (bit d: x_d >= centre_d)
-    static void distribute(const double *cen, int begin, int count, std::vector<double> &pos, std::vector<int> &order,
-                           int child_begin[8], int child_count[8]) {
+    // creates the 8 children of b (ids first_child[b] + k, bit d of k: upper half along d)
+    void make_children(int b) {
+        const int lvl = t.level[b];
+        const auto c = t.coord[b];
+        t.first_child[b] = t.nboxes;
+        for (int k = 0; k < 8; ++k)
+            new_box(lvl + 1, b, {2 * c[0] + (k & 1), 2 * c[1] + ((k >> 1) & 1), 2 * c[2] + ((k >> 2) & 1)});
+    }
+};
+
+// The points held on the host: level-0 counting sort, stable in-place distribution into
+// children (bit d of the child index: x_d >= centre_d).
+struct HostPoints final : PointBackend {
+    const Geometry &g;
+    std::vector<double> spos, tpos;
+    std::vector<int> sorder, torder;
+    explicit HostPoints(const Geometry &geo) : g(geo) {}
+
+    static void grid_sort(const Tree &t, const std::vector<double> &pos, int n, std::vector<int> &order,
+                          std::vector<int> &count) {
+        const double inv = 1.0 / t.rc;
+        const int nb0 = t.mle[0] * t.mle[1] * t.mle[2];
+        std::vector<int> box(n);
+        count.assign(nb0, 0);
+        for (int i = 0; i < n; ++i) {
+            int ii[3];
+            for (int d = 0; d < 3; ++d) {
+                int v = (int)((pos[3 * (size_t)i + d] - t.xelo[d]) * inv);
+                ii[d] = std::min(std::max(v, 0), t.mle[d] - 1);
+            }
+            box[i] = (ii[2] * t.mle[1] + ii[1]) * t.mle[0] + ii[0];
+            count[box[i]]++;
+        }
+        std::vector<int> fill(nb0, 0);
+        for (int b = 1; b < nb0; ++b) fill[b] = fill[b - 1] + count[b - 1];
+        order.resize(n);
+        for (int i = 0; i < n; ++i) order[fill[box[i]]++] = i;
+    }
+
+    void level0(const Tree &t, std::vector<int> &sc, std::vector<int> &tc) override {
+        const int nsall = g.nsall(), n = g.n_src;
+        std::vector<double> all(3 * (size_t)nsall);
+        std::copy(g.r_src.begin(), g.r_src.end(), all.begin());
+        std::copy(g.img_pos.begin(), g.img_pos.end(), all.begin() + 3 * (size_t)n);
+        grid_sort(t, all, nsall, sorder, sc);
+        grid_sort(t, g.r_src, n, torder, tc);
+        spos.resize(3 * (size_t)nsall);
+        tpos.resize(3 * (size_t)n);
+        for (int i = 0; i < nsall; ++i)
+            for (int d = 0; d < 3; ++d) spos[3 * (size_t)i + d] = all[3 * (size_t)sorder[i] + d];
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) tpos[3 * (size_t)i + d] = g.r_src[3 * (size_t)torder[i] + d];
+    }
+
+    static void split_range(const double *cen, int begin, int count, std::vector<double> &pos, std::vector<int> &order,
+                            int child_begin[8], int child_count[8]) {
         std::vector<uint8_t> code(count);
         int cnt[8] = {0};
         for (int i = 0; i < count; ++i) {
@@ -176,104 +241,82 @@ struct Builder {
         }
     }
 
-    void split(int b) {
-        const int lvl = t.level[b];
-        const auto c = t.coord[b];
-        const int first = t.nboxes;
-        t.first_child[b] = first;
-        for (int k = 0; k < 8; ++k)
-            new_box(lvl + 1, b, {2 * c[0] + (k & 1), 2 * c[1] + ((k >> 1) & 1), 2 * c[2] + ((k >> 2) & 1)});
-        const double cen[3] = {t.center[3 * b], t.center[3 * b + 1], t.center[3 * b + 2]};
-        int cb[8], cc[8];
-        if (t.src_count[b] > 0) {
-            distribute(cen, t.src_begin[b], t.src_count[b], spos, t.src_order, cb, cc);
-            for (int k = 0; k < 8; ++k) { t.src_begin[first + k] = cb[k]; t.src_count[first + k] = cc[k]; }
+    void distribute(Tree &t, const std::vector<int> &boxes) override {
+        for (int b : boxes) {
+            const int first = t.first_child[b];
+            const double cen[3] = {t.center[3 * b], t.center[3 * b + 1], t.center[3 * b + 2]};
+            int cb[8], cc[8];
+            if (t.src_count[b] > 0) {
+                split_range(cen, t.src_begin[b], t.src_count[b], spos, sorder, cb, cc);
+                for (int k = 0; k < 8; ++k) { t.src_begin[first + k] = cb[k]; t.src_count[first + k] = cc[k]; }
+            }
+            if (t.trg_count[b] > 0) {
+                split_range(cen, t.trg_begin[b], t.trg_count[b], tpos, torder, cb, cc);
+                for (int k = 0; k < 8; ++k) { t.trg_begin[first + k] = cb[k]; t.trg_count[first + k] = cc[k]; }
+            }
         }
-        if (t.trg_count[b] > 0) {
-            distribute(cen, t.trg_begin[b], t.trg_count[b], tpos, t.trg_order, cb, cc);
-            for (int k = 0; k < 8; ++k) { t.trg_begin[first + k] = cb[k]; t.trg_count[first + k] = cc[k]; }
-        }
+    }
+
+    void finish(Tree &t) override {
+        t.src_order = std::move(sorder);
+        t.trg_order = std::move(torder);
+        t.src_sorted = std::move(spos);
+        t.trg_sorted = std::move(tpos);
     }
 };
-
-// counting sort of points into the level-0 grid, index order preserved within a box
-void grid_sort(const Tree &t, const std::vector<double> &pos, int n, std::vector<int> &order, std::vector<int> &begin,
-               std::vector<int> &count) {
-    const double inv = 1.0 / t.rc;
-    const int nb0 = t.mle[0] * t.mle[1] * t.mle[2];
-    std::vector<int> box(n);
-    count.assign(nb0, 0);
-    for (int i = 0; i < n; ++i) {
-        int ii[3];
-        for (int d = 0; d < 3; ++d) {
-            int v = (int)((pos[3 * (size_t)i + d] - t.xelo[d]) * inv);
-            ii[d] = std::min(std::max(v, 0), t.mle[d] - 1);
-        }
-        box[i] = (ii[2] * t.mle[1] + ii[1]) * t.mle[0] + ii[0];
-        count[box[i]]++;
-    }
-    begin.assign(nb0, 0);
-    for (int b = 1; b < nb0; ++b) begin[b] = begin[b - 1] + count[b - 1];
-    std::vector<int> fill = begin;
-    order.resize(n);
-    for (int i = 0; i < n; ++i) order[fill[box[i]]++] = i;
-}
 
 }  // namespace
 
 Tree build_tree(const Geometry &g, int n_per_leaf) {
+    HostPoints pts(g);
+    return build_tree_with(g, n_per_leaf, pts);
+}
+
+Tree build_tree_with(const Geometry &g, int n_per_leaf, PointBackend &pts) {
     if (n_per_leaf < 1) throw std::invalid_argument("PeriodicDMK: n_per_leaf must be positive");
     Tree t;
     t.rc = g.rc;
     for (int d = 0; d < 3; ++d) { t.mle[d] = g.mgrid[d] + 2; t.xelo[d] = g.xgrid0[d] - g.rc; }
-    const int nsall = g.nsall(), n = g.n_src;
 
-    // combined sources: originals then images
-    std::vector<double> all(3 * (size_t)nsall);
-    std::copy(g.r_src.begin(), g.r_src.end(), all.begin());
-    std::copy(g.img_pos.begin(), g.img_pos.end(), all.begin() + 3 * (size_t)n);
-
-    std::vector<int> sb, sc, tb, tc;
-    grid_sort(t, all, nsall, t.src_order, sb, sc);
-    grid_sort(t, g.r_src, n, t.trg_order, tb, tc);
-    std::vector<double> spos(3 * (size_t)nsall), tpos(3 * (size_t)n);
-    for (int i = 0; i < nsall; ++i)
-        for (int d = 0; d < 3; ++d) spos[3 * (size_t)i + d] = all[3 * (size_t)t.src_order[i] + d];
-    for (int i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) tpos[3 * (size_t)i + d] = g.r_src[3 * (size_t)t.trg_order[i] + d];
-
-    Builder B{t, g, spos, tpos};
+    Builder B{t};
     for (int z = 0; z < t.mle[2]; ++z)
         for (int y = 0; y < t.mle[1]; ++y)
-            for (int x = 0; x < t.mle[0]; ++x) {
-                const int id = B.new_box(0, -1, {x, y, z});
-                t.src_begin[id] = sb[id]; t.src_count[id] = sc[id];
-                t.trg_begin[id] = tb[id]; t.trg_count[id] = tc[id];
-            }
+            for (int x = 0; x < t.mle[0]; ++x) B.new_box(0, -1, {x, y, z});
+    {
+        std::vector<int> sc, tc;
+        pts.level0(t, sc, tc);
+        int so = 0, to = 0;
+        for (int id = 0; id < t.nboxes; ++id) {
+            t.src_begin[id] = so; t.src_count[id] = sc[id]; so += sc[id];
+            t.trg_begin[id] = to; t.trg_count[id] = tc[id]; to += tc[id];
+        }
+    }
 
-    // refinement while a leaf holds more than ceil(1.5 * n_per_leaf) sources
+    // refinement while a leaf holds more than ceil(1.5 * n_per_leaf) sources; the boxes of
+    // one level are split together, then their points are distributed
     const int limit = (int)std::ceil(n_per_leaf * 1.5);
     int max_level = 0;
     for (int lvl = 0; lvl < 50; ++lvl) {
         if ((int)t.by_level.size() <= lvl) break;
-        const std::vector<int> boxes = t.by_level[lvl];
-        bool any = false;
-        for (int b : boxes)
-            if (t.first_child[b] < 0 && t.src_count[b] > limit) { B.split(b); any = true; }
-        if (!any) break;
+        std::vector<int> split;
+        for (int b : t.by_level[lvl])
+            if (t.first_child[b] < 0 && t.src_count[b] > limit) split.push_back(b);
+        if (split.empty()) break;
+        for (int b : split) B.make_children(b);
+        pts.distribute(t, split);
         max_level = lvl + 1;
     }
 
-    // 2:1 balance: a non-empty leaf at level L touching a non-leaf box of level L+1 is split
+    // 2:1 balance: a non-empty leaf at level L touching a non-leaf box of level L+1 is split.
+    // Whether a leaf of level L violates depends only on boxes of levels L+1 and L+2, which
+    // the splits of level L do not change, so each level's splits are decided together.
     for (int iter = 0; iter < 50; ++iter) {
         bool changed = false;
         for (int L = 0; L <= max_level; ++L) {
             if ((int)t.by_level.size() <= L) break;
-            std::vector<int> leaves;
-            for (int b : t.by_level[L])
-                if (t.first_child[b] < 0 && t.src_count[b] > 0) leaves.push_back(b);
-            for (int b : leaves) {
-                if (t.first_child[b] >= 0) continue;
+            std::vector<int> split;
+            for (int b : t.by_level[L]) {
+                if (t.first_child[b] >= 0 || t.src_count[b] <= 0) continue;
                 bool violate = false;
                 for (int s = 0; s < 27 && !violate; ++s) {
                     const int cc = t.colleague(b, s);
@@ -284,11 +327,13 @@ Tree build_tree(const Geometry &g, int n_per_leaf) {
                         if (touching(L, t.coord[b], L + 1, t.coord[gc])) { violate = true; break; }
                     }
                 }
-                if (!violate) continue;
-                B.split(b);
-                changed = true;
-                max_level = std::max(max_level, L + 1);
+                if (violate) split.push_back(b);
             }
+            if (split.empty()) continue;
+            for (int b : split) B.make_children(b);
+            pts.distribute(t, split);
+            changed = true;
+            max_level = std::max(max_level, L + 1);
         }
         if (!changed) break;
     }
@@ -309,8 +354,7 @@ Tree build_tree(const Geometry &g, int n_per_leaf) {
             t.src_count[t.parent[b]] += t.src_count[b];
             t.trg_count[t.parent[b]] += t.trg_count[b];
         }
-    t.src_sorted = std::move(spos);
-    t.trg_sorted = std::move(tpos);
+    pts.finish(t);
 
     // plane-wave flags (pbc_tree.cpp:776-815) and leaf evaluation flags (817-831)
     t.pwexp.assign(nb, 0);

@@ -38,6 +38,14 @@ struct Geometry {
 // Lattice frame, cutoff cube size rc = min diag(umat)/nz, rotated positions and images.
 Geometry make_geometry(const double *cell_colmajor, const double *pos, int n, int nz);
 
+// make_geometry without the images.
+Geometry make_frame(const double *cell_colmajor, const double *pos, int n, int nz);
+
+// Image translations in make_geometry's enumeration order (shells k3, k2, k1); an image
+// r + t of a rotated position r is kept iff elo <= r + t <= ehi componentwise, and the kept
+// images are numbered translation by translation, points in index order within each.
+std::vector<std::array<double, 3>> image_shifts(const Geometry &g, double elo[3], double ehi[3]);
+
 // Cover of the cell's bounding box by (m+2) x (n+2) x (p+2) cubes of side s in the frame
 // (e1, e2, e3): e1 along the longest lattice vector, e2 in the plane of the two longest,
 // p = nz layers across the e3 width (optimize_cubic_cover,
@@ -93,6 +101,19 @@ struct Tree {
     std::unordered_map<Key, int, KeyHash> index;
 };
 
-Tree build_tree(const Geometry &g, int n_per_leaf);
+// Where the points live while the tree is built.  build_tree_with asks the backend for the
+// level-0 counts (boxes in grid order, points stable by index within a box), then, after
+// each batch of splits (all boxes of one level), for the begin/count of the new children
+// (stable distribution, child bit d set when x_d >= centre_d of the parent), and at the end
+// for the sorted order and positions.
+struct PointBackend {
+    virtual ~PointBackend() = default;
+    virtual void level0(const Tree &t, std::vector<int> &src_count, std::vector<int> &trg_count) = 0;
+    virtual void distribute(Tree &t, const std::vector<int> &split_boxes) = 0;
+    virtual void finish(Tree &t) = 0;  // src_order, trg_order, src_sorted, trg_sorted
+};
+
+Tree build_tree(const Geometry &g, int n_per_leaf);  // points on the host
+Tree build_tree_with(const Geometry &g, int n_per_leaf, PointBackend &points);
 
 }  // namespace pdmk::host
