@@ -1102,6 +1102,75 @@ __device__ __forceinline__ void nu_load(NuPoint<NP> &s, const double *__restrict
     s.pb = pbase[p + vz];
 }
 
+// ---- per-point plan data of the NUFFT (built once per tree)
+struct NufftGridArgs {
+    int n[3], ntile[3], T, lo, B, W, E, KS;
+    double h[3], beta;
+};
+
+__device__ __forceinline__ double nu_es_kernel(double z, double beta) {
+    return fabs(z) < 1.0 ? exp(beta * (sqrt(1.0 - z * z) - 1.0)) : 0.0;
+}
+
+// Grid coordinate x in [0, n), tile coordinate and first grid point of the support along
+// axis d of the point with angle theta.
+__device__ __forceinline__ void nu_axis(double theta, const NufftGridArgs &a, int d, double &x, int &tc, int &g0) {
+    const int n = a.n[d];
+    x = theta / a.h[d];
+    if (x < 0) x = 0;
+    if (x >= n) x = nextafter((double)n, 0.0);
+    tc = min(n - 1, (int)floor(x)) / a.T;
+    g0 = (int)floor(x - 0.5 * a.W) + 1;
+}
+
+// key[u] = tile B^3 + (bz B + by) B + bx; bad[0] = 1 if a base leaves [0, B).
+__global__ __launch_bounds__(256) void k_nufft_keys(const double *__restrict__ theta, int n, NufftGridArgs a,
+                                                    int *__restrict__ key, int *__restrict__ bad) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    int tc[3], b[3];
+    for (int d = 0; d < 3; ++d) {
+        double x;
+        int g0;
+        nu_axis(theta[3 * (size_t)u + d], a, d, x, tc[d], g0);
+        b[d] = g0 - a.T * tc[d] - a.lo;
+        if (b[d] < 0 || b[d] >= a.B) {
+            bad[0] = 1;
+            b[d] = 0;
+        }
+    }
+    const int tile = (tc[2] * a.ntile[1] + tc[1]) * a.ntile[0] + tc[0];
+    key[u] = tile * a.B * a.B * a.B + (b[2] * a.B + b[1]) * a.B + b[0];
+}
+
+// Plan-order rows: kval[i] = (phi_x placed at bx .. bx+W-1 of the E-cell row, phi_y, phi_z)
+// and pbase[i] for the point u = order[i].
+__global__ __launch_bounds__(256) void k_nufft_rows(const double *__restrict__ theta, const int *__restrict__ order, int n,
+                                                    NufftGridArgs a, double *__restrict__ kval, int *__restrict__ pbase) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int u = order[i];
+    double *row = kval + (size_t)i * a.KS;
+    int b[3];
+    for (int d = 0; d < 3; ++d) {
+        double x;
+        int tc, g0;
+        nu_axis(theta[3 * (size_t)u + d], a, d, x, tc, g0);
+        b[d] = min(max(g0 - a.T * tc - a.lo, 0), a.B - 1);
+        double *dst = d == 0 ? row + b[0] : row + a.E + (d - 1) * a.W;
+        if (d == 0)
+            for (int k = 0; k < a.E; ++k) row[k] = 0.0;
+        for (int k = 0; k < a.W; ++k) dst[k] = nu_es_kernel((x - (g0 + k)) / (0.5 * a.W), a.beta);
+    }
+    pbase[i] = b[0] | (b[1] << 8) | (b[2] << 16);
+}
+
+// cnt[tile]++ for every point (keys of k_nufft_keys).
+__global__ __launch_bounds__(256) void k_nufft_tile_hist(const int *__restrict__ key, int n, int B3, int *__restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[key[i] / B3], 1);
+}
+
 // qs[i] = q[order[i]]: the charges in plan order.
 __global__ __launch_bounds__(256) void k_nufft_permute(const double *__restrict__ q, const int *__restrict__ order, int n,
                                                        double *__restrict__ qs) {

@@ -821,7 +821,7 @@ void FourierSum::run(const double *d_q, double *d_out, hipStream_t s) const {
 void NufftFarField::setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
                           const std::vector<int> &mode_ijk, const std::vector<double> &mode_w, int width) {
     const int Mm[3] = {ms_, mt_, mu_};
-    const host::NufftPlanHost P = host::make_nufft_plan(theta_host, n_points, Mm, width);
+    host::NufftPlanHost P = host::make_nufft_grid(Mm, width);
     n = n_points;
     n_modes = (int)mode_w.size();
     W = P.W;
@@ -835,11 +835,52 @@ void NufftFarField::setup(const std::vector<double> &theta_host, int n_points, i
         M[d] = P.M[d];
         nt[d] = P.ntile[d];
     }
-    order.upload(P.order);
-    position.assign(n, 0);
-    for (int i = 0; i < n; ++i) position[P.order[i]] = i;
-    kval.upload(P.kval);
-    pbase.upload(P.pbase);
+    // per point on the GPU: keys (tile, base), plan order = stable sort by key, kernel-value
+    // rows and packed bases in plan order, points per tile
+    {
+        NufftGridArgs a{};
+        for (int d = 0; d < 3; ++d) {
+            a.n[d] = P.n[d];
+            a.ntile[d] = P.ntile[d];
+            a.h[d] = P.h[d];
+        }
+        a.T = P.T;
+        a.lo = P.lo;
+        a.B = P.B;
+        a.W = P.W;
+        a.E = P.E;
+        a.KS = P.KS;
+        a.beta = P.beta;
+        const int ntiles = P.ntile[0] * P.ntile[1] * P.ntile[2], B3 = P.B * P.B * P.B;
+        DevBuf th, key, bad, cnt;
+        th.upload(theta_host);
+        key.alloc(sizeof(int) * std::max(1, n));
+        bad.upload(std::vector<int>{0});
+        cnt.upload(std::vector<int>(ntiles, 0));
+        order.alloc(sizeof(int) * std::max(1, n));
+        kval.alloc(sizeof(double) * (size_t)P.KS * std::max(1, n));
+        pbase.alloc(sizeof(int) * std::max(1, n));
+        const unsigned nb = (unsigned)((n + 255) / 256);
+        hipStream_t s0 = nullptr;
+        hipLaunchKernelGGL(k_nufft_keys, dim3(nb), dim3(256), 0, s0, th.as<double>(), n, a, key.as<int>(), bad.as<int>());
+        check(hipGetLastError(), "k_nufft_keys");
+        int bits = 1;
+        while ((1L << bits) < (long)ntiles * B3) ++bits;
+        stable_order(key.as<int>(), n, bits, order.as<int>(), s0);
+        hipLaunchKernelGGL(k_nufft_rows, dim3(nb), dim3(256), 0, s0, th.as<double>(), order.as<int>(), n, a,
+                           kval.as<double>(), pbase.as<int>());
+        hipLaunchKernelGGL(k_nufft_tile_hist, dim3(nb), dim3(256), 0, s0, key.as<int>(), n, B3, cnt.as<int>());
+        check(hipGetLastError(), "k_nufft_rows");
+        int flag = 0;
+        std::vector<int> tile_count(ntiles), ord(n);
+        check(hipMemcpy(&flag, bad.p, sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        if (flag) throw std::logic_error("NUFFT base outside the tile subgrid");
+        check(hipMemcpy(tile_count.data(), cnt.p, sizeof(int) * ntiles, hipMemcpyDeviceToHost), "D2H");
+        check(hipMemcpy(ord.data(), order.p, sizeof(int) * n, hipMemcpyDeviceToHost), "D2H");
+        position.assign(n, 0);
+        for (int i = 0; i < n; ++i) position[ord[i]] = i;
+        host::nufft_units(P, tile_count);
+    }
     unit_ptr.upload(P.unit_ptr);
     unit_tile.upload(P.unit_tile);
     unit_slot.upload(P.unit_slot);

@@ -237,6 +237,22 @@ struct GpuPoints final : host::PointBackend {
 
 }  // namespace
 
+void stable_order(const int *key, int n, int end_bit, int *order, hipStream_t s) {
+    if (n <= 0) return;
+    DevBuf kout, vin, tmp;
+    kout.alloc(sizeof(int) * n);
+    vin.alloc(sizeof(int) * n);
+    hipLaunchKernelGGL(k_iota, dim3(blocks(n)), dim3(TB), 0, s, vin.as<int>(), n);
+    check(hipGetLastError(), "k_iota");
+    size_t bytes = 0;
+    check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key, kout.as<int>(), vin.as<int>(), order, n, 0, end_bit, s),
+          "radix sort size");
+    tmp.alloc(std::max(bytes, (size_t)1));
+    check(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key, kout.as<int>(), vin.as<int>(), order, n, 0, end_bit, s),
+          "radix sort");
+    check(hipStreamSynchronize(s), "stable_order");
+}
+
 bool build_tree_gpu(const double *cell, const double *pos, int n, int nz, int n_per_leaf, hipStream_t s,
                     host::Geometry &g, host::Tree &t) {
     g = host::make_frame(cell, pos, n, nz);

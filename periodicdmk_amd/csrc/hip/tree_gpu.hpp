@@ -20,4 +20,8 @@ namespace pdmk::gpu {
 bool build_tree_gpu(const double *cell_colmajor, const double *pos, int n, int nz, int n_per_leaf, hipStream_t s,
                     host::Geometry &g, host::Tree &t);
 
+// order[i] = the index of the i-th smallest of key[0 .. n) (stable: equal keys keep index
+// order), by an LSD radix sort over bits [0, end_bit).  Device pointers, enqueued on s.
+void stable_order(const int *key, int n, int end_bit, int *order, hipStream_t s);
+
 }  // namespace pdmk::gpu

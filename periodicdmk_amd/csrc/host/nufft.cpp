@@ -50,7 +50,7 @@ int nufft_tile_edge(const int M[3], int W) {
     return tiles >= 2048 ? 4 : 2;
 }
 
-NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const int Mm[3], int W) {
+NufftPlanHost make_nufft_grid(const int Mm[3], int W) {
     if (W < 4 || W > 14) throw std::invalid_argument("NUFFT kernel width out of range");
     NufftPlanHost P;
     const int T = nufft_tile_edge(Mm, W);
@@ -59,6 +59,7 @@ NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const in
     P.B = (W & 1) ? T + 1 : T;
     P.lo = 1 - (W + 1) / 2;
     P.E = P.B - 1 + W;
+    P.KS = P.E + 2 * W;
     P.beta = 2.30 * W;  // exponential-of-semicircle shape for an upsampling factor of 2
     subgrid_strides(P.E, W, P.SY, P.SZ);
     for (int d = 0; d < 3; ++d) {
@@ -67,78 +68,6 @@ NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const in
         P.n[d] = n;
         P.h[d] = kTwoPi / n;
         P.ntile[d] = n / T;
-    }
-    const int B = P.B;
-    // ---- per point: tile, base and kernel values
-    const int ntiles = P.ntile[0] * P.ntile[1] * P.ntile[2];
-    std::vector<int> key(np);  // tile * B^3 + base index
-    std::vector<double> kv(3 * (size_t)W * np);
-    for (int u = 0; u < np; ++u) {
-        int tc[3], b[3];
-        for (int d = 0; d < 3; ++d) {
-            const int n = P.n[d];
-            double x = theta[3 * (size_t)u + d] / P.h[d];  // grid coordinate in [0, n)
-            if (x < 0) x = 0;
-            if (x >= n) x = std::nextafter((double)n, 0.0);
-            const int c = std::min(n - 1, (int)std::floor(x));
-            tc[d] = c / T;
-            const int g0 = (int)std::floor(x - 0.5 * W) + 1;  // first grid point inside the support
-            b[d] = g0 - T * tc[d] - P.lo;
-            if (b[d] < 0 || b[d] >= B) throw std::logic_error("NUFFT base outside the tile subgrid");
-            for (int k = 0; k < W; ++k)
-                kv[((size_t)u * 3 + d) * W + k] = es_kernel((x - (g0 + k)) / (0.5 * W), P.beta);
-        }
-        const int tile = (tc[2] * P.ntile[1] + tc[1]) * P.ntile[0] + tc[0];
-        key[u] = tile * B * B * B + (b[2] * B + b[1]) * B + b[0];
-    }
-    // ---- counting sort by (tile, base), stable in the user order
-    const long nkeys = (long)ntiles * B * B * B;
-    std::vector<int> kstart(nkeys + 1, 0);
-    for (int u = 0; u < np; ++u) kstart[key[u] + 1]++;
-    for (long k = 0; k < nkeys; ++k) kstart[k + 1] += kstart[k];
-    P.order.resize(np);
-    {
-        std::vector<int> fill(kstart.begin(), kstart.end() - 1);
-        for (int u = 0; u < np; ++u) P.order[fill[key[u]]++] = u;
-    }
-    P.KS = P.E + 2 * W;
-    P.kval.assign((size_t)P.KS * np, 0.0);
-    P.pbase.resize(np);
-    for (int i = 0; i < np; ++i) {
-        const int u = P.order[i];
-        const int bi = key[u] % (B * B * B);
-        const int bx = bi % B;
-        P.pbase[i] = bx | (((bi / B) % B) << 8) | ((bi / (B * B)) << 16);
-        double *dst = P.kval.data() + (size_t)P.KS * i;
-        const double *src = kv.data() + 3 * (size_t)W * u;
-        for (int k = 0; k < W; ++k) {
-            dst[bx + k] = src[k];
-            dst[P.E + k] = src[W + k];
-            dst[P.E + W + k] = src[2 * W + k];
-        }
-    }
-    // ---- units: the points of a tile, split into equal parts of at most `part` points when a
-    // tile holds many more than the average
-    const int B3 = B * B * B;
-    int nonempty = 0;
-    for (int t = 0; t < ntiles; ++t) nonempty += kstart[(long)(t + 1) * B3] > kstart[(long)t * B3];
-    const int part = std::max(256, 4 * ((np + std::max(1, nonempty) - 1) / std::max(1, nonempty)));
-    P.unit_ptr.push_back(0);
-    int n_extra = 0;
-    P.extra_ptr.push_back(0);
-    for (int t = 0; t < ntiles; ++t) {
-        const int a = kstart[(long)t * B3], e = kstart[(long)(t + 1) * B3];
-        if (e == a) continue;
-        const int nparts = (e - a + part - 1) / part;
-        for (int k = 0; k < nparts; ++k) {
-            P.unit_ptr.push_back(a + (int)((long)(e - a) * (k + 1) / nparts));
-            P.unit_tile.push_back(t);
-            P.unit_slot.push_back(k == 0 ? t : ntiles + n_extra++);
-        }
-        if (nparts > 1) {
-            P.extra_tile.push_back(t);
-            P.extra_ptr.push_back(n_extra);
-        }
     }
     // ---- per axis: tiles whose subgrid covers grid index g, with the local offset
     for (int d = 0; d < 3; ++d) {
@@ -185,6 +114,36 @@ NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int np, const in
         }
     }
     return P;
+}
+
+void nufft_units(NufftPlanHost &P, const std::vector<int> &tile_count) {
+    // ---- units: the points of a tile, split into equal parts of at most `part` points when a
+    // tile holds many more than the average
+    const int ntiles = P.ntile[0] * P.ntile[1] * P.ntile[2];
+    if ((int)tile_count.size() != ntiles) throw std::logic_error("NUFFT tile count size");
+    std::vector<int> tstart(ntiles + 1, 0);
+    for (int t = 0; t < ntiles; ++t) tstart[t + 1] = tstart[t] + tile_count[t];
+    const int np = tstart[ntiles];
+    int nonempty = 0;
+    for (int t = 0; t < ntiles; ++t) nonempty += tile_count[t] > 0;
+    const int part = std::max(256, 4 * ((np + std::max(1, nonempty) - 1) / std::max(1, nonempty)));
+    P.unit_ptr.push_back(0);
+    int n_extra = 0;
+    P.extra_ptr.push_back(0);
+    for (int t = 0; t < ntiles; ++t) {
+        const int a = tstart[t], e = tstart[t + 1];
+        if (e == a) continue;
+        const int nparts = (e - a + part - 1) / part;
+        for (int k = 0; k < nparts; ++k) {
+            P.unit_ptr.push_back(a + (int)((long)(e - a) * (k + 1) / nparts));
+            P.unit_tile.push_back(t);
+            P.unit_slot.push_back(k == 0 ? t : ntiles + n_extra++);
+        }
+        if (nparts > 1) {
+            P.extra_tile.push_back(t);
+            P.extra_ptr.push_back(n_extra);
+        }
+    }
 }
 
 }  // namespace pdmk::host

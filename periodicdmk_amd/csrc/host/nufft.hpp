@@ -30,14 +30,11 @@ struct NufftPlanHost {
                                // wavefront hit distinct LDS banks
     int ntile[3] = {0, 0, 0};  // tiles per axis
     double beta = 0, h[3] = {0, 0, 0};
-    // points in plan order (tile, then base, then user order)
-    std::vector<int> order;     // user index per point
-    // KS = E + 2 W values per point: the kernel along x placed in its subgrid row (E values,
-    // phi_x at bx .. bx + W - 1, zero elsewhere), then ky and kz at the W points of its block
+    // per point, in plan order (tile, then base (bz, by, bx), then user order), built on the
+    // GPU: the user index, KS = E + 2 W kernel values (phi along x placed in its subgrid row,
+    // phi_x at bx .. bx + W - 1 and zero elsewhere, then phi_y and phi_z at the W points of
+    // its block) and the packed base bx | by << 8 | bz << 16
     int KS = 0;
-    std::vector<double> kval;
-    // packed base per point: bx | by << 8 | bz << 16
-    std::vector<int> pbase;
     // work units: a point range of one tile; unit_ptr has n_units + 1 entries
     std::vector<int> unit_ptr, unit_tile;
     // subgrid slot per unit: the tile index for the first unit of a tile, n_tiles + k for the
@@ -57,8 +54,12 @@ struct NufftPlanHost {
 // interpolation kernels take at most 5 bases per axis (T + 1 for odd W).
 int nufft_tile_edge(const int M[3], int W);
 
-// theta[3u + d] = 2 pi (fractional coordinate d of point u) in [0, 2 pi); modes per axis M[d];
-// W = kernel width (4 .. 14).
-NufftPlanHost make_nufft_plan(const std::vector<double> &theta, int n_points, const int M[3], int W);
+// Grid, tiles, subgrid strides, covering tables, DFT matrices and deconvolution for modes
+// per axis M[d] and kernel width W (4 .. 14): everything that does not depend on the points.
+NufftPlanHost make_nufft_grid(const int M[3], int W);
+
+// Work units from the number of points per tile (tiles z-major, points in plan order): the
+// points of a tile, split into equal parts when a tile holds far more than the average.
+void nufft_units(NufftPlanHost &P, const std::vector<int> &tile_count);
 
 }  // namespace pdmk::host
