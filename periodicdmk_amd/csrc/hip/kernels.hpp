@@ -286,7 +286,9 @@ constexpr int tf_slices(int total, int per_slice, int reserved = 0) {
 // output planes c' in [slice kc, slice kc + kc), kc <= T3Cfg::KC.
 template <class R, int P> struct T3Cfg {
     static constexpr int PP = P * P, TS = OddStride<P>::v;
-    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * PP * (int)sizeof(R));
+    static constexpr int VW = 16 / (int)sizeof(R);           // matrix entries per 128-bit LDS read
+    static constexpr int PR = (P + VW - 1) / VW * VW;        // padded matrix row
+    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * P * PR * (int)sizeof(R));
     static constexpr int threads = PP <= 384 ? (PP + 63) / 64 * 64 : (P >= 38 ? 512 : 256);
 };
 
@@ -296,21 +298,37 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
     const int *__restrict__ item_add, const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
     const int *__restrict__ in_code, const R *__restrict__ mats, int n_items, int kc) {
     constexpr int PP = P * P, TS = T3Cfg<R, P>::TS, KC = T3Cfg<R, P>::KC, NT = T3Cfg<R, P>::threads;
+    constexpr int VW = T3Cfg<R, P>::VW, PR = T3Cfg<R, P>::PR;
     constexpr int XC = (KC * P + NT - 1) / NT;
+    typedef R vec_t __attribute__((ext_vector_type(VW)));
     __shared__ R T1[KC * PP];       // [k][b][a]
     __shared__ R T2[KC * P * TS];   // [k][b'][a] rows of TS
-    __shared__ R Ms[2 * PP];        // minus / plus matrices
+    // U_s(r, c) = M_s[r + P c] (parent -> child) or M_s[c + P r] (child -> parent), stored
+    // row-major with rows padded to PR (zeros), so a row is read VW entries at a time
+    __shared__ __attribute__((aligned(16))) R Ur[2 * P * PR];
     const int it = blockIdx.x;
     if (it >= n_items) return;
     const int k0 = blockIdx.y * kc;
     if (k0 >= P) return;
-    for (int e = threadIdx.x; e < 2 * PP; e += NT) Ms[e] = mats[e];
+    for (int e = threadIdx.x; e < 2 * P * PR; e += NT) {
+        const int sm = e / (P * PR), r = (e / PR) % P, c = e % PR;
+        Ur[e] = c < P ? mats[sm * PP + (TRANSPOSE ? c + P * r : r + P * c)] : R(0);
+    }
     const int nk = min(kc, P - k0);
     R *out = out_base + (size_t)item_out[it] * P * PP;
     const bool add = item_add[it] != 0;
     const int e0 = item_ptr[it], e1 = item_ptr[it + 1];
-    // U_d(r, c) = M_s[r + P c] (parent -> child) or M_s[c + P r] (child -> parent)
-    constexpr int sr = TRANSPOSE ? P : 1, sc = TRANSPOSE ? 1 : P;
+    // s += sum_c U(r, c) x[c] for the row of U starting at urow
+    auto row_dot = [](const R *urow, const R (&x)[P], R s) {
+#pragma unroll
+        for (int c0 = 0; c0 < PR; c0 += VW) {
+            const vec_t u = *reinterpret_cast<const vec_t *>(urow + c0);
+#pragma unroll
+            for (int v = 0; v < VW; ++v)
+                if (c0 + v < P) s = fma(u[v], x[c0 + v], s);
+        }
+        return s;
+    };
     // x-step accumulators: thread column (k, b'), outputs a' -- persistent over children
     R acc[XC][P];
 #pragma unroll
@@ -320,19 +338,14 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
     for (int e = e0; e < e1; ++e) {
         const R *in = in_base + (size_t)in_slot[e] * P * PP;
         const int code = in_code[e];
-        const R *M0 = Ms + (code & 1) * PP, *M1 = Ms + ((code >> 1) & 1) * PP, *M2 = Ms + ((code >> 2) & 1) * PP;
+        const R *U0 = Ur + (code & 1) * P * PR, *U1 = Ur + ((code >> 1) & 1) * P * PR, *U2 = Ur + ((code >> 2) & 1) * P * PR;
         __syncthreads();
         // z: T1[k][b][a] = sum_c U2(k0 + k, c) in[c][b][a]
         for (int j = threadIdx.x; j < PP; j += NT) {
             R x[P];
 #pragma unroll
             for (int c = 0; c < P; ++c) x[c] = in[c * PP + j];
-            for (int k = 0; k < nk; ++k) {
-                R s = 0;
-#pragma unroll
-                for (int c = 0; c < P; ++c) s = fma(M2[(k0 + k) * sr + c * sc], x[c], s);
-                T1[k * PP + j] = s;
-            }
+            for (int k = 0; k < nk; ++k) T1[k * PP + j] = row_dot(U2 + (k0 + k) * PR, x, R(0));
         }
         __syncthreads();
         // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]
@@ -341,12 +354,7 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
             R x[P];
 #pragma unroll
             for (int b = 0; b < P; ++b) x[b] = T1[k * PP + b * P + a];
-            for (int b2 = 0; b2 < P; ++b2) {
-                R s = 0;
-#pragma unroll
-                for (int b = 0; b < P; ++b) s = fma(M1[b2 * sr + b * sc], x[b], s);
-                T2[(k * P + b2) * TS + a] = s;
-            }
+            for (int b2 = 0; b2 < P; ++b2) T2[(k * P + b2) * TS + a] = row_dot(U1 + b2 * PR, x, R(0));
         }
         __syncthreads();
         // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a], column (k, b')
@@ -358,12 +366,7 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
 #pragma unroll
                 for (int a = 0; a < P; ++a) x[a] = T2[w * TS + a];
 #pragma unroll
-                for (int a2 = 0; a2 < P; ++a2) {
-                    R s = acc[q][a2];
-#pragma unroll
-                    for (int a = 0; a < P; ++a) s = fma(M0[a2 * sr + a * sc], x[a], s);
-                    acc[q][a2] = s;
-                }
+                for (int a2 = 0; a2 < P; ++a2) acc[q][a2] = row_dot(U0 + a2 * PR, x, acc[q][a2]);
             }
         }
     }
