@@ -795,7 +795,7 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
 // Coordinates are relative to each point's leaf centre; geom[e] = (centre of the target
 // leaf - centre of the source leaf, bs), so r = |(t + off) - s|.  Each list entry is
 // summed in R and the per-target total in FP64.
-constexpr int NF_CHUNK = 8;
+constexpr int NF_CHUNK = 4;
 constexpr int NF_SPLIT_STRIDE = 16;  // coefficient stride of the even/odd table
 
 struct NfEntry { int sbeg, scnt, cbase, cfidx; };
