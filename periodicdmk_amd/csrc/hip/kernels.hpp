@@ -45,21 +45,24 @@ template <> __device__ __forceinline__ double rsqrt_full<double>(double x) { ret
 // ------------------------------------------------------------------ O(N) stages
 template <class R>
 __global__ void k_gather_charges(const double *__restrict__ q, const int *__restrict__ src_user, P4<R> *__restrict__ src,
-                                 int nsall, R *__restrict__ pot_sorted, int n) {
+                                 int nsall, R *__restrict__ pot_sorted, R *__restrict__ pot_near, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nsall) src[i].w = (R)q[src_user[i]];
-    if (i < n) pot_sorted[i] = R(0);
+    if (i < n) {
+        pot_sorted[i] = R(0);
+        pot_near[i] = R(0);
+    }
 }
 
 // pot[u] = pot_sorted[i] - w0_i q_u + far[u], u = trg_user[i]
 template <class R>
-__global__ void k_finalize(const R *__restrict__ pot_sorted, const double *__restrict__ w0, const int *__restrict__ trg_user,
-                           const double *__restrict__ q, const double *__restrict__ far, const int *__restrict__ far_idx,
-                           R *__restrict__ pot, int n, int use_self, int use_far) {
+__global__ void k_finalize(const R *__restrict__ pot_sorted, const R *__restrict__ pot_near, const double *__restrict__ w0,
+                           const int *__restrict__ trg_user, const double *__restrict__ q, const double *__restrict__ far,
+                           const int *__restrict__ far_idx, R *__restrict__ pot, int n, int use_self, int use_far) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int u = trg_user[i];
-    double v = (double)pot_sorted[i];
+    double v = (double)pot_sorted[i] + (double)pot_near[i];
     if (use_self) v -= w0[i] * q[u];
     if (use_far) v += far[far_idx[i]];
     pot[u] = (R)v;

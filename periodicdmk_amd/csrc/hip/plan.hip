@@ -74,6 +74,10 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     if (n_digits_user < 1 || n_digits_user > 12) throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
     check(hipStreamCreate(&stream), "hipStreamCreate");
     for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    check(hipStreamCreate(&s_far), "hipStreamCreate");
+    check(hipStreamCreate(&s_near), "hipStreamCreate");
+    for (hipEvent_t *e : {&e_in, &e_gather, &e_far, &e_near, &e_up})
+        check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
     using clk = std::chrono::steady_clock;
     const auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
     const auto t_start = clk::now();
@@ -90,6 +94,9 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         build_ms[1] = ms_since(t_start);
     } catch (...) {
         for (auto &e : ev) hipEventDestroy(e);
+        for (hipEvent_t e : {e_in, e_gather, e_far, e_near, e_up}) hipEventDestroy(e);
+        hipStreamDestroy(s_far);
+        hipStreamDestroy(s_near);
         hipStreamDestroy(stream);
         throw;
     }
@@ -167,6 +174,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     trg_user.upload(tu);
     trg_w0.upload(tw0);
     pot_sorted.alloc(sizeof(Real) * (size_t)n);
+    pot_near.alloc(sizeof(Real) * (size_t)n);
     q_dev.alloc(sizeof(double) * (size_t)n);
     pot_dev.alloc(sizeof(Real) * (size_t)n);
     far.alloc(sizeof(double) * (size_t)n);
@@ -532,11 +540,15 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
 
 template <typename Real>
 Plan<Real>::~Plan() {
-    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipStream_t st : {stream, s_far, s_near})
+        if (st) (void)hipStreamSynchronize(st);
     for (auto e : kev) (void)hipEventDestroy(e);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    for (hipEvent_t e : {e_in, e_gather, e_far, e_near, e_up})
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t st : {stream, s_far, s_near})
+        if (st) (void)hipStreamDestroy(st);
 }
 
 namespace {
@@ -590,6 +602,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
                            pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(), pl.n_up);
         check(hipGetLastError(), "k_proxy2pw");
     }
+    if (pl.e_upward) check(hipEventRecord(pl.e_upward, s), "hipEventRecord");
     for (size_t l = 0; l < pl.eval_levels.size(); ++l) {
         auto &E = pl.eval_levels[l];
         const Real lscale = (Real)std::ldexp(1.0, E.level);
@@ -633,7 +646,7 @@ void launch_near(Plan<Real> &pl, hipStream_t s) {
                        pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_items.template as<int4>(),
                        pl.nf_wbox.template as<P4<Real>>(), pl.nf_ptr.template as<int>(), pl.nf_l1.template as<NfEntry>(),
                        pl.nf_geom.template as<P4<Real>>(), pl.nf_cbox.template as<P4<Real>>(),
-                       pl.coeffs.template as<Real>(), pl.pot_sorted.template as<Real>(), pl.n_nf_items);
+                       pl.coeffs.template as<Real>(), pl.pot_near.template as<Real>(), pl.n_nf_items);
     check(hipGetLastError(), "k_near_field");
 }
 
@@ -643,26 +656,70 @@ template <typename Real>
 void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int stages, double *times_ms) {
     if (!s) s = stream;
     hipEvent_t *K = nullptr;
-    if (kslots > 0) K = &kev[(size_t)(kcall++ % kslots) * (kGroups + 1)];
-    auto mark = [&](int g) {
-        if (K) check(hipEventRecord(K[g], s), "hipEventRecord");
+    if (kslots > 0) K = &kev[(size_t)(kcall++ % kslots) * 2 * kGroups];
+    auto mark = [&](int g, int end, hipStream_t st) {
+        if (K) check(hipEventRecord(K[2 * g + end], st), "hipEventRecord");
     };
-    if (times_ms) check(hipEventRecord(ev[0], s), "hipEventRecord");
-    mark(0);
-    // far field first: it depends only on the user-order charges
-    if (stages & 8) {
-        if (far_nufft) nf.run(d_q, far.as<double>(), s);
-        else ff.run(d_q, far.as<double>(), s);
+    // evaluate_timed's split (far field, proxy/plane waves, near field) needs the stages one
+    // after the other; otherwise the far field and the near field run on their own streams
+    // beside the transform chain and join before the final combination
+    const bool serial = times_ms != nullptr;
+    hipStream_t sf = serial ? s : s_far, sn = serial ? s : s_near;
+    if (!serial) {
+        check(hipEventRecord(e_in, s), "hipEventRecord");
+        check(hipStreamWaitEvent(sf, e_in, 0), "hipStreamWaitEvent");
     }
-    mark(1);
+    if (times_ms) check(hipEventRecord(ev[0], s), "hipEventRecord");
+    // far field: depends only on the user-order charges
+    mark(0, 0, sf);
+    if (stages & 8) {
+        if (far_nufft) nf.run(d_q, far.as<double>(), sf);
+        else ff.run(d_q, far.as<double>(), sf);
+    }
+    mark(0, 1, sf);
+    if (!serial) check(hipEventRecord(e_far, sf), "hipEventRecord");
     if (times_ms) check(hipEventRecord(ev[1], s), "hipEventRecord");
+    mark(1, 0, s);
     {
         const int m = std::max(nsall, n);
         hipLaunchKernelGGL(k_gather_charges<Real>, dim3((m + 255) / 256), dim3(256), 0, s, d_q, src_user.as<int>(),
-                           src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), n);
+                           src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), pot_near.as<Real>(), n);
         check(hipGetLastError(), "k_gather_charges");
     }
-    mark(2);
+    mark(1, 1, s);
+    if (!serial) {
+        check(hipEventRecord(e_gather, s), "hipEventRecord");
+        check(hipStreamWaitEvent(sn, e_gather, 0), "hipStreamWaitEvent");
+    }
+    auto near_field = [&] {
+        mark(3, 0, sn);
+        if (stages & 2) {
+            switch (n_digits) {
+            case 1: case 2: launch_near<Real, 5, true>(*this, sn); break;
+            case 3: launch_near<Real, 6, true>(*this, sn); break;
+            case 4: launch_near<Real, 9, true>(*this, sn); break;
+            case 5: launch_near<Real, 11, true>(*this, sn); break;
+            case 6: launch_near<Real, 12, false>(*this, sn); break;
+            case 7: launch_near<Real, 15, false>(*this, sn); break;
+            case 8: launch_near<Real, 17, false>(*this, sn); break;
+            case 9: launch_near<Real, 18, false>(*this, sn); break;
+            case 10: launch_near<Real, 22, false>(*this, sn); break;
+            case 11: launch_near<Real, 23, false>(*this, sn); break;
+            case 12: launch_near<Real, 25, false>(*this, sn); break;
+            default: throw std::logic_error("unsupported digit count");
+            }
+        }
+        mark(3, 1, sn);
+    };
+    // Start of the near field: at once when its cost is comparable to the transform chain's
+    // (p < 28); at p >= 28 the chain is several times longer and its first kernels (proxy
+    // formation on the matrix cores, proxy -> plane wave) fill the GPU, so the near field
+    // starts after them and overlaps the latency-bound rest (measured at N=1e6: 9 digits
+    // 17.9 -> 17.2 ms; 6 digits unchanged or slower with a later start).
+    const bool near_late = !serial && p >= 28;
+    e_upward = near_late ? e_up : nullptr;
+    if (!serial && !near_late) near_field();
+    mark(2, 0, s);
     if (stages & 1) {
         switch (p) {
         case 9: launch_transforms<Real, 9, 13>(*this, s); break;
@@ -672,30 +729,24 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
         default: throw std::logic_error("unsupported proxy order");
         }
     }
-    mark(3);
+    mark(2, 1, s);
     if (times_ms) check(hipEventRecord(ev[2], s), "hipEventRecord");
-    if (stages & 2) {
-        switch (n_digits) {
-        case 1: case 2: launch_near<Real, 5, true>(*this, s); break;
-        case 3: launch_near<Real, 6, true>(*this, s); break;
-        case 4: launch_near<Real, 9, true>(*this, s); break;
-        case 5: launch_near<Real, 11, true>(*this, s); break;
-        case 6: launch_near<Real, 12, false>(*this, s); break;
-        case 7: launch_near<Real, 15, false>(*this, s); break;
-        case 8: launch_near<Real, 17, false>(*this, s); break;
-        case 9: launch_near<Real, 18, false>(*this, s); break;
-        case 10: launch_near<Real, 22, false>(*this, s); break;
-        case 11: launch_near<Real, 23, false>(*this, s); break;
-        case 12: launch_near<Real, 25, false>(*this, s); break;
-        default: throw std::logic_error("unsupported digit count");
-        }
+    if (serial) near_field();
+    if (near_late) {
+        check(hipStreamWaitEvent(sn, e_up, 0), "hipStreamWaitEvent");
+        near_field();
     }
-    mark(4);
-    hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(), trg_w0.as<double>(),
-                       trg_user.as<int>(), d_q, far.as<double>(), far_idx.as<int>(), d_pot, n, (stages & 4) ? 1 : 0,
-                       (stages & 8) ? 1 : 0);
+    if (!serial) {
+        check(hipEventRecord(e_near, sn), "hipEventRecord");
+        check(hipStreamWaitEvent(s, e_near, 0), "hipStreamWaitEvent");
+        check(hipStreamWaitEvent(s, e_far, 0), "hipStreamWaitEvent");
+    }
+    mark(4, 0, s);
+    hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(), pot_near.as<Real>(),
+                       trg_w0.as<double>(), trg_user.as<int>(), d_q, far.as<double>(), far_idx.as<int>(), d_pot, n,
+                       (stages & 4) ? 1 : 0, (stages & 8) ? 1 : 0);
     check(hipGetLastError(), "k_finalize");
-    mark(5);
+    mark(4, 1, s);
     if (times_ms) {
         check(hipEventRecord(ev[3], s), "hipEventRecord");
         check(hipEventSynchronize(ev[3]), "hipEventSynchronize");
@@ -723,7 +774,7 @@ void Plan<Real>::enable_kernel_timing(int slots) {
     kev.clear();
     kslots = std::max(0, slots);
     kcall = 0;
-    kev.resize((size_t)kslots * (kGroups + 1));
+    kev.resize((size_t)kslots * 2 * kGroups);
     for (auto &e : kev) check(hipEventCreate(&e), "hipEventCreate");
 }
 
@@ -734,7 +785,7 @@ int Plan<Real>::kernel_times(double *avg_ms, int *n_calls) {
     for (int c = 0; c < nc; ++c)
         for (int g = 0; g < kGroups; ++g) {
             float ms = 0;
-            check(hipEventElapsedTime(&ms, kev[(size_t)c * (kGroups + 1) + g], kev[(size_t)c * (kGroups + 1) + g + 1]),
+            check(hipEventElapsedTime(&ms, kev[(size_t)c * 2 * kGroups + 2 * g], kev[(size_t)c * 2 * kGroups + 2 * g + 1]),
                   "hipEventElapsedTime");
             avg_ms[g] += ms / nc;
         }

@@ -86,9 +86,14 @@ struct Plan {
     Work work;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // concurrent evaluate(): far field and near field on their own streams (default
+    // priority), joined before the final combination
+    hipStream_t s_far = nullptr, s_near = nullptr;
+    hipEvent_t e_in = nullptr, e_gather = nullptr, e_far = nullptr, e_near = nullptr, e_up = nullptr;
+    hipEvent_t e_upward = nullptr;  // when set, recorded by the transform chain after the upward pass
 
     // O(N) data
-    DevBuf src, src_user, trg, trg_user, trg_w0, pot_sorted, q_dev, pot_dev, far, far_idx;
+    DevBuf src, src_user, trg, trg_user, trg_w0, pot_sorted, pot_near, q_dev, pot_dev, far, far_idx;
     // upward
     DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, rng_off, proxy_up;
     struct Tensor3Launch { int n_items; DevBuf out, add, ptr, in_slot, in_code; };
@@ -131,10 +136,10 @@ struct Plan {
     // host buffers in, host buffers out
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
 
-    // Per-kernel-group timing with HIP events on the evaluate stream.  Groups:
-    // 0 far field, 1 charge gather, 2 proxy/plane-wave transforms, 3 near field,
-    // 4 finalize.  A ring of `slots` calls is kept; kernel_times() averages the
-    // recorded calls (after the caller has synchronized the stream).
+    // Per-kernel-group timing with HIP events on the stream each group runs on (start and
+    // end event per group).  Groups: 0 far field, 1 charge gather, 2 proxy/plane-wave
+    // transforms, 3 near field, 4 finalize.  A ring of `slots` calls is kept;
+    // kernel_times() averages the recorded calls (after the caller has synchronized).
     static constexpr int kGroups = 5;
     void enable_kernel_timing(int slots);
     int kernel_times(double *avg_ms, int *n_calls);
