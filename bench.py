@@ -50,6 +50,10 @@ def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, char
     wall = time.perf_counter() - t0
     ktimes, ncalls = s.kernel_times()
     ms = 1000.0 * wall / steps
+    # potentials of the last step, for the error check against the CPU oracle (read before
+    # the pass below overwrites them)
+    last_q = (steps - 1) % n_vec
+    pot = out.numpy().astype("float64")
     # the near-field kernel alone (stage bit 2 only), for its duration without the other
     # streams of evaluate() competing for the CUs
     s.enable_kernel_timing(max(1, steps))
@@ -57,14 +61,11 @@ def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, char
         s.evaluate_stages(charges_dev.offset((k % n_vec) * n), out.ptr, 2)
     P.synchronize()
     near_alone_ms = s.kernel_times()[0]["near_field"]
-    # potentials of the last step, for the error check against the CPU oracle
-    last_q = (steps - 1) % n_vec
-    pot = out.numpy().astype("float64")
     out.free()
     res = {
         "precision": precision, "n_digits": n_digits, "ms_per_step": ms, "mpts_per_s": n / (ms / 1000.0) / 1e6,
         "build_s": build_s, "build": s.build_info(), "kernel_ms": ktimes, "near_field_alone_ms": near_alone_ms,
-        "stats": s.stats(), "work": s.work(),
+        "stats": s.stats(), "work": s.work(), "nf_steps": s.stats()["n_nf_steps"],
         "last_charge_index": last_q,
     }
     s.close()
@@ -225,7 +226,14 @@ def main():
     }
     if "mixed6" in results and "fp64_6" in results:
         line["mixed_vs_fp64_6"] = round(results["fp64_6"]["ms_per_step"] / results["mixed6"]["ms_per_step"], 3)
+    # a throughput of wrong potentials is no measurement: flag any path whose potentials are
+    # off the CPU oracle by more than 100x its digit target, and exit non-zero
+    bad = [nm for nm in wanted if results[nm].get("err_vs_cpu_oracle_rel_l2", 0.0) > 10.0 ** (2 - paths[nm][1])]
+    line["accuracy_ok"] = None if args.no_cpu_baseline else not bad
     print(json.dumps(line))
+    if bad:
+        print(f"[bench] ERROR: potentials of {bad} disagree with the CPU oracle", file=sys.stderr, flush=True)
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -47,13 +47,15 @@ int pdmk_gpu_evaluate_stages(void* h, const double* d_charges, void* d_pot, int 
 
 /// Tree and work statistics: out[0..15] = {n_src, nsall, nboxes, n_levels, p, n_pw,
 /// ms, mt, mu, n_leaves, n_list1, n_direct_leaves, n_far_modes, n_digits,
-/// is_float, n_pair_tiles}.
+/// is_float, n_nf_steps}.
 int pdmk_gpu_stats(void* h, long long* out16);
 
 /// Construction of the handle: out[0..3] = {total ms, tree ms (geometry, images, tree),
 /// far-field plan ms, 1 if the tree's per-point work (images, binning, distribution, sort)
-/// ran on the GPU else 0}.  No reference counterpart (measurement hook).
-int pdmk_gpu_build_info(void* h, double* out4);
+/// ran on the GPU else 0, boxes folded onto a periodic counterpart (their proxies and
+/// outgoing plane waves are not computed), proxy slots computed}.  No reference counterpart
+/// (measurement hook).
+int pdmk_gpu_build_info(void* h, double* out6);
 
 /// Algorithmic work per evaluate(): out[0..7] = {near-field pairs in support,
 /// near-field candidate pairs, near-field flops (in-support pairs x (2*NC+12)),

@@ -26,7 +26,7 @@ _lib = None
 
 STAGE_PW, STAGE_DIRECT, STAGE_SELF, STAGE_FAR = 1, 2, 4, 8
 STATS_KEYS = ("n_src", "nsall", "nboxes", "n_levels", "p", "n_pw", "ms", "mt", "mu", "n_leaves", "n_list1",
-              "n_direct_leaves", "n_far_modes", "n_digits", "is_float", "n_pair_tiles")
+              "n_direct_leaves", "n_far_modes", "n_digits", "is_float", "n_nf_steps")
 WORK_KEYS = ("nf_pairs_support", "nf_pairs_candidate", "nf_flops", "transform_flops", "ff_mode_points", "ff_flops",
              "on_bytes", "nf_bytes")
 
@@ -263,9 +263,10 @@ class PDMKSolver:
 
     def build_info(self):
         """Construction times (ms) and whether the tree's per-point work ran on the GPU."""
-        o = np.zeros(4)
+        o = np.zeros(6)
         lib().pdmk_gpu_build_info(self._h, _d(o))
-        return {"build_ms": float(o[0]), "tree_ms": float(o[1]), "far_plan_ms": float(o[2]), "tree_on_gpu": bool(o[3])}
+        return {"build_ms": float(o[0]), "tree_ms": float(o[1]), "far_plan_ms": float(o[2]), "tree_on_gpu": bool(o[3]),
+                "folded_boxes": int(o[4]), "proxy_slots": int(o[5])}
 
     def work(self):
         o = np.zeros(8)

@@ -312,7 +312,7 @@ int pdmk_gpu_stats(void *h, long long *o) {
         static_cast<Handle *>(h)->is_float ? tree_of<float>(h)->plan().stats : tree_of<double>(h)->plan().stats;
     const long long v[16] = {s.n_src,    s.nsall,     s.nboxes,          s.n_levels,    s.p,        s.n_pw,
                              s.ms,       s.mt,        s.mu,              s.n_leaves,    s.n_list1,  s.n_direct_leaves,
-                             s.n_far_modes, s.n_digits, s.is_float,      s.n_pair_tiles};
+                             s.n_far_modes, s.n_digits, s.is_float,      s.n_nf_steps};
     std::memcpy(o, v, sizeof(v));
     return 0;
 }
@@ -322,7 +322,9 @@ int pdmk_gpu_build_info(void *h, double *o) {
     const bool f = static_cast<Handle *>(h)->is_float;
     const double *ms = f ? tree_of<float>(h)->plan().build_ms : tree_of<double>(h)->plan().build_ms;
     const bool gpu = f ? tree_of<float>(h)->plan().tree_on_gpu : tree_of<double>(h)->plan().tree_on_gpu;
-    const double v[4] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0};
+    const int folded = f ? tree_of<float>(h)->plan().n_folded : tree_of<double>(h)->plan().n_folded;
+    const int slots = f ? tree_of<float>(h)->plan().n_up : tree_of<double>(h)->plan().n_up;
+    const double v[6] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0, (double)folded, (double)slots};
     std::memcpy(o, v, sizeof(v));
     return 0;
 }

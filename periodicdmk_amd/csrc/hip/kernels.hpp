@@ -38,6 +38,10 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + min(x, r) + i;
 }
 
+// Kernels of the proxy / plane-wave chain raise their waves' issue priority: evaluate()
+// runs them beside the near-field kernel, and the chain is the longer path.
+__device__ __forceinline__ void chain_priority() { __builtin_amdgcn_s_setprio(2); }
+
 template <class R> __device__ __forceinline__ R rsqrt_full(R x);
 template <> __device__ __forceinline__ float rsqrt_full<float>(float x) { return __frsqrt_rn(x); }
 template <> __device__ __forceinline__ double rsqrt_full<double>(double x) { return rsqrt(x); }
@@ -92,6 +96,7 @@ __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
     const P4<R> *__restrict__ src, const int *__restrict__ grp_slot, const R *__restrict__ grp_geom,
     const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
     const R *__restrict__ rng_off, R *__restrict__ proxy) {
+    chain_priority();
     constexpr int RB = C2pCfg<P>::RB, NB = C2pCfg<P>::NB, NT = C2pCfg<P>::threads;
     constexpr int PX = (P + 3) & ~3, PY = NB * RB;  // padded table rows
     __shared__ R Tx[C2P_TS * PX], Ty[C2P_TS * PY], Tz[C2P_TS * P];
@@ -183,6 +188,7 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
     const P4<double> *__restrict__ src, const int *__restrict__ grp_slot, const double *__restrict__ grp_geom,
     const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
     const double *__restrict__ rng_off, double *__restrict__ proxy) {
+    chain_priority();
     using C = C2pMfmaCfg<P>;
     constexpr int TS = C::TS, PA = C::PA, RT = C::RT, NW = C::NW, TPW = C::TPW;
     static_assert(P <= PA, "order above the padded column count");
@@ -300,6 +306,7 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
     const R *__restrict__ in_base, R *__restrict__ out_base, const int *__restrict__ item_out,
     const int *__restrict__ item_add, const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
     const int *__restrict__ in_code, const R *__restrict__ mats, int n_items, int kc) {
+    chain_priority();
     constexpr int PP = P * P, TS = T3Cfg<R, P>::TS, KC = T3Cfg<R, P>::KC, NT = T3Cfg<R, P>::threads;
     constexpr int VW = T3Cfg<R, P>::VW, PR = T3Cfg<R, P>::PR;
     constexpr int XC = (KC * P + NT - 1) / NT;
@@ -390,6 +397,7 @@ template <class R, int P>
 __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, const R *__restrict__ scratch,
                                                        const int *__restrict__ parent_slot, const int *__restrict__ child_ptr,
                                                        int n_parents) {
+    chain_priority();
     constexpr int P3 = P * P * P;
     const int it = blockIdx.x, e = blockIdx.y * blockDim.x + threadIdx.x;
     if (it >= n_parents || e >= P3) return;
@@ -406,6 +414,7 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
                                                    const Cx<R> *__restrict__ W, int n_boxes) {
+    chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, TS = OddStride<P>::v;
     constexpr int K3 = tf_slices(H, (PP + NPW * TS) * (int)sizeof(Cx<R>));
     __shared__ Cx<R> T1[K3 * PP];        // [k][b][a]
@@ -480,6 +489,7 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
                                                    R lscale, const int *__restrict__ nbr_ptr,
                                                    const int *__restrict__ nbr_slot, const int *__restrict__ nbr_sten,
                                                    int nm, int item0, Cx<R> *__restrict__ pin) {
+    chain_priority();
     const int it = item0 + blockIdx.y;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nm) return;
@@ -500,6 +510,7 @@ template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
                                                    int item0, int n_items, R *__restrict__ down_base) {
+    chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, half = NPW / 2, TS2 = OddStride<P>::v;
     constexpr int AC = tf_slices(P, (COL + H * TS2) * (int)sizeof(Cx<R>));
     __shared__ Cx<R> U1[AC * COL];       // [a][k3][k2]
@@ -584,6 +595,7 @@ template <class R, int P>
 __global__ __launch_bounds__(EV_THREADS) void k_eval_targets(const R *__restrict__ down_base, const int *__restrict__ leaf_down,
                                                               const R *__restrict__ leaf_geom, const int *__restrict__ leaf_trg,
                                                               const P4<R> *__restrict__ trg, R *__restrict__ pot_sorted) {
+    chain_priority();
     constexpr int PP = P * P;
     __shared__ R plane[PP];
     const int lf = blockIdx.x;
@@ -679,6 +691,7 @@ template <int P>
 __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
     const double *__restrict__ down_base, const int *__restrict__ leaf_down, const double *__restrict__ leaf_geom,
     const int *__restrict__ leaf_trg, const P4<double> *__restrict__ trg, double *__restrict__ pot_sorted) {
+    chain_priority();
     using C = EvMfmaCfg<P>;
     constexpr int RT = C::RT, NW = C::NW, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
     static_assert(P <= PA, "order above the padded a count");
@@ -786,56 +799,60 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
 // does.
 //
 // Work item = one wavefront = up to 64 targets of one leaf (consecutive in the leaf's
-// Morton order, so the item is spatially compact; wbox = its bounding box).  Sources of
-// every leaf are cut into chunks of NF_CHUNK consecutive (Morton-ordered) points with
-// bounding boxes cbox.  For each list1 entry the 64 lanes test 64 chunks at once against
-// the item's box (shifted into the source leaf's frame) and only chunks that can hold an
-// in-support pair are walked; the test is evaluated with the same rounded operations as
-// the per-pair r^2, so it never drops a pair the per-pair test would keep.  Sources are
-// read with wave-uniform addresses; the polynomial is skipped when no lane of the wave is
-// in support for a source.
+// Morton order, so the item is spatially compact).  The geometry is fixed for the life of a
+// tree, so which sources can reach which wavefront is decided once, at construction
+// (k_near_steps): a step is one source of one list1 entry for which at least one lane of
+// the wavefront is in support.  The evaluation walks only the steps, in the order the
+// construction found them, and applies the per-lane support test to each; sources are read
+// with wave-uniform addresses.
 //
 // Coordinates are relative to each point's leaf centre; geom[e] = (centre of the target
 // leaf - centre of the source leaf, bs), so r = |(t + off) - s|.  Each list entry is
 // summed in R and the per-target total in FP64.
 constexpr int NF_CHUNK = 4;
 constexpr int NF_SPLIT_STRIDE = 16;  // coefficient stride of the even/odd table
+// steps whose sources are loaded together (scalar loads: U points in SGPRs); the step list
+// is padded by NF_STEP_PAD valid indices so a group may read past the end of its run
+template <class R> constexpr int nf_unroll() { return sizeof(R) == 4 ? 8 : 4; }
+constexpr int NF_STEP_PAD = 16;
 
 struct NfEntry { int sbeg, scnt, cbase, cfidx; };
 
-template <class R, int NC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
+// Step lists, once per tree.  Sources of every leaf are cut into chunks of NF_CHUNK
+// consecutive (Morton-ordered) points with bounding boxes cbox; for each list1 entry the 64
+// lanes test 64 chunks at once against the item's box wbox (shifted into the source leaf's
+// frame), evaluated with the same rounded operations as the per-pair r^2 so that it never
+// drops a pair the per-pair test keeps; the sources of kept chunks are then tested per lane.
+// PASS 0 writes the step count of the k-th entry of item i to cnt[ie_base[i] + k] and adds
+// the in-support pairs to *pairs; PASS 1 writes the steps' source indices from
+// ie_ptr[ie_base[i] + k] on.
+template <class R, int PASS>
+__global__ __launch_bounds__(256) void k_near_steps(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
                                                      const int4 *__restrict__ items, const P4<R> *__restrict__ wbox,
                                                      const int *__restrict__ l1_ptr, const NfEntry *__restrict__ l1,
                                                      const P4<R> *__restrict__ geom, const P4<R> *__restrict__ cbox,
-                                                     const R *__restrict__ coeffs, R *__restrict__ pot_sorted, int n_items) {
-    // wave index made wave-uniform explicitly, so everything derived from it lives in SGPRs
-    const int item = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+                                                     const int *__restrict__ ie_base, int *__restrict__ cnt,
+                                                     const long long *__restrict__ ie_ptr, int *__restrict__ step_src,
+                                                     unsigned long long *__restrict__ pairs, int n_items) {
+    const int item = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (item >= n_items) return;  // whole wavefront
     const int lane = threadIdx.x & 63;
-    const int4 it = items[item];  // (first target, count, target-leaf list index, -)
+    const int4 it = items[item];
     const bool active = lane < it.y;
     const P4<R> me = active ? trg[it.x + lane] : P4<R>{R(1e18), R(1e18), R(1e18), R(0)};
     const P4<R> blo = wbox[2 * item], bhi = wbox[2 * item + 1];
-    R c[NC];
-    if (!SPLIT) {
-#pragma unroll
-        for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
-    }
-    double acc = 0;
-    for (int e = l1_ptr[it.z]; e < l1_ptr[it.z + 1]; ++e) {
+    const int eb = l1_ptr[it.z], base = ie_base[item];
+    unsigned long long np = 0;
+    for (int e = eb; e < l1_ptr[it.z + 1]; ++e) {
         const NfEntry E = l1[e];
         const P4<R> gm = geom[e];
-        const R bs = gm.w, d2max = bs * bs, rsc = R(2) / bs, cen = -bs / R(2);
+        const R bs = gm.w, d2max = bs * bs;
         const R tx = me.x + gm.x, ty = me.y + gm.y, tz = me.z + gm.z;
         const R lx = blo.x + gm.x, ly = blo.y + gm.y, lz = blo.z + gm.z;
         const R hx = bhi.x + gm.x, hy = bhi.y + gm.y, hz = bhi.z + gm.z;
-        if (SPLIT) {
-#pragma unroll
-            for (int i = 0; i < NC; ++i) c[i] = coeffs[E.cfidx * NF_SPLIT_STRIDE + i];
-        }
         const int nch = (E.scnt + NF_CHUNK - 1) / NF_CHUNK;
-        R ea = 0;
+        long long out = PASS == 1 ? ie_ptr[base + e - eb] : 0;
+        int steps = 0;
         for (int c0 = 0; c0 < nch; c0 += 64) {
             bool keep = false;
             if (c0 + lane < nch) {
@@ -851,40 +868,110 @@ __global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ sr
                 mask &= mask - 1;
                 const int s0 = E.sbeg + (c0 + k) * NF_CHUNK;
                 const int ns = min(NF_CHUNK, E.sbeg + E.scnt - s0);
-                // the source array is padded by NF_CHUNK points, so a whole chunk is always
-                // readable; points past the leaf's range are masked by j < ns
-                P4<R> qs[NF_CHUNK];
-#pragma unroll
-                for (int j = 0; j < NF_CHUNK; ++j) qs[j] = src[s0 + j];
-#pragma unroll
-                for (int j = 0; j < NF_CHUNK; ++j) {
-                    {
-                        const P4<R> q = qs[j];
-                        const R dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
-                        const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                        if (j < ns && r2 < d2max && r2 > R(1e-30)) {
-                            const R rinv = rsqrt_full<R>(r2);
-                            R u;
-                            if (SPLIT) {
-                                constexpr int SE = (NC - 1) & ~1, SO = ((NC - 2) & ~1) + 1;
-                                R Ev = c[SE];
-#pragma unroll
-                                for (int i = SE - 2; i >= 0; i -= 2) Ev = fma(Ev, r2, c[i]);
-                                R Od = c[SO];
-#pragma unroll
-                                for (int i = SO - 2; i >= 0; i -= 2) Od = fma(Od, r2, c[i]);
-                                u = fma(Ev, rinv, Od);
-                            } else {
-                                const R x = fma(r2, rinv, cen) * rsc;
-                                R p = c[NC - 1];
-#pragma unroll
-                                for (int i = NC - 2; i >= 0; --i) p = fma(p, x, c[i]);
-                                u = p * rinv;
-                            }
-                            ea = fma(u, q.w, ea);
-                        }
+                for (int j = 0; j < ns; ++j) {
+                    const P4<R> q = src[s0 + j];
+                    const R dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
+                    const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const unsigned long long in = __ballot(r2 < d2max && r2 > R(1e-30));
+                    if (in) {
+                        if (PASS == 1 && lane == 0) step_src[out] = s0 + j;
+                        out += 1;
+                        steps += 1;
+                        np += __popcll(in);
                     }
                 }
+            }
+        }
+        if (PASS == 0 && lane == 0) cnt[base + e - eb] = steps;
+    }
+    if (PASS == 0 && lane == 0) atomicAdd(pairs, np);
+}
+
+// Residual evaluation over the step lists (steps of the k-th entry of item i:
+// step_src[ie_ptr[ie_base[i] + k] .. ie_ptr[ie_base[i] + k + 1])).
+template <class R, int NC, bool SPLIT>
+__global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
+                                                     const int4 *__restrict__ items, const int *__restrict__ l1_ptr,
+                                                     const NfEntry *__restrict__ l1, const P4<R> *__restrict__ geom,
+                                                     const int *__restrict__ ie_base, const long long *__restrict__ ie_ptr,
+                                                     const int *__restrict__ step_src, const R *__restrict__ coeffs,
+                                                     R *__restrict__ pot_sorted, int n_items) {
+    // wave index made wave-uniform explicitly, so everything derived from it lives in SGPRs
+    const int item = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= n_items) return;  // whole wavefront
+    const int lane = threadIdx.x & 63;
+    const int4 it = items[item];  // (first target, count, target-leaf list index, -)
+    const bool active = lane < it.y;
+    const P4<R> me = active ? trg[it.x + lane] : P4<R>{R(1e18), R(1e18), R(1e18), R(0)};
+    R c[NC];
+    if (!SPLIT) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    }
+    const int eb = l1_ptr[it.z];
+    const long long *iep = ie_ptr + ie_base[item] - eb;
+    double acc = 0;
+    for (int e = eb; e < l1_ptr[it.z + 1]; ++e) {
+        const long long sb = iep[e], se = iep[e + 1];
+        if (sb == se) continue;
+        const P4<R> gm = geom[e];
+        const R bs = gm.w, d2max = bs * bs, rsc = R(2) / bs, cen = -bs / R(2);
+        const R tx = me.x + gm.x, ty = me.y + gm.y, tz = me.z + gm.z;
+        if (SPLIT) {
+            const int cf = l1[e].cfidx;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) c[i] = coeffs[cf * NF_SPLIT_STRIDE + i];
+        }
+        constexpr int U = nf_unroll<R>();
+        const int *sp = step_src + sb;
+        const int cnt = (int)(se - sb);
+        R ea = 0;
+        for (int g = 0; g < cnt; g += U) {
+            // all U indices and then all U points are requested before the first use
+            int id[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) id[j] = sp[g + j];
+            P4<R> qs[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) qs[j] = src[id[j]];
+            // distances of all U points first and every point used unconditionally, so that
+            // all point loads are in flight before the first wait; steps past the run get
+            // r^2 = bs^2, outside the support
+            R r2s[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const R dx = tx - qs[j].x, dy = ty - qs[j].y, dz = tz - qs[j].z;
+                const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                r2s[j] = g + j < cnt ? r2 : d2max;
+            }
+            // (empty statements that keep the U distances ahead of the first step: the compiler
+            // would otherwise sink them, and with them the point loads, into later steps)
+#pragma unroll
+            for (int j = 0; j < U; ++j) asm volatile("" ::"v"(r2s[j]));
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const R r2 = r2s[j];
+                R u = 0;
+                if (r2 < d2max && r2 > R(1e-30)) {
+                    const R rinv = rsqrt_full<R>(r2);
+                    if (SPLIT) {
+                        constexpr int SE = (NC - 1) & ~1, SO = ((NC - 2) & ~1) + 1;
+                        R Ev = c[SE];
+#pragma unroll
+                        for (int i = SE - 2; i >= 0; i -= 2) Ev = fma(Ev, r2, c[i]);
+                        R Od = c[SO];
+#pragma unroll
+                        for (int i = SO - 2; i >= 0; i -= 2) Od = fma(Od, r2, c[i]);
+                        u = fma(Ev, rinv, Od);
+                    } else {
+                        const R x = fma(r2, rinv, cen) * rsc;
+                        R p = c[NC - 1];
+#pragma unroll
+                        for (int i = NC - 2; i >= 0; --i) p = fma(p, x, c[i]);
+                        u = p * rinv;
+                    }
+                }
+                ea = fma(u, qs[j].w, ea);
             }
         }
         acc += (double)ea;

@@ -8,11 +8,14 @@
 #include "tree_gpu.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
-#include <cstdlib>
-#include <type_traits>
-#include <string>
 #include <cmath>
+#include <cstdlib>
+#include <functional>
+#include <string>
+#include <type_traits>
+#include <unordered_map>
 #include <cstdio>
 #include <stdexcept>
 #include <unordered_map>
@@ -42,30 +45,6 @@ int nc_for_digits(int d) {
     return nc[std::min(12, std::max(0, d))];
 }
 
-
-// Exact count of near-field pairs inside the support (1e-30 < r^2 < bs^2), once per tree.
-template <class R>
-__global__ void k_near_count(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg, const int *__restrict__ leaf_trg,
-                             const int *__restrict__ l1_ptr, const NfEntry *__restrict__ l1, const P4<R> *__restrict__ geom,
-                             unsigned long long *__restrict__ counter) {
-    const int lf = blockIdx.x;
-    const int tbeg = leaf_trg[2 * lf], tcnt = leaf_trg[2 * lf + 1];
-    unsigned long long c = 0;
-    for (int t = threadIdx.x; t < tcnt; t += blockDim.x) {
-        const P4<R> me = trg[tbeg + t];
-        for (int e = l1_ptr[lf]; e < l1_ptr[lf + 1]; ++e) {
-            const P4<R> gm = geom[e];
-            const R bs = gm.w, d2 = bs * bs;
-            for (int s = 0; s < l1[e].scnt; ++s) {
-                const P4<R> q = src[l1[e].sbeg + s];
-                const R dx = me.x + gm.x - q.x, dy = me.y + gm.y - q.y, dz = me.z + gm.z - q.z;
-                const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                c += (r2 > R(1e-30) && r2 < d2) ? 1 : 0;
-            }
-        }
-    }
-    atomicAdd(counter, c);
-}
 
 }  // namespace
 
@@ -179,12 +158,89 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     pot_dev.alloc(sizeof(Real) * (size_t)n);
     far.alloc(sizeof(double) * (size_t)n);
 
+    // ---------------- periodic folding
+    // When the box grid is aligned with the lattice, a level-0 box of the image layer holds
+    // the images, under one lattice translation, of the points of an inner box.  If the two
+    // subtrees are the same (refinement, plane-wave flags, the same points leaf by leaf at the
+    // same positions relative to the box centres), every box of the image subtree has its
+    // counterpart's proxy charges and outgoing plane waves, which are relative to the box
+    // centre: those boxes are not computed, and the downward pass reads the counterpart's.
+    // fold[b] = the counterpart of b, or -1.  PDMK_FOLD=0 turns it off.
+    std::vector<int> fold(nboxes, -1);
+    {
+        const char *fe = std::getenv("PDMK_FOLD");
+        const bool enabled = !(fe && std::string(fe) == "0");
+        auto user_of = [&](int j) {  // tree-order source -> user index
+            const int o = t.src_order[j];
+            return o < N ? o : g.img_src[o - N];
+        };
+        const double bs0 = t.boxsize[0], tol = 1e-12 * bs0;
+        std::vector<int> ub, up;
+        std::function<bool(int, int)> same = [&](int b, int q) {
+            if (t.level[b] != t.level[q] || t.leaf[b] != t.leaf[q] || t.pwexp[b] != t.pwexp[q] ||
+                t.src_count[b] != t.src_count[q] || t.trg_count[b] != 0)
+                return false;
+            if (!t.leaf[b]) {
+                for (int k = 0; k < 8; ++k)
+                    if (!same(t.first_child[b] + k, t.first_child[q] + k)) return false;
+                return true;
+            }
+            ub.clear();
+            up.clear();
+            for (int i = 0; i < t.src_count[b]; ++i) ub.push_back(user_of(t.src_begin[b] + i));
+            for (int i = 0; i < t.src_count[q]; ++i) up.push_back(user_of(t.src_begin[q] + i));
+            std::sort(ub.begin(), ub.end());
+            std::sort(up.begin(), up.end());
+            return ub == up;
+        };
+        std::function<void(int, int)> mark = [&](int b, int q) {
+            fold[b] = q;
+            n_folded++;
+            if (!t.leaf[b])
+                for (int k = 0; k < 8; ++k) mark(t.first_child[b] + k, t.first_child[q] + k);
+        };
+        std::unordered_map<int, std::array<double, 3>> local;  // user index -> position in its level-0 box
+        for (int b : (enabled ? t.by_level[0] : std::vector<int>())) {
+            const int cnt = t.src_count[b], beg = t.src_begin[b];
+            if (cnt == 0 || t.src_order[beg] < N) continue;
+            // the translation of the first point; every point of b must be an image under it
+            const int u0 = user_of(beg);
+            double R[3];
+            for (int d = 0; d < 3; ++d) R[d] = t.src_sorted[3 * (size_t)beg + d] - g.r_src[3 * (size_t)u0 + d];
+            int qc[3];
+            for (int d = 0; d < 3; ++d) qc[d] = (int)std::lround((t.center[3 * b + d] - R[d] - t.xelo[d]) / bs0 - 0.5);
+            const int q = t.find(0, qc[0], qc[1], qc[2]);
+            if (q < 0 || t.src_count[q] != cnt) continue;
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) ok = ok && std::fabs(t.center[3 * q + d] - (t.center[3 * b + d] - R[d])) <= tol;
+            local.clear();
+            for (int i = 0; ok && i < cnt; ++i) {
+                const int j = t.src_begin[q] + i;
+                ok = t.src_order[j] < N;
+                std::array<double, 3> x;
+                for (int d = 0; d < 3; ++d) x[d] = t.src_sorted[3 * (size_t)j + d] - t.center[3 * q + d];
+                ok = ok && local.emplace(t.src_order[j], x).second;
+            }
+            for (int i = 0; ok && i < cnt; ++i) {
+                const int j = beg + i;
+                const auto it = local.find(user_of(j));
+                ok = t.src_order[j] >= N && it != local.end();
+                for (int d = 0; ok && d < 3; ++d)
+                    ok = std::fabs(t.src_sorted[3 * (size_t)j + d] - t.center[3 * b + d] - it->second[d]) <= tol;
+            }
+            if (ok && same(b, q)) mark(b, q);
+        }
+    }
+
     // ---------------- proxy slots
-    std::vector<int> up_of(nboxes, -1), down_of(nboxes, -1);
+    // up_of: the slot a box's proxy and outgoing plane waves are computed in (-1 for folded
+    // boxes); up_src: the slot they are read from (the counterpart's for folded boxes)
+    std::vector<int> up_of(nboxes, -1), up_src(nboxes, -1), down_of(nboxes, -1);
     for (int b = 0; b < nboxes; ++b) {
-        if (t.pwexp[b] && t.src_count[b] > 0) up_of[b] = n_up++;
+        if (t.pwexp[b] && t.src_count[b] > 0 && fold[b] < 0) up_of[b] = n_up++;
         if ((t.pwexp[b] || t.tpeval[b]) && t.src_count[b] + t.trg_count[b] > 0) down_of[b] = n_down++;
     }
+    for (int b = 0; b < nboxes; ++b) up_src[b] = fold[b] < 0 ? up_of[b] : up_of[fold[b]];
     const size_t P3 = (size_t)p * p * p;
     const int H = (n_pw + 1) / 2, NM = n_pw * n_pw * H;
     proxy_up.alloc(sizeof(Real) * P3 * std::max(1, n_up));
@@ -251,7 +307,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     for (int l = nlevels - 1; l >= 0; --l) {
         std::vector<int> out, add, ptr{0}, in, code, pslot, cptr{0};
         for (int b : t.by_level[l]) {
-            if (!t.proxy_from_children[b]) continue;
+            if (!t.proxy_from_children[b] || fold[b] >= 0) continue;
             pslot.push_back(up_of[b]);
             for (int k = 0; k < 8; ++k) {
                 const int ch = t.first_child[b] + k;
@@ -304,15 +360,15 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             if (!(t.pwexp[b] && t.src_count[b] + t.trg_count[b] > 0 && has_out)) continue;
             down.push_back(down_of[b]);
             zero.push_back(set_by_parent[b] ? 0 : 1);
-            nslot.push_back(up_of[b]);
+            nslot.push_back(up_src[b]);
             nsten.push_back(13);
             for (int s = 0; s < 27; ++s) {
                 if (s == 13) continue;
                 const int nb = t.colleague(b, s);
                 if (nb < 0) continue;
                 if (t.leaf[b] && t.leaf[nb]) continue;
-                if (up_of[nb] < 0) continue;
-                nslot.push_back(up_of[nb]);
+                if (up_src[nb] < 0) continue;
+                nslot.push_back(up_src[nb]);
                 nsten.push_back(26 - s);  // stencil of (centre_box - centre_nbr)
             }
             nptr.push_back((int)nslot.size());
@@ -450,6 +506,47 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         else coeffs.upload(std::vector<Real>(T.residual.begin(), T.residual.end()));
         work.nf_pairs_candidate = cand;
         stats.n_list1 = (long long)l1v.size();
+        // step lists: counts per (item, entry), prefix on the host, then the source indices;
+        // the first pass also counts the pairs inside the support exactly
+        std::vector<int> ieb(n_nf_items);
+        int n_ie = 0;
+        for (int i = 0; i < n_nf_items; ++i) {
+            ieb[i] = n_ie;
+            n_ie += ptr[itv[i].z + 1] - ptr[itv[i].z];
+        }
+        nf_ie_base.upload(ieb);
+        if (n_nf_items > 0) {
+            DevBuf cnt, pairs;
+            cnt.alloc(sizeof(int) * (size_t)std::max(1, n_ie));
+            pairs.alloc(sizeof(unsigned long long));
+            check(hipMemsetAsync(pairs.p, 0, sizeof(unsigned long long), stream), "hipMemsetAsync");
+            const dim3 grid((n_nf_items + 3) / 4);
+            hipLaunchKernelGGL((k_near_steps<Real, 0>), grid, dim3(256), 0, stream, src.as<P4<Real>>(), trg.as<P4<Real>>(),
+                               nf_items.as<int4>(), nf_wbox.as<P4<Real>>(), nf_ptr.as<int>(), nf_l1.as<NfEntry>(),
+                               nf_geom.as<P4<Real>>(), nf_cbox.as<P4<Real>>(), nf_ie_base.as<int>(), cnt.as<int>(),
+                               (const long long *)nullptr, (int *)nullptr, pairs.as<unsigned long long>(), n_nf_items);
+            check(hipGetLastError(), "k_near_steps count");
+            std::vector<int> c(std::max(1, n_ie));
+            unsigned long long np = 0;
+            check(hipMemcpyAsync(c.data(), cnt.p, sizeof(int) * c.size(), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+            check(hipMemcpyAsync(&np, pairs.p, sizeof(np), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            std::vector<long long> iep((size_t)n_ie + 1, 0);
+            for (int k = 0; k < n_ie; ++k) iep[k + 1] = iep[k] + c[k];
+            n_nf_steps = iep.back();
+            nf_ie_ptr.upload(iep);
+            // zero-filled padding: a group of steps may read up to NF_STEP_PAD indices past the
+            // end of the last run (point 0, never used)
+            nf_steps.alloc(sizeof(int) * (size_t)(n_nf_steps + NF_STEP_PAD));
+            check(hipMemsetAsync(nf_steps.p, 0, sizeof(int) * (size_t)(n_nf_steps + NF_STEP_PAD), stream), "hipMemsetAsync");
+            hipLaunchKernelGGL((k_near_steps<Real, 1>), grid, dim3(256), 0, stream, src.as<P4<Real>>(), trg.as<P4<Real>>(),
+                               nf_items.as<int4>(), nf_wbox.as<P4<Real>>(), nf_ptr.as<int>(), nf_l1.as<NfEntry>(),
+                               nf_geom.as<P4<Real>>(), nf_cbox.as<P4<Real>>(), nf_ie_base.as<int>(), (int *)nullptr,
+                               nf_ie_ptr.as<long long>(), nf_steps.as<int>(), (unsigned long long *)nullptr, n_nf_items);
+            check(hipGetLastError(), "k_near_steps write");
+            work.nf_pairs_support = (double)np;
+        }
+        stats.n_nf_steps = n_nf_steps;
         // near-field algorithmic HBM bytes: every target read and written once, every
         // list entry's sources read once per target leaf
         work.nf_bytes = src_pts * sizeof(P4<Real>) + (double)n * (sizeof(P4<Real>) + 2 * sizeof(Real));
@@ -506,19 +603,6 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     stats.n_pw = n_pw; stats.ms = ms; stats.mt = mt; stats.mu = mu; stats.n_direct_leaves = n_nf_leaves;
     stats.n_far_modes = n_ff_modes; stats.n_digits = n_digits; stats.is_float = sizeof(Real) == 4;
     for (int b = 0; b < nboxes; ++b) stats.n_leaves += t.leaf[b];
-    if (n_nf_leaves > 0) {
-        DevBuf cnt;
-        cnt.alloc(sizeof(unsigned long long));
-        check(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), stream), "hipMemsetAsync");
-        hipLaunchKernelGGL(k_near_count<Real>, dim3(n_nf_leaves), dim3(256), 0, stream, src.as<P4<Real>>(),
-                           trg.as<P4<Real>>(), nf_trg.as<int>(), nf_ptr.as<int>(), nf_l1.as<NfEntry>(), nf_geom.as<P4<Real>>(),
-                           cnt.as<unsigned long long>());
-        check(hipGetLastError(), "k_near_count launch");
-        unsigned long long c = 0;
-        check(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        work.nf_pairs_support = (double)c;
-    }
     work.nf_flops = work.nf_pairs_support * (2.0 * nc + 12.0);
     work.on_bytes = (double)nsall * (sizeof(int) + sizeof(Real) + sizeof(double)) +
                     (double)n * (2 * sizeof(Real) + 2 * sizeof(double) + sizeof(int) + sizeof(double) + sizeof(Real));
@@ -644,8 +728,8 @@ void launch_near(Plan<Real> &pl, hipStream_t s) {
     if (pl.n_nf_items == 0) return;
     hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3((pl.n_nf_items + 3) / 4), dim3(256), 0, s,
                        pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_items.template as<int4>(),
-                       pl.nf_wbox.template as<P4<Real>>(), pl.nf_ptr.template as<int>(), pl.nf_l1.template as<NfEntry>(),
-                       pl.nf_geom.template as<P4<Real>>(), pl.nf_cbox.template as<P4<Real>>(),
+                       pl.nf_ptr.template as<int>(), pl.nf_l1.template as<NfEntry>(), pl.nf_geom.template as<P4<Real>>(),
+                       pl.nf_ie_base.template as<int>(), pl.nf_ie_ptr.template as<long long>(), pl.nf_steps.template as<int>(),
                        pl.coeffs.template as<Real>(), pl.pot_near.template as<Real>(), pl.n_nf_items);
     check(hipGetLastError(), "k_near_field");
 }

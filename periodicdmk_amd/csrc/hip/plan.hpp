@@ -41,7 +41,7 @@ struct DevBuf {
 struct Stats {
     long long n_src = 0, nsall = 0, nboxes = 0, n_levels = 0, p = 0, n_pw = 0, ms = 0, mt = 0, mu = 0;
     long long n_leaves = 0, n_list1 = 0, n_direct_leaves = 0, n_far_modes = 0, n_digits = 0, is_float = 0,
-              n_pair_tiles = 0;
+              n_nf_steps = 0;
 };
 
 struct Work {
@@ -112,9 +112,11 @@ struct Plan {
     int n_leaf_eval = 0;
     DevBuf leaf_down, leaf_geom, leaf_trg;
     // near field: per target leaf (nf_trg, nf_ptr -> list1 entries nf_l1/nf_geom), per
-    // wavefront item (nf_items, nf_wbox), per source chunk (nf_cbox)
+    // wavefront item (nf_items, nf_wbox), per source chunk (nf_cbox); step lists per
+    // (item, entry): nf_ie_base[item] + k -> nf_ie_ptr -> source indices nf_steps
     int n_nf_leaves = 0, n_nf_items = 0;
-    DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs;
+    long long n_nf_steps = 0;
+    DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs, nf_ie_base, nf_ie_ptr, nf_steps;
     // far field: exact Fourier sum, or the NUFFT for large N x modes
     int far_nufft = 0;
     FourierSum ff;
@@ -151,6 +153,7 @@ struct Plan {
     // whether the tree's per-point work ran on the GPU
     double build_ms[3] = {0, 0, 0};
     bool tree_on_gpu = false;
+    int n_folded = 0;  // boxes whose proxies and outgoing plane waves are a periodic partner's
 
     // topology export for parity tests
     std::vector<int> box7;
