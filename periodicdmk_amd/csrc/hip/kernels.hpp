@@ -490,8 +490,11 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
                                                    const int *__restrict__ nbr_slot, const int *__restrict__ nbr_sten,
                                                    int nm, int item0, Cx<R> *__restrict__ pin) {
     chain_priority();
-    const int it = item0 + blockIdx.y;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    // x = target box (consecutive workgroups share neighbours; each XCD takes a contiguous,
+    // spatially coherent range of boxes), y = mode chunk: the neighbours' chunk is read from
+    // L2 by the 27 boxes around it instead of from memory
+    const int bi = xcd_block(blockIdx.x, gridDim.x), it = item0 + bi;
+    const int m = blockIdx.y * blockDim.x + threadIdx.x;
     if (m >= nm) return;
     R sr = 0, si = 0;
     for (int e = nbr_ptr[it]; e < nbr_ptr[it + 1]; ++e) {
@@ -500,7 +503,7 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
         sr = fma(a.re, s.re, fma(-a.im, s.im, sr));
         si = fma(a.re, s.im, fma(a.im, s.re, si));
     }
-    pin[(size_t)blockIdx.y * nm + m] = {sr * lscale, si * lscale};
+    pin[(size_t)bi * nm + m] = {sr * lscale, si * lscale};
 }
 
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
@@ -890,15 +893,16 @@ __global__ __launch_bounds__(256) void k_near_steps(const P4<R> *__restrict__ sr
 // Residual evaluation over the step lists (steps of the k-th entry of item i:
 // step_src[ie_ptr[ie_base[i] + k] .. ie_ptr[ie_base[i] + k + 1])).
 template <class R, int NC, bool SPLIT>
-__global__ __launch_bounds__(256) void k_near_field(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
+__global__ __launch_bounds__(1024) void k_near_field(const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg,
                                                      const int4 *__restrict__ items, const int *__restrict__ l1_ptr,
                                                      const NfEntry *__restrict__ l1, const P4<R> *__restrict__ geom,
                                                      const int *__restrict__ ie_base, const long long *__restrict__ ie_ptr,
                                                      const int *__restrict__ step_src, const R *__restrict__ coeffs,
-                                                     R *__restrict__ pot_sorted, int n_items) {
+                                                     R *__restrict__ pot_sorted, int item0, int item1) {
     // wave index made wave-uniform explicitly, so everything derived from it lives in SGPRs
-    const int item = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (item >= n_items) return;  // whole wavefront
+    const int item = item0 + xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= item1) return;  // whole wavefront
     const int lane = threadIdx.x & 63;
     const int4 it = items[item];  // (first target, count, target-leaf list index, -)
     const bool active = lane < it.y;

@@ -692,7 +692,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         const Real lscale = (Real)std::ldexp(1.0, E.level);
         for (int i0 = 0; i0 < E.n_items; i0 += pl.pin_batch) {
             const int nb = std::min(pl.pin_batch, E.n_items - i0);
-            hipLaunchKernelGGL(k_pw_shift<Real>, dim3((NM + 255) / 256, nb), dim3(256), 0, s, pl.pw_out.template as<C>(),
+            hipLaunchKernelGGL(k_pw_shift<Real>, dim3(nb, (NM + 255) / 256), dim3(256), 0, s, pl.pw_out.template as<C>(),
                                pl.shift0.template as<C>(), lscale, E.nbr_ptr.template as<int>(),
                                E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), NM, i0,
                                pl.pin.template as<C>());
@@ -723,14 +723,17 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     }
 }
 
+// wpb = wavefronts (items) per workgroup: 4 when the near field runs beside the whole
+// transform chain; 16 when it starts late beside the long FP64 downward pass, where whole
+// 16-wave workgroups retiring at once leave room for the chain's large workgroups.
 template <class Real, int NC, bool SPLIT>
-void launch_near(Plan<Real> &pl, hipStream_t s) {
-    if (pl.n_nf_items == 0) return;
-    hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3((pl.n_nf_items + 3) / 4), dim3(256), 0, s,
+void launch_near(Plan<Real> &pl, hipStream_t s, int wpb, int i0, int i1) {
+    if (i1 <= i0) return;
+    hipLaunchKernelGGL((k_near_field<Real, NC, SPLIT>), dim3((i1 - i0 + wpb - 1) / wpb), dim3(64 * wpb), 0, s,
                        pl.src.template as<P4<Real>>(), pl.trg.template as<P4<Real>>(), pl.nf_items.template as<int4>(),
                        pl.nf_ptr.template as<int>(), pl.nf_l1.template as<NfEntry>(), pl.nf_geom.template as<P4<Real>>(),
                        pl.nf_ie_base.template as<int>(), pl.nf_ie_ptr.template as<long long>(), pl.nf_steps.template as<int>(),
-                       pl.coeffs.template as<Real>(), pl.pot_near.template as<Real>(), pl.n_nf_items);
+                       pl.coeffs.template as<Real>(), pl.pot_near.template as<Real>(), i0, i1);
     check(hipGetLastError(), "k_near_field");
 }
 
@@ -775,21 +778,23 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
         check(hipEventRecord(e_gather, s), "hipEventRecord");
         check(hipStreamWaitEvent(sn, e_gather, 0), "hipStreamWaitEvent");
     }
-    auto near_field = [&] {
+    const bool near_late = !serial && p >= 28;
+    const int wpb = near_late ? 16 : 4;
+    auto near_field = [&](int i0, int i1) {
         mark(3, 0, sn);
         if (stages & 2) {
             switch (n_digits) {
-            case 1: case 2: launch_near<Real, 5, true>(*this, sn); break;
-            case 3: launch_near<Real, 6, true>(*this, sn); break;
-            case 4: launch_near<Real, 9, true>(*this, sn); break;
-            case 5: launch_near<Real, 11, true>(*this, sn); break;
-            case 6: launch_near<Real, 12, false>(*this, sn); break;
-            case 7: launch_near<Real, 15, false>(*this, sn); break;
-            case 8: launch_near<Real, 17, false>(*this, sn); break;
-            case 9: launch_near<Real, 18, false>(*this, sn); break;
-            case 10: launch_near<Real, 22, false>(*this, sn); break;
-            case 11: launch_near<Real, 23, false>(*this, sn); break;
-            case 12: launch_near<Real, 25, false>(*this, sn); break;
+            case 1: case 2: launch_near<Real, 5, true>(*this, sn, wpb, i0, i1); break;
+            case 3: launch_near<Real, 6, true>(*this, sn, wpb, i0, i1); break;
+            case 4: launch_near<Real, 9, true>(*this, sn, wpb, i0, i1); break;
+            case 5: launch_near<Real, 11, true>(*this, sn, wpb, i0, i1); break;
+            case 6: launch_near<Real, 12, false>(*this, sn, wpb, i0, i1); break;
+            case 7: launch_near<Real, 15, false>(*this, sn, wpb, i0, i1); break;
+            case 8: launch_near<Real, 17, false>(*this, sn, wpb, i0, i1); break;
+            case 9: launch_near<Real, 18, false>(*this, sn, wpb, i0, i1); break;
+            case 10: launch_near<Real, 22, false>(*this, sn, wpb, i0, i1); break;
+            case 11: launch_near<Real, 23, false>(*this, sn, wpb, i0, i1); break;
+            case 12: launch_near<Real, 25, false>(*this, sn, wpb, i0, i1); break;
             default: throw std::logic_error("unsupported digit count");
             }
         }
@@ -798,11 +803,10 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     // Start of the near field: at once when its cost is comparable to the transform chain's
     // (p < 28); at p >= 28 the chain is several times longer and its first kernels (proxy
     // formation on the matrix cores, proxy -> plane wave) fill the GPU, so the near field
-    // starts after them and overlaps the latency-bound rest (measured at N=1e6: 9 digits
-    // 17.9 -> 17.2 ms; 6 digits unchanged or slower with a later start).
-    const bool near_late = !serial && p >= 28;
+    // starts after them and overlaps the rest.  (Measured at N=1e6: a later start, or the near
+    // field cut into two launches, is slower at 6 digits; the late start is faster at 9.)
     e_upward = near_late ? e_up : nullptr;
-    if (!serial && !near_late) near_field();
+    if (!serial && !near_late) near_field(0, n_nf_items);
     mark(2, 0, s);
     if (stages & 1) {
         switch (p) {
@@ -815,10 +819,10 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     }
     mark(2, 1, s);
     if (times_ms) check(hipEventRecord(ev[2], s), "hipEventRecord");
-    if (serial) near_field();
+    if (serial) near_field(0, n_nf_items);
     if (near_late) {
         check(hipStreamWaitEvent(sn, e_up, 0), "hipStreamWaitEvent");
-        near_field();
+        near_field(0, n_nf_items);
     }
     if (!serial) {
         check(hipEventRecord(e_near, sn), "hipEventRecord");
