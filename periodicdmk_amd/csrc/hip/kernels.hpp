@@ -287,6 +287,19 @@ constexpr int tf_slices(int total, int per_slice, int reserved = 0) {
     return k < 1 ? 1 : (k > total ? total : k);
 }
 
+// Slices of the proxy <-> plane-wave transforms that fit the LDS budget; each slice is a
+// workgroup of its own (grid y), so a level with few boxes still fills the GPU.
+template <class R, int P, int NPW> struct P2pwCfg {
+    static constexpr int H = (NPW + 1) / 2;
+    static constexpr int K3 = tf_slices(H, (P * P + NPW * OddStride<P>::v) * (int)sizeof(Cx<R>));
+    static constexpr int slices = (H + K3 - 1) / K3;
+};
+template <class R, int P, int NPW> struct Pw2pCfg {
+    static constexpr int H = (NPW + 1) / 2;
+    static constexpr int AC = tf_slices(P, (H * NPW + H * OddStride<P>::v) * (int)sizeof(Cx<R>));
+    static constexpr int slices = (P + AC - 1) / AC;
+};
+
 // 3-mode tensor transform out(a',b',c') (+)= sum_children U0(a',a) U1(b',b) U2(c',c) in(a,b,c)
 // (tensor layout [c][b][a], a fastest).  U_d is the minus/plus parent<->child matrix selected
 // by bit d of the child code, transposed for child -> parent.  Item i writes out slot
@@ -410,20 +423,21 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // ------------------------------------------------------------------ plane waves
 // Plane-wave mode layout: m = k1 * (H n) + k3 * n + k2 ([x][z][y], y fastest), k3 < H.
 // Outgoing expansion pw(k1,k2,k3) = sum W(k1,a) W(k2,b) W(k3,c) C[c][b][a], W = poly2pw
-// (W(k, j) at W[k + n j]); one workgroup per box, sliced over k3.
+// (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3).
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
                                                    const Cx<R> *__restrict__ W, int n_boxes) {
     chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, TS = OddStride<P>::v;
-    constexpr int K3 = tf_slices(H, (PP + NPW * TS) * (int)sizeof(Cx<R>));
+    constexpr int K3 = P2pwCfg<R, P, NPW>::K3;
     __shared__ Cx<R> T1[K3 * PP];        // [k][b][a]
     __shared__ Cx<R> T2[K3 * NPW * TS];  // [k][k2][a] rows of TS
     const int bx = blockIdx.x;
     if (bx >= n_boxes) return;
     const R *C = proxy + (size_t)bx * P * PP;
     Cx<R> *pw = pw_out + (size_t)bx * NPW * COL;
-    for (int k30 = 0; k30 < H; k30 += K3) {
+    {
+        const int k30 = blockIdx.y * K3;  // this workgroup's slice of k3
         const int nk = min(K3, H - k30);
         // z: T1[k][b][a] = sum_c W(k30 + k, c) C[c][b][a]
         for (int j = threadIdx.x; j < PP; j += blockDim.x) {
@@ -477,7 +491,6 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
                 dst[(size_t)k1 * COL] = {sr, si};
             }
         }
-        __syncthreads();
     }
 }
 
@@ -507,15 +520,15 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
 }
 
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
-// (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  One
-// workgroup per box, sliced over a.
+// (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  Grid x =
+// box, grid y = slices of a (strided when grid y has fewer workgroups than slices).
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
                                                    int item0, int n_items, R *__restrict__ down_base) {
     chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, half = NPW / 2, TS2 = OddStride<P>::v;
-    constexpr int AC = tf_slices(P, (COL + H * TS2) * (int)sizeof(Cx<R>));
+    constexpr int AC = Pw2pCfg<R, P, NPW>::AC;
     __shared__ Cx<R> U1[AC * COL];       // [a][k3][k2]
     __shared__ Cx<R> U2[AC * H * TS2];   // [a][k3][b] rows of TS2
     const int it = blockIdx.x;
@@ -523,7 +536,8 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
     const Cx<R> *X = pin + (size_t)it * NPW * COL;
     R *down = down_base + (size_t)box_down[item0 + it] * P * PP;
     const bool zero = box_zero[item0 + it] != 0;
-    for (int a0 = 0; a0 < P; a0 += AC) {
+    // slices of a, strided over the workgroups of grid y
+    for (int a0 = blockIdx.y * AC; a0 < P; a0 += gridDim.y * AC) {
         const int na = min(AC, P - a0);
         // x: U1[a][k3][k2] = sum_k1 V(k1, a0 + a) pin[k1][k3][k2], column = (k3, k2)
         for (int col = threadIdx.x; col < COL; col += blockDim.x) {

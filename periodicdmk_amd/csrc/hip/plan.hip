@@ -682,7 +682,8 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         check(hipGetLastError(), "k_sum_children");
     }
     if (pl.n_up > 0) {
-        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(pl.n_up), dim3(256), 0, s, pl.proxy_up.template as<Real>(),
+        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(pl.n_up, P2pwCfg<Real, P, NPW>::slices), dim3(256), 0, s,
+                           pl.proxy_up.template as<Real>(),
                            pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(), pl.n_up);
         check(hipGetLastError(), "k_proxy2pw");
     }
@@ -697,7 +698,8 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
                                E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), NM, i0,
                                pl.pin.template as<C>());
             check(hipGetLastError(), "k_pw_shift");
-            hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb), dim3(256), 0, s, pl.pin.template as<C>(),
+            hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb, Pw2pCfg<Real, P, NPW>::slices), dim3(256), 0, s,
+                               pl.pin.template as<C>(),
                                pl.pw2poly.template as<C>(), E.down.template as<int>(), E.zero.template as<int>(), i0, nb,
                                pl.proxy_down.template as<Real>());
             check(hipGetLastError(), "k_pw2proxy");
