@@ -45,6 +45,26 @@ int pdmk_gpu_evaluate_timed(void* h, const double* d_charges, void* d_pot, doubl
 /// Stage-level parity checks use it; stages=15 equals pdmk_gpu_evaluate_*.
 int pdmk_gpu_evaluate_stages(void* h, const double* d_charges, void* d_pot, int stages);
 
+/// Quasi-periodic trees (Bloch vector `bloch[3]`, in the frame the reference uses: it is
+/// dotted with lattice-frame translations).  Replaces the C++ constructor with `bloch` and
+/// PeriodicDMKTreeT::evaluate_complex (cpp/include/pdmk/periodic_dmk.hpp:17-144,
+/// cpp/src/periodic_dmk.cpp:251-277), which the reference's C ABI does not expose.
+/// create: NULL on error (message on stderr), or when bloch is NULL.  evaluate_complex:
+/// pot_out[2i], pot_out[2i+1] = Re, Im at point i (user order); N and positions are ignored
+/// as in pdmk_evaluate_*; -1 on error or on a handle without bloch.  The plain
+/// pdmk_evaluate_* on a quasi-periodic handle return -1.
+void* pdmk_create_qp_double(const double* cell_3x3, int N, const double* positions_3xN, int n_digits,
+                            int n_per_leaf, int nz, const double* bloch);
+void* pdmk_create_qp_float(const double* cell_3x3, int N, const double* positions_3xN, int n_digits,
+                           int n_per_leaf, int nz, const double* bloch);
+int pdmk_evaluate_complex_double(void* h, int N, const double* positions_3xN, const double* charges,
+                                 double* pot_out);
+int pdmk_evaluate_complex_float(void* h, int N, const double* positions_3xN, const double* charges,
+                                float* pot_out);
+/// Device pointers: d_charges FP64 (N), d_pot Real (2N); synchronous.  stages as in
+/// pdmk_gpu_evaluate_stages (15 = everything).
+int pdmk_gpu_evaluate_complex(void* h, const double* d_charges, void* d_pot, int stages);
+
 /// Tree and work statistics: out[0..15] = {n_src, nsall, nboxes, n_levels, p, n_pw,
 /// ms, mt, mu, n_leaves, n_list1, n_direct_leaves, n_far_modes, n_digits,
 /// is_float, n_nf_steps}.

@@ -1,7 +1,7 @@
 /// C++ API of the MI355X PeriodicDMK library, mirroring
 /// cpp/include/pdmk/periodic_dmk.hpp:17-144 of the reference: build once on the
-/// host, evaluate many times on the GPU.  Only DIM = 3 is provided; the
-/// quasi-periodic evaluate_complex() throws std::logic_error.
+/// host, evaluate many times on the GPU.  Only DIM = 3 is provided.  With a Bloch
+/// vector the tree is quasi-periodic and only evaluate_complex() applies.
 #pragma once
 
 #include <pdmk/eval_timing.hpp>
@@ -47,7 +47,10 @@ public:
     /// Same as evaluate(), filling the three-way time breakdown.
     std::vector<Real> evaluate_timed(const std::vector<double>& charges, EvalTiming& timing);
 
-    /// Quasi-periodic evaluation is not provided by this build: throws std::logic_error.
+    /// Quasi-periodic potentials (bloch given at construction): sum over images R of
+    /// q e^{i bloch.R} / r, (Re, Im) per point.  Throws std::logic_error without bloch,
+    /// std::invalid_argument for a size mismatch (cpp/src/periodic_dmk.cpp:251-277).
+    /// evaluate() and evaluate_timed() throw std::logic_error when bloch is set.
     std::vector<std::complex<Real>> evaluate_complex(const std::vector<double>& charges);
 
     int n_points() const { return n_src_; }

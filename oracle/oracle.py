@@ -41,6 +41,7 @@ def lib():
         L.oracle_create.argtypes = [C.c_int, dp, C.c_int, dp, C.c_int, C.c_int, C.c_int]
         L.oracle_destroy.argtypes = [C.c_void_p]
         L.oracle_evaluate.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.oracle_evaluate_complex.argtypes = [C.c_void_p, dp, dp, C.c_int, dp]
         L.oracle_info.argtypes = [C.c_void_p, ip, dp]
         L.oracle_tree_arrays.argtypes = [C.c_void_p, ip, ip, ip, dp]
         L.oracle_pswf.restype = C.c_double
@@ -91,6 +92,18 @@ class OracleTree:
         if lib().oracle_evaluate(self._h, _d(q), int(stages), _d(out)) != 0:
             raise RuntimeError("oracle_evaluate failed")
         return out
+
+    def evaluate_complex(self, charges, bloch, stages=STAGE_ALL):
+        """Quasi-periodic potentials (complex, user order) for Bloch vector `bloch`
+        (PeriodicDMKTreeT::evaluate_complex, cpp/src/periodic_dmk.cpp:251-277)."""
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        if q.shape[0] != self.n:
+            raise ValueError("charges size != n_src")
+        b = np.ascontiguousarray(np.asarray(bloch, dtype=np.float64).reshape(3))
+        out = np.zeros(2 * self.n)
+        if lib().oracle_evaluate_complex(self._h, _d(q), _d(b), int(stages), _d(out)) != 0:
+            raise RuntimeError("oracle_evaluate_complex failed")
+        return out[0::2] + 1j * out[1::2]
 
     def info(self):
         ints = np.zeros(len(self.INFO_KEYS), dtype=np.int32)

@@ -58,6 +58,25 @@ int oracle_evaluate(void *hv, const double *q, int stages, double *pot_out) {
     }
 }
 
+// Quasi-periodic evaluate: pot_out[2i], pot_out[2i+1] = Re, Im at point i (user order).
+int oracle_evaluate_complex(void *hv, const double *q, const double *bloch, int stages, double *pot_out) {
+    auto *h = static_cast<Handle *>(hv);
+    if (!h || !bloch) return -1;
+    try {
+        if (h->is_float) {
+            auto p = T<float>(h)->evaluate_complex(q, bloch, stages);
+            for (size_t i = 0; i < p.size(); ++i) pot_out[i] = p[i];
+        } else {
+            auto p = T<double>(h)->evaluate_complex(q, bloch, stages);
+            for (size_t i = 0; i < p.size(); ++i) pot_out[i] = p[i];
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "oracle_evaluate_complex: %s\n", e.what());
+        return -1;
+    }
+}
+
 // ints: [n_src, nsall, nboxes, n_levels, p, n_pw, ms, mt, mu, n_leaves, n_list1_total, n_direct_work,
 //        n_c2p_work, n_digits]
 // dbls: [rc, beta, psi0, c0, g0d2, w0[0]]

@@ -89,9 +89,12 @@ void compute_grid(const Real *umat, Real rc, int *mgrid, Real *xgrid0) {
     }
 }
 
+// shift (optional) receives the lattice translation (s1, s2, s3) of every image, from which
+// the quasi-periodic phase bloch . shift is formed (cpp/include/dmk/pbc.hpp:221-229).
 template <typename Real>
 void images_with_src_idx(const Real *umat, Real rc, const int *mgrid, const Real *xgrid0, int ns,
-                         const Real *src, std::vector<Real> &img, std::vector<int> &idx) {
+                         const Real *src, std::vector<Real> &img, std::vector<int> &idx,
+                         std::vector<Real> *shift = nullptr) {
     Real xextlo[3], xexthi[3], xlo[3], xhi[3];
     for (int i = 0; i < 3; ++i) {
         xextlo[i] = xgrid0[i] - rc;
@@ -103,6 +106,7 @@ void images_with_src_idx(const Real *umat, Real rc, const int *mgrid, const Real
     }
     img.clear();
     idx.clear();
+    if (shift) shift->clear();
     int n3max = (int)((mgrid[2] + 1) * rc / umat[8]) + 1;
     for (int n3 = -n3max; n3 <= n3max; ++n3) {
         Real s3 = umat[8] * n3;
@@ -124,6 +128,7 @@ void images_with_src_idx(const Real *umat, Real rc, const int *mgrid, const Real
                         continue;
                     img.push_back(x0); img.push_back(x1); img.push_back(x2);
                     idx.push_back(j);
+                    if (shift) { shift->push_back(s1); shift->push_back(s2); shift->push_back(s3); }
                 }
             }
         }
@@ -143,6 +148,7 @@ struct Tree {
     int n_src = 0, nsall = 0;
     std::vector<Real> r_src;       // rotated originals [3*n_src]
     std::vector<int> img_src_idx;  // per image, originating source
+    std::vector<Real> img_shift;   // per image, its lattice translation (3 per image)
     // topology
     int nboxes = 0;
     std::vector<int> depth, parent, nchild;
@@ -195,7 +201,7 @@ struct Tree {
                 r_src[3 * (size_t)i + j] = s;
             }
         std::vector<Real> img;
-        images_with_src_idx<Real>(umat, rc, mgrid, xgrid0, N, r_src.data(), img, img_src_idx);
+        images_with_src_idx<Real>(umat, rc, mgrid, xgrid0, N, r_src.data(), img, img_src_idx, &img_shift);
         nsall = N + (int)img_src_idx.size();
         std::vector<Real> srcall(3 * (size_t)nsall);
         std::copy(r_src.begin(), r_src.end(), srcall.begin());
@@ -1030,6 +1036,124 @@ struct Tree {
                 pot[ipt] += (Real)acc;
             }
         }
+    }
+
+    // ---------------------------------------------------------------- quasi-periodic far field
+    // cpp/src/dmk/pbc.cpp:249-355: wave vectors K_m = 2 pi U^{-T} m + bloch over the full
+    // mode box (dkmax = beta/rc + |bloch|, no half-space symmetry), charges phased by
+    // e^{-i bloch.x}, psi(x) = e^{+i bloch.x} sum_m w(K_m) S(m) e^{i m.theta}; the K = 0 mode
+    // (bloch on the reciprocal lattice) has weight 0.  Evaluated exactly (the reference uses
+    // a tolerance-eps/4 NUFFT).  pot_re / pot_im in user order.
+    void far_field_qp(const Real *q, const double *bloch, Real *pot_re, Real *pot_im) {
+        const double twopi = 2 * M_PI;
+        const double u0 = umat[0], u3 = umat[3], u4 = umat[4], u6 = umat[6], u7 = umat[7], u8 = umat[8];
+        const double vol = u0 * u4 * u8;
+        const double bn = std::sqrt(bloch[0] * bloch[0] + bloch[1] * bloch[1] + bloch[2] * bloch[2]);
+        const double dkmax = (double)beta / (double)rc + bn;
+        const double an1 = u0, an2 = std::sqrt(u3 * u3 + u4 * u4), an3 = std::sqrt(u6 * u6 + u7 * u7 + u8 * u8);
+        const int qs = std::max(2, 2 * (int)std::ceil(dkmax * an1 / twopi));
+        const int qt = std::max(2, 2 * (int)std::ceil(dkmax * an2 / twopi));
+        const int qu = std::max(2, 2 * (int)std::ceil(dkmax * an3 / twopi));
+        const double a11 = 1 / u0, a21 = -u3 / (u0 * u4), a22 = 1 / u4;
+        const double a31 = (u3 * u7 - u6 * u4) / (u0 * u4 * u8), a32 = -u7 / (u4 * u8), a33 = 1 / u8;
+        const double dfac = 4 * M_PI / (tab.psi0 * vol);
+        struct Mode { int i, j, k; double w; };
+        std::vector<Mode> modes;
+        for (int k = -qu / 2; k <= (qu - 1) / 2; ++k)
+            for (int j = -qt / 2; j <= (qt - 1) / 2; ++j)
+                for (int i = -qs / 2; i <= (qs - 1) / 2; ++i) {
+                    const double k1 = twopi * a11 * i + bloch[0], k2 = twopi * (a21 * i + a22 * j) + bloch[1],
+                                 k3 = twopi * (a31 * i + a32 * j + a33 * k) + bloch[2];
+                    const double kk = k1 * k1 + k2 * k2 + k3 * k3;
+                    if (kk < 1e-30) continue;
+                    const double x = std::sqrt(kk) * (double)rc / (double)beta;
+                    if (x <= 1) modes.push_back({i, j, k, dfac * (double)tab.pf.eval(x) / kk});
+                }
+        const int M = (int)modes.size(), N = n_src;
+        std::vector<double> th(3 * (size_t)N), bx(N);
+        for (int j = 0; j < N; ++j) {
+            const double x = r_src[3 * (size_t)j], y = r_src[3 * (size_t)j + 1], z = r_src[3 * (size_t)j + 2];
+            th[3 * (size_t)j + 0] = twopi * (a11 * x + a21 * y + a31 * z);
+            th[3 * (size_t)j + 1] = twopi * (a22 * y + a32 * z);
+            th[3 * (size_t)j + 2] = twopi * (a33 * z);
+            bx[j] = bloch[0] * x + bloch[1] * y + bloch[2] * z;
+        }
+        auto axis = [](double t, int m, std::complex<double> *e) {  // e[k] = e^{-i (k - m/2) t}
+            const std::complex<double> step = std::polar(1.0, -t);
+            std::complex<double> cur = std::polar(1.0, t * (m / 2));
+            for (int k = 0; k < m; ++k) { e[k] = cur; cur *= step; }
+        };
+        std::vector<std::complex<double>> S(M, 0);
+#pragma omp parallel
+        {
+            std::vector<std::complex<double>> loc(M, 0), e1(qs), e2(qt), e3(qu);
+#pragma omp for schedule(static)
+            for (int l = 0; l < N; ++l) {
+                axis(th[3 * (size_t)l], qs, e1.data());
+                axis(th[3 * (size_t)l + 1], qt, e2.data());
+                axis(th[3 * (size_t)l + 2], qu, e3.data());
+                const std::complex<double> ql = (double)q[l] * std::polar(1.0, -bx[l]);
+                for (int m = 0; m < M; ++m)
+                    loc[m] += ql * e1[modes[m].i + qs / 2] * e2[modes[m].j + qt / 2] * e3[modes[m].k + qu / 2];
+            }
+#pragma omp critical
+            for (int m = 0; m < M; ++m) S[m] += loc[m];
+        }
+#pragma omp parallel
+        {
+            std::vector<std::complex<double>> e1(qs), e2(qt), e3(qu);
+#pragma omp for schedule(static)
+            for (int j = 0; j < N; ++j) {
+                axis(th[3 * (size_t)j], qs, e1.data());
+                axis(th[3 * (size_t)j + 1], qt, e2.data());
+                axis(th[3 * (size_t)j + 2], qu, e3.data());
+                std::complex<double> acc = 0;
+                for (int m = 0; m < M; ++m)  // e^{+i m.theta} = conj(e1 e2 e3)
+                    acc += modes[m].w * S[m] *
+                           std::conj(e1[modes[m].i + qs / 2] * e2[modes[m].j + qt / 2] * e3[modes[m].k + qu / 2]);
+                acc *= std::polar(1.0, bx[j]);
+                pot_re[j] += (Real)acc.real();
+                pot_im[j] += (Real)acc.imag();
+            }
+        }
+    }
+
+    // Quasi-periodic evaluate (PeriodicDMKTreeT::evaluate_complex -> pbcdmk_adaptive,
+    // cpp/src/periodic_dmk.cpp:251-277, cpp/src/dmk/pbc.cpp:786-879): densities (Re, Im) with
+    // originals (q, 0) and images q e^{+i bloch.R} (build_combined_sources,
+    // cpp/include/dmk/pbc.hpp:394-447), the same tree and kernels per density, the self
+    // correction on Re only (pbc.cpp:834-843), and the quasi-periodic far field.  Stage bits
+    // as evaluate().  Output (Re, Im) per point, user order.
+    std::vector<Real> evaluate_complex(const double *charges, const double *bloch, int stages = 15) {
+        std::vector<Real> q(n_src);
+        for (int i = 0; i < n_src; ++i) q[i] = (Real)charges[i];
+        std::vector<Real> out(2 * (size_t)n_src, 0), part[2];
+        for (int comp = 0; comp < 2; ++comp) {
+            std::vector<Real> chgall(nsall);
+            for (int i = 0; i < n_src; ++i) chgall[i] = comp == 0 ? q[i] : Real(0);
+            for (size_t j = 0; j < img_src_idx.size(); ++j) {
+                const Real ph = (Real)(bloch[0] * img_shift[3 * j] + bloch[1] * img_shift[3 * j + 1] +
+                                       bloch[2] * img_shift[3 * j + 2]);
+                chgall[n_src + j] = q[img_src_idx[j]] * (comp == 0 ? std::cos(ph) : std::sin(ph));
+            }
+            update_charges(chgall.data());
+            if (stages & 1) {
+                upward_pass();
+                std::fill(down_zeroed.begin(), down_zeroed.end(), 0);
+                form_outgoing();
+                for (int l = 0; l < n_levels(); ++l) form_eval(l);
+            }
+            if (stages & 2) direct();
+            if (comp == 0 && (stages & 4)) self_correction(q.data());
+            part[comp].assign(n_src, 0);
+            for (int i = 0; i < n_src; ++i) part[comp][trg_perm[i]] = pot_trg_sorted[i];
+        }
+        if (stages & 8) far_field_qp(q.data(), bloch, part[0].data(), part[1].data());
+        for (int i = 0; i < n_src; ++i) {
+            out[2 * (size_t)i] = part[0][i];
+            out[2 * (size_t)i + 1] = part[1][i];
+        }
+        return out;
     }
 
     // ---------------------------------------------------------------- evaluate

@@ -68,6 +68,11 @@ def lib():
         "pdmk_gpu_kernel_times": (C.c_int, [vp, dp]),
         "pdmk_host_tree": (C.c_int, [dp, C.c_int, dp, C.c_int, C.c_int, lp, ip, dp, ip, ip, dp]),
         "pdmk_host_tables": (C.c_int, [C.c_int, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "pdmk_create_qp_double": (C.c_void_p, [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]),
+        "pdmk_create_qp_float": (C.c_void_p, [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int, dp]),
+        "pdmk_evaluate_complex_double": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "pdmk_evaluate_complex_float": (C.c_int, [vp, C.c_int, dp, dp, C.POINTER(C.c_float)]),
+        "pdmk_gpu_evaluate_complex": (C.c_int, [vp, vp, vp, C.c_int]),
         "pdmk_gpu_build_info": (C.c_int, [vp, dp]),
         "pdmk_campaign_sha256": (C.c_int, [vp, C.c_size_t, vp]),
         "pdmk_campaign_write": (C.c_int, [C.c_char_p, C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_ulonglong, dp,
@@ -199,7 +204,7 @@ class DeviceArray:
 class PDMKSolver:
     """Reusable periodic DMK tree on the GPU (mirror of the Julia ``PDMKSolver``)."""
 
-    def __init__(self, cell, positions, n_digits=6, n_per_leaf=300, nz=4, precision="auto"):
+    def __init__(self, cell, positions, n_digits=6, n_per_leaf=300, nz=4, precision="auto", bloch=None):
         if precision == "auto":
             precision = "float" if n_digits <= 6 else "double"
         if precision not in ("float", "double"):
@@ -208,8 +213,13 @@ class PDMKSolver:
         self.dtype = np.float32 if precision == "float" else np.float64
         cellc = _cell_colmajor(cell)
         pos = _positions(positions)
-        create = lib().pdmk_create_float if precision == "float" else lib().pdmk_create_double
-        self._h = create(_d(cellc), pos.shape[0], _d(pos), int(n_digits), int(n_per_leaf), int(nz))
+        self.bloch = None if bloch is None else np.ascontiguousarray(np.asarray(bloch, dtype=np.float64).reshape(3))
+        if self.bloch is None:
+            create = lib().pdmk_create_float if precision == "float" else lib().pdmk_create_double
+            self._h = create(_d(cellc), pos.shape[0], _d(pos), int(n_digits), int(n_per_leaf), int(nz))
+        else:
+            create = lib().pdmk_create_qp_float if precision == "float" else lib().pdmk_create_qp_double
+            self._h = create(_d(cellc), pos.shape[0], _d(pos), int(n_digits), int(n_per_leaf), int(nz), _d(self.bloch))
         if not self._h:
             raise RuntimeError("pdmk_create failed (see stderr)")
         self.n = pos.shape[0]
@@ -229,6 +239,35 @@ class PDMKSolver:
         if rc != 0:
             raise RuntimeError("pdmk_evaluate failed (see stderr)")
         return out
+
+    def evaluate_complex(self, charges, positions=None):
+        """Quasi-periodic potentials (complex, user order); needs ``bloch`` at construction
+        (PeriodicDMKTreeT::evaluate_complex)."""
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        if q.shape != (self.n,):
+            raise ValueError("charges size != n_src")
+        out = np.empty(2 * self.n, dtype=self.dtype)
+        if self.precision == "float":
+            rc = lib().pdmk_evaluate_complex_float(self._h, self.n, C.POINTER(C.c_double)(), _d(q),
+                                                   out.ctypes.data_as(C.POINTER(C.c_float)))
+        else:
+            rc = lib().pdmk_evaluate_complex_double(self._h, self.n, C.POINTER(C.c_double)(), _d(q), _d(out))
+        if rc != 0:
+            raise RuntimeError("pdmk_evaluate_complex failed (see stderr)")
+        return out[0::2].astype(np.float64) + 1j * out[1::2].astype(np.float64)
+
+    def run_complex_stages(self, charges, stages: int):
+        """Quasi-periodic, device path, selected stages only (bits as in STAGE_*)."""
+        qd = DeviceArray.from_numpy(np.asarray(charges, dtype=np.float64))
+        out = DeviceArray(2 * self.n, self.dtype)
+        try:
+            if lib().pdmk_gpu_evaluate_complex(self._h, C.c_void_p(qd.ptr), C.c_void_p(out.ptr), int(stages)) != 0:
+                raise RuntimeError("pdmk_gpu_evaluate_complex failed (see stderr)")
+            res = out.numpy().astype(np.float64)
+        finally:
+            qd.free()
+            out.free()
+        return res[0::2] + 1j * res[1::2]
 
     def evaluate_device(self, d_charges: int, d_pot: int, stream: int = 0):
         """Enqueue evaluate() on device pointers (FP64 charges in, Real potentials out)."""

@@ -12,9 +12,11 @@
 #include "plan.hpp"
 #include "../host/tree_build.hpp"
 
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <optional>
 #include <stdexcept>
 
 namespace pdmk {
@@ -23,15 +25,14 @@ template <typename Real, int DIM>
 PeriodicDMKTreeT<Real, DIM>::PeriodicDMKTreeT(const MatND<DIM> &cell, const std::vector<VecND<DIM>> &positions,
                                               int n_digits, int n_per_leaf, int nz,
                                               std::optional<std::array<double, DIM>> bloch) {
-    if (bloch.has_value())
-        throw std::logic_error("PeriodicDMKTreeT: quasi-periodic (bloch) evaluation is not provided by this build");
     double c[9];
     for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) c[i + 3 * j] = cell[j][i];
     std::vector<double> pos(3 * positions.size());
     for (size_t i = 0; i < positions.size(); ++i)
         for (int d = 0; d < 3; ++d) pos[3 * i + d] = positions[i][d];
-    p_ = std::make_unique<gpu::Plan<Real>>(c, pos.data(), (int)positions.size(), n_digits, n_per_leaf, nz);
+    p_ = std::make_unique<gpu::Plan<Real>>(c, pos.data(), (int)positions.size(), n_digits, n_per_leaf, nz,
+                                           bloch.has_value() ? bloch->data() : nullptr);
     n_src_ = (int)positions.size();
     rc_ = p_->rc;
 }
@@ -41,6 +42,7 @@ PeriodicDMKTreeT<Real, DIM>::~PeriodicDMKTreeT() = default;
 
 template <typename Real, int DIM>
 std::vector<Real> PeriodicDMKTreeT<Real, DIM>::evaluate(const std::vector<double> &charges) {
+    if (p_->qp) throw std::logic_error("evaluate: bloch is set - use evaluate_complex()");
     if ((int)charges.size() != n_src_) throw std::invalid_argument("charges size != n_src");
     std::vector<Real> pot(n_src_);
     p_->evaluate_host(charges.data(), pot.data(), nullptr);
@@ -49,6 +51,7 @@ std::vector<Real> PeriodicDMKTreeT<Real, DIM>::evaluate(const std::vector<double
 
 template <typename Real, int DIM>
 std::vector<Real> PeriodicDMKTreeT<Real, DIM>::evaluate_timed(const std::vector<double> &charges, EvalTiming &timing) {
+    if (p_->qp) throw std::logic_error("evaluate_timed: bloch is set - use evaluate_complex()");
     if ((int)charges.size() != n_src_) throw std::invalid_argument("charges size != n_src");
     std::vector<Real> pot(n_src_);
     double t[3] = {0, 0, 0};
@@ -60,13 +63,22 @@ std::vector<Real> PeriodicDMKTreeT<Real, DIM>::evaluate_timed(const std::vector<
     return pot;
 }
 
+// cpp/src/periodic_dmk.cpp:251-277: logic_error without bloch, invalid_argument for a size
+// mismatch, (Re, Im) per point.
 template <typename Real, int DIM>
-std::vector<std::complex<Real>> PeriodicDMKTreeT<Real, DIM>::evaluate_complex(const std::vector<double> &) {
-    throw std::logic_error("evaluate_complex: quasi-periodic evaluation is not provided by this build");
+std::vector<std::complex<Real>> PeriodicDMKTreeT<Real, DIM>::evaluate_complex(const std::vector<double> &charges) {
+    if (!p_->qp) throw std::logic_error("evaluate_complex: bloch not set at construction");
+    if ((int)charges.size() != n_src_) throw std::invalid_argument("charges size != n_src");
+    std::vector<Real> flat(2 * (size_t)n_src_);
+    p_->evaluate_complex_host(charges.data(), flat.data());
+    std::vector<std::complex<Real>> out(n_src_);
+    for (int i = 0; i < n_src_; ++i) out[i] = std::complex<Real>(flat[2 * (size_t)i], flat[2 * (size_t)i + 1]);
+    return out;
 }
 
 template <typename Real, int DIM>
 void PeriodicDMKTreeT<Real, DIM>::evaluate_device(const double *d_charges, Real *d_pot, void *stream) {
+    if (p_->qp) throw std::logic_error("evaluate: bloch is set - use evaluate_complex()");
     p_->evaluate(d_charges, d_pot, static_cast<hipStream_t>(stream));
 }
 
@@ -126,10 +138,13 @@ std::vector<pdmk::Vec3> to_positions(const double *p, int N) {
 }
 
 template <typename Real>
-void *create_impl(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz) {
+void *create_impl(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz,
+                  const double *bloch = nullptr) {
     try {
         if (!cell || !pos) throw std::invalid_argument("null cell or positions");
-        auto *t = new pdmk::PeriodicDMKTreeT<Real, 3>(to_mat3(cell), to_positions(pos, N), n_digits, n_per_leaf, nz);
+        std::optional<std::array<double, 3>> b;
+        if (bloch) b = std::array<double, 3>{bloch[0], bloch[1], bloch[2]};
+        auto *t = new pdmk::PeriodicDMKTreeT<Real, 3>(to_mat3(cell), to_positions(pos, N), n_digits, n_per_leaf, nz, b);
         return new Handle{sizeof(Real) == 4, t};
     } catch (const std::exception &e) {
         std::fprintf(stderr, "pdmk_create: %s\n", e.what());
@@ -150,6 +165,26 @@ int evaluate_impl(void *h, const double *charges, Out *pot_out) {
         return 0;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "pdmk_evaluate: %s\n", e.what());
+        return -1;
+    }
+}
+
+template <typename Real, typename Out>
+int evaluate_complex_impl(void *h, const double *charges, Out *pot_out) {
+    if (!h || !charges || !pot_out) return -1;
+    try {
+        if (static_cast<Handle *>(h)->is_float != (sizeof(Real) == 4)) throw std::invalid_argument("handle precision mismatch");
+        auto *t = tree_of<Real>(h);
+        const int N = t->n_points();
+        std::vector<double> q(charges, charges + N);
+        auto pot = t->evaluate_complex(q);
+        for (int i = 0; i < N; ++i) {
+            pot_out[2 * (size_t)i] = static_cast<Out>(pot[i].real());
+            pot_out[2 * (size_t)i + 1] = static_cast<Out>(pot[i].imag());
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_evaluate_complex: %s\n", e.what());
         return -1;
     }
 }
@@ -227,6 +262,41 @@ int pdmk_evaluate_float(void *h, int, const double *, const double *charges, flo
     return evaluate_impl<float, float>(h, charges, pot_out);
 }
 void pdmk_destroy_float(void *h) { destroy_impl<float>(h); }
+
+// ---------------------------------------------------------------- quasi-periodic
+void *pdmk_create_qp_double(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz,
+                            const double *bloch) {
+    if (!bloch) return nullptr;
+    return create_impl<double>(cell, N, pos, n_digits, n_per_leaf, nz, bloch);
+}
+void *pdmk_create_qp_float(const double *cell, int N, const double *pos, int n_digits, int n_per_leaf, int nz,
+                           const double *bloch) {
+    if (!bloch) return nullptr;
+    return create_impl<float>(cell, N, pos, n_digits, n_per_leaf, nz, bloch);
+}
+int pdmk_evaluate_complex_double(void *h, int, const double *, const double *charges, double *pot_out) {
+    return evaluate_complex_impl<double, double>(h, charges, pot_out);
+}
+int pdmk_evaluate_complex_float(void *h, int, const double *, const double *charges, float *pot_out) {
+    return evaluate_complex_impl<float, float>(h, charges, pot_out);
+}
+int pdmk_gpu_evaluate_complex(void *h, const double *d_charges, void *d_pot, int stages) {
+    if (!h) return -1;
+    return guarded("pdmk_gpu_evaluate_complex", [&] {
+        hipStream_t s = nullptr;
+        if (static_cast<Handle *>(h)->is_float) {
+            auto &pl = tree_of<float>(h)->plan();
+            pl.evaluate_complex(d_charges, static_cast<float *>(d_pot), nullptr, stages);
+            s = pl.stream;
+        } else {
+            auto &pl = tree_of<double>(h)->plan();
+            pl.evaluate_complex(d_charges, static_cast<double *>(d_pot), nullptr, stages);
+            s = pl.stream;
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) throw std::runtime_error("hipStreamSynchronize failed");
+        return 0;
+    });
+}
 
 // ---------------------------------------------------------------- device memory helpers
 int pdmk_gpu_device_count(void) {

@@ -47,29 +47,38 @@ template <> __device__ __forceinline__ float rsqrt_full<float>(float x) { return
 template <> __device__ __forceinline__ double rsqrt_full<double>(double x) { return rsqrt(x); }
 
 // ------------------------------------------------------------------ O(N) stages
+// Charges of the sorted sources.  Quasi-periodic passes multiply each by the real (comp 0)
+// or imaginary (comp 1) part of its image phase e^{i b.R} (originals: phase 1).
 template <class R>
 __global__ void k_gather_charges(const double *__restrict__ q, const int *__restrict__ src_user, P4<R> *__restrict__ src,
-                                 int nsall, R *__restrict__ pot_sorted, R *__restrict__ pot_near, int n) {
+                                 int nsall, R *__restrict__ pot_sorted, R *__restrict__ pot_near, int n,
+                                 const double2 *__restrict__ phase, int comp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nsall) src[i].w = (R)q[src_user[i]];
+    if (i < nsall) {
+        double v = q[src_user[i]];
+        if (phase) v *= comp == 0 ? phase[i].x : phase[i].y;
+        src[i].w = (R)v;
+    }
     if (i < n) {
         pot_sorted[i] = R(0);
         pot_near[i] = R(0);
     }
 }
 
-// pot[u] = pot_sorted[i] - w0_i q_u + far[u], u = trg_user[i]
+// pot[stride u] = pot_sorted[i] - w0_i q_u + far[u], u = trg_user[i] (stride 2: one component
+// of an interleaved complex output)
 template <class R>
 __global__ void k_finalize(const R *__restrict__ pot_sorted, const R *__restrict__ pot_near, const double *__restrict__ w0,
                            const int *__restrict__ trg_user, const double *__restrict__ q, const double *__restrict__ far,
-                           const int *__restrict__ far_idx, R *__restrict__ pot, int n, int use_self, int use_far) {
+                           const int *__restrict__ far_idx, R *__restrict__ pot, int n, int use_self, int use_far,
+                           int stride) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int u = trg_user[i];
     double v = (double)pot_sorted[i] + (double)pot_near[i];
     if (use_self) v -= w0[i] * q[u];
     if (use_far) v += far[far_idx[i]];
-    pot[u] = (R)v;
+    pot[(size_t)stride * u] = (R)v;
 }
 
 // ------------------------------------------------------------------ upward pass
@@ -1542,6 +1551,124 @@ __global__ __launch_bounds__(64) void k_nufft_interp(const double *__restrict__ 
         }
     }
     reduce(nb);
+}
+
+
+// ------------------------------------------------------------------ quasi-periodic far field
+// psi(x_u) = e^{i b.x_u} sum_m w_m S(m) e^{i m.theta_u},  S(m) = sum_l q_l e^{-i b.x_l}
+// e^{-i m.theta_l}, over the full mode box m = (i, j, k) - (ms/2, mt/2, mu/2), index
+// i + ms (j + mt k); w_m = 0 outside the window (cpp/src/dmk/pbc.cpp:249-355).  FP64.
+// Sp[y][m]: partial structure factors of the points of workgroup column y; per tile of tp
+// points the per-axis factors e^{-i m_d theta_d} of every point are tabulated in LDS by
+// recurrence (tp (ms + mt + mu + 1) complex values).
+__global__ __launch_bounds__(256) void k_qp_structure(const double *__restrict__ theta, const double *__restrict__ bx,
+                                                       const double *__restrict__ q, int n, int ms, int mt, int mu, int tp,
+                                                       double2 *__restrict__ Sp) {
+    extern __shared__ double2 qp_lds[];
+    double2 *E1 = qp_lds, *E2 = E1 + tp * ms, *E3 = E2 + tp * mt, *Q = E3 + tp * mu;
+    const int M = ms * mt * mu, m = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = m < M;
+    const int i = active ? m % ms : 0, j = active ? (m / ms) % mt : 0, k = active ? m / (ms * mt) : 0;
+    double2 acc = make_double2(0, 0);
+    for (int p0 = blockIdx.y * tp; p0 < n; p0 += gridDim.y * tp) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < tp * 4; t += blockDim.x) {
+            const int pp = t % tp, ax = t / tp, gp = p0 + pp;
+            if (ax == 3) {  // q e^{-i b.x}
+                double sn = 0, cs = 0;
+                if (gp < n) {
+                    sincos(-bx[gp], &sn, &cs);
+                    sn *= q[gp];
+                    cs *= q[gp];
+                }
+                Q[pp] = make_double2(cs, sn);
+                continue;
+            }
+            const int cnt = ax == 0 ? ms : (ax == 1 ? mt : mu);
+            double2 *row = (ax == 0 ? E1 : (ax == 1 ? E2 : E3)) + pp * cnt;
+            const double th = gp < n ? theta[3 * (size_t)gp + ax] : 0.0;
+            double s1, c1, sl, cl;
+            sincos(th, &s1, &c1);
+            sincos(th * (cnt / 2), &sl, &cl);
+            const double2 step = make_double2(c1, -s1);  // e^{-i th}
+            double2 cur = make_double2(cl, sl);          // e^{-i (-(cnt/2)) th}
+            for (int mm = 0; mm < cnt; ++mm) {
+                row[mm] = cur;
+                cur = cmul(cur, step);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const int np = min(tp, n - p0);
+            for (int pp = 0; pp < np; ++pp) {
+                const double2 e = cmul(cmul(E1[pp * ms + i], E2[pp * mt + j]), E3[pp * mu + k]);
+                const double2 qq = Q[pp];
+                acc.x = fma(qq.x, e.x, fma(-qq.y, e.y, acc.x));
+                acc.y = fma(qq.x, e.y, fma(qq.y, e.x, acc.y));
+            }
+        }
+    }
+    if (active) Sp[(size_t)blockIdx.y * M + m] = acc;
+}
+
+// S[m] = w[m] sum_y Sp[y][m], partials added in column order (deterministic).
+__global__ __launch_bounds__(256) void k_qp_reduce(const double2 *__restrict__ Sp, int nblk, const double *__restrict__ w,
+                                                    int M, double2 *__restrict__ S) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    double2 a = make_double2(0, 0);
+    for (int b = 0; b < nblk; ++b) {
+        const double2 v = Sp[(size_t)b * M + m];
+        a.x += v.x;
+        a.y += v.y;
+    }
+    S[m] = make_double2(a.x * w[m], a.y * w[m]);
+}
+
+// out[2u], out[2u + 1] += Re, Im psi(x_u): nested sums over the axes, each a recurrence in
+// e^{i theta_d}, with the weighted structure factors read at wave-uniform addresses.
+template <class R>
+__global__ __launch_bounds__(256) void k_qp_eval(const double *__restrict__ theta, const double *__restrict__ bx, int n,
+                                                  int ms, int mt, int mu, const double2 *__restrict__ S, R *__restrict__ out) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    double2 st[3], e[3];
+    const int cnt[3] = {ms, mt, mu};
+    for (int d = 0; d < 3; ++d) {
+        const double th = theta[3 * (size_t)u + d];
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        e[d] = make_double2(cs, sn);  // e^{i th}
+        sincos(-th * (cnt[d] / 2), &sn, &cs);
+        st[d] = make_double2(cs, sn);  // e^{i (-(cnt/2)) th}
+    }
+    double2 acc = make_double2(0, 0), c3 = st[2];
+    for (int k = 0; k < mu; ++k) {
+        double2 acc2 = make_double2(0, 0), c2 = st[1];
+        for (int j = 0; j < mt; ++j) {
+            const double2 *row = S + ((size_t)k * mt + j) * ms;
+            double2 acc1 = make_double2(0, 0), c1 = st[0];
+            for (int i = 0; i < ms; ++i) {
+                const double2 v = row[i];
+                acc1.x = fma(v.x, c1.x, fma(-v.y, c1.y, acc1.x));
+                acc1.y = fma(v.x, c1.y, fma(v.y, c1.x, acc1.y));
+                c1 = cmul(c1, e[0]);
+            }
+            const double2 t2 = cmul(acc1, c2);
+            acc2.x += t2.x;
+            acc2.y += t2.y;
+            c2 = cmul(c2, e[1]);
+        }
+        const double2 t3 = cmul(acc2, c3);
+        acc.x += t3.x;
+        acc.y += t3.y;
+        c3 = cmul(c3, e[2]);
+    }
+    double sn, cs;
+    sincos(bx[u], &sn, &cs);
+    const double2 psi = cmul(acc, make_double2(cs, sn));
+    out[2 * (size_t)u] = (R)((double)out[2 * (size_t)u] + psi.x);
+    out[2 * (size_t)u + 1] = (R)((double)out[2 * (size_t)u + 1] + psi.y);
 }
 
 }  // namespace pdmk::gpu

@@ -49,7 +49,8 @@ int nc_for_digits(int d) {
 }  // namespace
 
 template <typename Real>
-Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user, int n_per_leaf, int nz) {
+Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user, int n_per_leaf, int nz,
+                 const double *bloch_in) {
     if (n_digits_user < 1 || n_digits_user > 12) throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
     check(hipStreamCreate(&stream), "hipStreamCreate");
     for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
@@ -132,6 +133,22 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         const int o = t.src_order[j];
         su[i] = o < N ? o : g.img_src[o - N];
     }
+    if (bloch_in) {
+        // quasi-periodic image phases e^{i b.R}, R = image position - original position in the
+        // lattice frame (the translation of generate_images, cpp/include/dmk/pbc.hpp:221-229)
+        qp = true;
+        for (int d = 0; d < 3; ++d) bloch[d] = bloch_in[d];
+        std::vector<double2> ph(nsall);
+        for (int i = 0; i < nsall; ++i) {
+            const int j = src_ord[i], o = t.src_order[j];
+            double phi = 0;
+            if (o >= N)
+                for (int d = 0; d < 3; ++d)
+                    phi += bloch[d] * (t.src_sorted[3 * (size_t)j + d] - g.r_src[3 * (size_t)su[i] + d]);
+            ph[i] = make_double2(std::cos(phi), std::sin(phi));
+        }
+        src_phase.upload(ph);
+    }
     std::vector<P4<Real>> trgv(n);
     std::vector<int> tu(n);
     std::vector<double> tw0(n, 0.0);
@@ -165,11 +182,13 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     // same positions relative to the box centres), every box of the image subtree has its
     // counterpart's proxy charges and outgoing plane waves, which are relative to the box
     // centre: those boxes are not computed, and the downward pass reads the counterpart's.
-    // fold[b] = the counterpart of b, or -1.  PDMK_FOLD=0 turns it off.
+    // fold[b] = the counterpart of b, or -1.  PDMK_FOLD=0 turns it off.  Quasi-periodic trees
+    // never fold: an image's charge carries its translation's phase, so image boxes are not
+    // copies of their counterparts.
     std::vector<int> fold(nboxes, -1);
     {
         const char *fe = std::getenv("PDMK_FOLD");
-        const bool enabled = !(fe && std::string(fe) == "0");
+        const bool enabled = !(fe && std::string(fe) == "0") && !qp;
         auto user_of = [&](int j) {  // tree-order source -> user index
             const int o = t.src_order[j];
             return o < N ? o : g.img_src[o - N];
@@ -585,6 +604,38 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w);
         }
         far_idx.upload(fidx);
+        if (qp) {
+            // quasi-periodic far field (cpp/src/dmk/pbc.cpp:249-303): mode box from
+            // dkmax = beta/rc + |b|, weights at the shifted wave vectors, b.x per point
+            const double twopi = 2 * M_PI, *U = g.umat;
+            const double bn = std::sqrt(bloch[0] * bloch[0] + bloch[1] * bloch[1] + bloch[2] * bloch[2]);
+            const double dk = T.beta / rc + bn;
+            const double n1 = U[0], n2 = std::sqrt(U[3] * U[3] + U[4] * U[4]),
+                         n3 = std::sqrt(U[6] * U[6] + U[7] * U[7] + U[8] * U[8]);
+            const int qs = std::max(2, 2 * (int)std::ceil(dk * n1 / twopi));
+            const int qt = std::max(2, 2 * (int)std::ceil(dk * n2 / twopi));
+            const int qu = std::max(2, 2 * (int)std::ceil(dk * n3 / twopi));
+            const double *a = T.ainv;  // a11, a21, a22, a31, a32, a33
+            const double dfac = 4 * M_PI / (T.psi0 * U[0] * U[4] * U[8]);
+            const host::Pswf pf(T.beta);
+            std::vector<double> wq((size_t)qs * qt * qu, 0.0), bx(n);
+            for (int k = 0; k < qu; ++k)
+                for (int j = 0; j < qt; ++j)
+                    for (int i = 0; i < qs; ++i) {
+                        const int mi = i - qs / 2, mj = j - qt / 2, mk = k - qu / 2;
+                        const double k1 = twopi * a[0] * mi + bloch[0], k2 = twopi * (a[1] * mi + a[2] * mj) + bloch[1],
+                                     k3 = twopi * (a[3] * mi + a[4] * mj + a[5] * mk) + bloch[2];
+                        const double kk = k1 * k1 + k2 * k2 + k3 * k3;
+                        if (kk < 1e-30) continue;
+                        const double x = std::sqrt(kk) * rc / T.beta;
+                        if (x <= 1) wq[i + (size_t)qs * (j + (size_t)qt * k)] = dfac * pf(x) / kk;
+                    }
+            for (int u = 0; u < n; ++u)
+                bx[u] = bloch[0] * g.r_src[3 * (size_t)u] + bloch[1] * g.r_src[3 * (size_t)u + 1] +
+                        bloch[2] * g.r_src[3 * (size_t)u + 2];
+            qpf.setup(th, bx, n, qs, qt, qu, wq);
+            pot_dev2.alloc(2 * sizeof(Real) * (size_t)n);
+        }
         build_ms[2] = ms_since(t_far);
         work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
@@ -772,7 +823,8 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     {
         const int m = std::max(nsall, n);
         hipLaunchKernelGGL(k_gather_charges<Real>, dim3((m + 255) / 256), dim3(256), 0, s, d_q, src_user.as<int>(),
-                           src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), pot_near.as<Real>(), n);
+                           src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), pot_near.as<Real>(), n,
+                           qp_comp >= 0 ? src_phase.as<double2>() : (const double2 *)nullptr, qp_comp < 0 ? 0 : qp_comp);
         check(hipGetLastError(), "k_gather_charges");
     }
     mark(1, 1, s);
@@ -834,7 +886,7 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     mark(4, 0, s);
     hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(), pot_near.as<Real>(),
                        trg_w0.as<double>(), trg_user.as<int>(), d_q, far.as<double>(), far_idx.as<int>(), d_pot, n,
-                       (stages & 4) ? 1 : 0, (stages & 8) ? 1 : 0);
+                       (stages & 4) ? 1 : 0, (stages & 8) ? 1 : 0, qp_comp >= 0 ? 2 : 1);
     check(hipGetLastError(), "k_finalize");
     mark(4, 1, s);
     if (times_ms) {
@@ -881,6 +933,70 @@ int Plan<Real>::kernel_times(double *avg_ms, int *n_calls) {
         }
     if (n_calls) *n_calls = nc;
     return nc;
+}
+
+void QpFarField::setup(const std::vector<double> &theta_host, const std::vector<double> &bx_host, int n_points,
+                       int ms_, int mt_, int mu_, const std::vector<double> &w_full) {
+    n = n_points;
+    ms = ms_;
+    mt = mt_;
+    mu = mu_;
+    M = ms * mt * mu;
+    theta.upload(theta_host);
+    bx.upload(bx_host);
+    w.upload(w_full);
+    // points per LDS tile: up to 16 within 48 KB of tables; point columns: enough workgroups
+    // to fill the GPU, partials M complex each
+    tp = std::max(1, std::min(16, (int)((48 << 10) / (sizeof(double2) * (size_t)(ms + mt + mu + 1)))));
+    const int mblk = (M + 255) / 256;
+    nblk = std::max(1, std::min((n + tp - 1) / tp, std::max(1, 4096 / mblk)));
+    Sp.alloc(sizeof(double2) * (size_t)nblk * M);
+    S.alloc(sizeof(double2) * (size_t)M);
+}
+
+template <class R>
+void QpFarField::run(const double *d_q, R *d_out, hipStream_t s) const {
+    if (n == 0) return;
+    const size_t lds = sizeof(double2) * (size_t)tp * (ms + mt + mu + 1);
+    hipLaunchKernelGGL(k_qp_structure, dim3((M + 255) / 256, nblk), dim3(256), lds, s, theta.as<double>(), bx.as<double>(),
+                       d_q, n, ms, mt, mu, tp, Sp.as<double2>());
+    check(hipGetLastError(), "k_qp_structure");
+    hipLaunchKernelGGL(k_qp_reduce, dim3((M + 255) / 256), dim3(256), 0, s, Sp.as<double2>(), nblk, w.as<double>(), M,
+                       S.as<double2>());
+    check(hipGetLastError(), "k_qp_reduce");
+    hipLaunchKernelGGL(k_qp_eval<R>, dim3((n + 255) / 256), dim3(256), 0, s, theta.as<double>(), bx.as<double>(), n, ms, mt,
+                       mu, S.as<double2>(), d_out);
+    check(hipGetLastError(), "k_qp_eval");
+}
+template void QpFarField::run<float>(const double *, float *, hipStream_t) const;
+template void QpFarField::run<double>(const double *, double *, hipStream_t) const;
+
+template <typename Real>
+void Plan<Real>::evaluate_complex(const double *d_q, Real *d_out, hipStream_t s, int stages) {
+    if (!qp) throw std::logic_error("evaluate_complex: bloch not set at construction");
+    if (!s) s = stream;
+    // real part: proxy / plane waves, near field and self correction on Re(charges x phase);
+    // imaginary part: the same without the self correction (the originals' charges are real)
+    qp_comp = 0;
+    try {
+        evaluate(d_q, d_out, s, stages & 7, nullptr);
+        qp_comp = 1;
+        evaluate(d_q, d_out + 1, s, stages & 3, nullptr);
+    } catch (...) {
+        qp_comp = -1;
+        throw;
+    }
+    qp_comp = -1;
+    if (stages & 8) qpf.run(d_q, d_out, s);
+}
+
+template <typename Real>
+void Plan<Real>::evaluate_complex_host(const double *q, Real *out) {
+    check(hipMemcpyAsync(q_dev.p, q, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, stream), "hipMemcpyAsync H2D");
+    evaluate_complex(q_dev.as<double>(), pot_dev2.as<Real>(), stream);
+    check(hipMemcpyAsync(out, pot_dev2.p, 2 * sizeof(Real) * (size_t)n, hipMemcpyDeviceToHost, stream),
+          "hipMemcpyAsync D2H");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
 void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,

@@ -75,6 +75,16 @@ struct NufftFarField {
     void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
 };
 
+// Quasi-periodic far field, exact (all N x modes products): wave vectors 2 pi U^{-T} m + b
+// over the full mode box, complex output added to an interleaved (Re, Im) array.
+struct QpFarField {
+    int n = 0, ms = 0, mt = 0, mu = 0, M = 0, nblk = 0, tp = 0;
+    DevBuf theta, bx, w, Sp, S;
+    void setup(const std::vector<double> &theta_host, const std::vector<double> &bx_host, int n_points, int ms_,
+               int mt_, int mu_, const std::vector<double> &w_full);
+    template <class R> void run(const double *d_q, R *d_out, hipStream_t s) const;  // enqueue only
+};
+
 template <typename Real>
 struct Plan {
     // sizes
@@ -125,7 +135,9 @@ struct Plan {
     DevBuf pin;
     int pin_batch = 0;
 
-    Plan(const double *cell_colmajor, const double *pos, int N, int n_digits, int n_per_leaf, int nz);
+    // bloch (optional, 3 values): quasi-periodic tree; then only evaluate_complex applies
+    Plan(const double *cell_colmajor, const double *pos, int N, int n_digits, int n_per_leaf, int nz,
+         const double *bloch = nullptr);
     ~Plan();
     Plan(const Plan &) = delete;
     Plan &operator=(const Plan &) = delete;
@@ -137,6 +149,19 @@ struct Plan {
 
     // host buffers in, host buffers out
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
+
+    // Quasi-periodic evaluate (bloch given at construction): d_out holds 2N Real, (Re, Im)
+    // per point in user order.  Two passes of the proxy / plane-wave chain and the near field
+    // (charges times the real, then the imaginary part of each image's phase), the self
+    // correction on the real part, then the quasi-periodic far field.
+    bool qp = false;
+    double bloch[3] = {0, 0, 0};
+    DevBuf src_phase;   // (cos, sin) of b.R per sorted source (R: the image's lattice translation)
+    QpFarField qpf;
+    int qp_comp = -1;   // component of the pass in progress (-1: not quasi-periodic)
+    DevBuf pot_dev2;    // 2N Real for the host-buffer path
+    void evaluate_complex(const double *d_q, Real *d_out, hipStream_t s, int stages = 15);
+    void evaluate_complex_host(const double *q, Real *out);
 
     // Per-kernel-group timing with HIP events on the stream each group runs on (start and
     // end event per group).  Groups: 0 far field, 1 charge gather, 2 proxy/plane-wave
