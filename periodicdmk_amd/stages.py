@@ -2,13 +2,17 @@
 stage on its own (no other stream competing for the GPU):
 
     python -m periodicdmk_amd.stages --paths mixed6,fp64_9 --stages 1 --calls 3
+    python -m periodicdmk_amd.stages --paths fp64_6 --stages 15 --calls 4 --bloch 2,-1,0.5
 
 Stage bits: 1 proxy / plane-wave chain, 2 near field, 4 self correction, 8 far field.
+With --bloch the trees are quasi-periodic and the calls are evaluate_complex (device
+pointers).  Prints the mean wall time per call after the first (synchronized).
 Workload: cubic L=1, N uniform positions (splitmix64 seed 11), neutral U[-1,1] charges
 (seed 100), n_per_leaf 300, nz 4 -- the same points as bench.py.
 """
 import argparse
 import sys
+import time
 
 import periodicdmk_amd as P
 from periodicdmk_amd import workloads as W
@@ -22,6 +26,7 @@ def main():
     ap.add_argument("--stages", type=int, default=1)
     ap.add_argument("--calls", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--bloch", default="", help="bx,by,bz: quasi-periodic trees")
     args = ap.parse_args()
     if P.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -31,12 +36,23 @@ def main():
     q = P.DeviceArray.from_numpy(W.neutral_uniform_charges(args.n, 100))
     for name in [p for p in args.paths.split(",") if p]:
         prec, nd = PATHS[name]
-        s = P.PDMKSolver(cell, pos, n_digits=nd, n_per_leaf=300, nz=4, precision=prec)
-        out = P.DeviceArray(args.n, s.dtype)
+        bloch = [float(v) for v in args.bloch.split(",")] if args.bloch else None
+        s = P.PDMKSolver(cell, pos, n_digits=nd, n_per_leaf=300, nz=4, precision=prec, bloch=bloch)
+        out = P.DeviceArray(2 * args.n if bloch else args.n, s.dtype)
+        times = []
         for _ in range(args.calls):
-            s.evaluate_stages(q.ptr, out.ptr, args.stages)
-        P.synchronize()
-        print(f"[stages] {name}: {args.calls} calls of stages {args.stages}", flush=True)
+            t0 = time.perf_counter()
+            if bloch:
+                if P.lib().pdmk_gpu_evaluate_complex(s._h, P.C.c_void_p(q.ptr), P.C.c_void_p(out.ptr), args.stages) != 0:
+                    raise RuntimeError("pdmk_gpu_evaluate_complex failed")
+            else:
+                s.evaluate_stages(q.ptr, out.ptr, args.stages)
+            P.synchronize()
+            times.append(time.perf_counter() - t0)
+        ms = 1e3 * sum(times[1:]) / max(1, len(times) - 1)
+        kind = "evaluate_complex" if bloch else "evaluate_stages"
+        print(f"[stages] {name}: {args.calls} calls of {kind} stages {args.stages}, {ms:.3f} ms per call after the first",
+              flush=True)
         out.free()
         s.close()
     q.free()
