@@ -184,6 +184,26 @@ __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
 // D[l / 16 + 4 r][l % 16] in accumulator element r (layout read off gfx950 hardware).  Wave w of the workgroup owns the row
 // tiles w, w + NW, ...  and both column tiles.
 typedef double c2p_d4 __attribute__((ext_vector_type(4)));
+typedef float c2p_f4 __attribute__((ext_vector_type(4)));
+
+// 16 x 16 x 4 matrix-core products.  Operands: lane l holds A[l%16][l/16] and B[l/16][l%16].
+// Accumulator element e of lane l is D[row(l/16, e)][l%16]: rows l/16 + 4e for FP64 and
+// 4(l/16) + e for FP32 (both read off the hardware with in-bounds probes).
+template <class R> struct Mfma16;
+template <> struct Mfma16<double> {
+    typedef c2p_d4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kk, int e) { return kk + 4 * e; }
+};
+template <> struct Mfma16<float> {
+    typedef c2p_f4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kk, int e) { return 4 * kk + e; }
+};
 template <int P> struct C2pMfmaCfg {
     static constexpr int RT = (P * P + 15) / 16;      // row tiles
     static constexpr int NW = 8;                      // waves per workgroup (two per SIMD)
@@ -192,33 +212,34 @@ template <int P> struct C2pMfmaCfg {
     static constexpr int PA = 32;                     // a, padded
 };
 
-template <int P>
+template <class R, int P>
 __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
-    const P4<double> *__restrict__ src, const int *__restrict__ grp_slot, const double *__restrict__ grp_geom,
+    const P4<R> *__restrict__ src, const int *__restrict__ grp_slot, const R *__restrict__ grp_geom,
     const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
-    const double *__restrict__ rng_off, double *__restrict__ proxy) {
+    const R *__restrict__ rng_off, R *__restrict__ proxy) {
     chain_priority();
     using C = C2pMfmaCfg<P>;
+    using M = Mfma16<R>;
     constexpr int TS = C::TS, PA = C::PA, RT = C::RT, NW = C::NW, TPW = C::TPW;
     static_assert(P <= PA, "order above the padded column count");
-    __shared__ double Tx[TS * PA], Ty[TS * P], Tz[TS * P];
+    __shared__ R Tx[TS * PA], Ty[TS * P], Tz[TS * P];
     const int g = blockIdx.x, lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, jj = lane & 15;
-    const double sc = grp_geom[4 * g + 3];
+    const R sc = grp_geom[4 * g + 3];
     int ci[TPW], bi[TPW];
-    double rv[TPW];
+    R rv[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         const int row = (w + NW * t) * 16 + jj;
         const bool ok = row < P * P;
-        rv[t] = ok ? 1.0 : 0.0;
+        rv[t] = ok ? R(1) : R(0);
         ci[t] = ok ? row / P : 0;
         bi[t] = ok ? row % P : 0;
     }
-    c2p_d4 acc[TPW][2];
+    typename M::acc_t acc[TPW][2];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) acc[t][0] = acc[t][1] = c2p_d4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < TPW; ++t) acc[t][0] = acc[t][1] = typename M::acc_t{R(0), R(0), R(0), R(0)};
     for (int r = grp_rng[g]; r < grp_rng[g + 1]; ++r) {
         const int beg = rng_begin[r], cnt = rng_count[r];
         for (int s0 = 0; s0 < cnt; s0 += TS) {
@@ -226,42 +247,42 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
             __syncthreads();
             for (int t = threadIdx.x; t < 3 * TS; t += blockDim.x) {
                 const int s = t % TS, d = t / TS;
-                double *T = d == 0 ? Tx + s * PA : (d == 1 ? Ty + s * P : Tz + s * P);
+                R *T = d == 0 ? Tx + s * PA : (d == 1 ? Ty + s * P : Tz + s * P);
                 const int len = d == 0 ? PA : P;
                 if (s < ns) {
-                    const P4<double> pt = src[beg + s0 + s];
-                    const double x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) + rng_off[4 * r + d]);
-                    const double m = (d == 2) ? pt.w : 1.0;
-                    double t0 = m, t1 = x * m;
+                    const P4<R> pt = src[beg + s0 + s];
+                    const R x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) + rng_off[4 * r + d]);
+                    const R m = (d == 2) ? pt.w : R(1);
+                    R t0 = m, t1 = x * m;
                     T[0] = t0;
                     T[1] = t1;
                     for (int k = 2; k < P; ++k) {
-                        const double t2 = 2.0 * x * t1 - t0;
+                        const R t2 = R(2) * x * t1 - t0;
                         T[k] = t2;
                         t0 = t1;
                         t1 = t2;
                     }
-                    for (int k = P; k < len; ++k) T[k] = 0.0;
+                    for (int k = P; k < len; ++k) T[k] = R(0);
                 } else {
-                    for (int k = 0; k < len; ++k) T[k] = 0.0;
+                    for (int k = 0; k < len; ++k) T[k] = R(0);
                 }
             }
             __syncthreads();
             for (int k0 = 0; k0 < ns; k0 += 4) {
                 const int s = k0 + kk;
-                const double b0 = Tx[s * PA + jj], b1 = Tx[s * PA + 16 + jj];
+                const R b0 = Tx[s * PA + jj], b1 = Tx[s * PA + 16 + jj];
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
                     if (w + NW * t < RT) {
-                        const double a = Tz[s * P + ci[t]] * Ty[s * P + bi[t]] * rv[t];
-                        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[t][0], 0, 0, 0);
-                        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[t][1], 0, 0, 0);
+                        const R a = Tz[s * P + ci[t]] * Ty[s * P + bi[t]] * rv[t];
+                        acc[t][0] = M::mma(a, b0, acc[t][0]);
+                        acc[t][1] = M::mma(a, b1, acc[t][1]);
                     }
                 }
             }
         }
     }
-    double *out = proxy + (size_t)grp_slot[g] * (P * P * P);
+    R *out = proxy + (size_t)grp_slot[g] * (P * P * P);
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         if (w + NW * t >= RT) continue;
@@ -270,7 +291,7 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
             const int a = ct * 16 + jj;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int row = (w + NW * t) * 16 + kk + 4 * e;
+                const int row = (w + NW * t) * 16 + M::row(kk, e);
                 if (row < P * P && a < P) out[(size_t)row * P + a] = acc[t][ct][e];
             }
         }
@@ -320,7 +341,9 @@ template <class R, int P> struct T3Cfg {
     static constexpr int VW = 16 / (int)sizeof(R);           // matrix entries per 128-bit LDS read
     static constexpr int PR = (P + VW - 1) / VW * VW;        // padded matrix row
     static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * P * PR * (int)sizeof(R));
-    static constexpr int threads = PP <= 384 ? (PP + 63) / 64 * 64 : (P >= 38 ? 512 : 256);
+    // at most 4 waves below p = 38: evaluate() runs these beside the near field, whose
+    // retiring 4-wave workgroups free 4 wave slots at a time
+    static constexpr int threads = PP <= 256 ? (PP + 63) / 64 * 64 : (P >= 38 ? 512 : 256);
 };
 
 template <class R, int P, bool TRANSPOSE>

@@ -708,19 +708,21 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     using C = Cx<Real>;
     const int NM = NPW * NPW * ((NPW + 1) / 2);
     if (pl.n_up > 0) {
-        // FP64 at P = 28 (9 digits): matrix cores (a padded to 32 costs 1/8); smaller orders pad
-        // too much and stay on the vector ALUs
+        // FP64 at P = 28 (9 digits): matrix cores (a padded to 32 costs 1/8).  FP64 at P = 18
+        // pads too much.  FP32 at P = 18 on the matrix cores is faster alone (395 -> 307 us at
+        // N=1e6) but slower inside evaluate() (3.02 -> 3.24 ms): its 8-wave workgroups take
+        // wave slots from the near field beside it; it stays on the vector ALUs.
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
-            hipLaunchKernelGGL((k_charge2proxy_mfma<P>), dim3(pl.n_up), dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
+            hipLaunchKernelGGL((k_charge2proxy_mfma<Real, P>), dim3(pl.n_up), dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
                                pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
                                pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
                                pl.rng_off.template as<Real>(), pl.proxy_up.template as<Real>());
             check(hipGetLastError(), "k_charge2proxy_mfma");
         } else {
-            hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(C2pCfg<P>::threads), 0, s, pl.src.template as<P4<Real>>(),
-                           pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(), pl.up_rng.template as<int>(),
-                           pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(), pl.rng_off.template as<Real>(),
-                           pl.proxy_up.template as<Real>());
+            hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(C2pCfg<P>::threads), 0, s,
+                               pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
+                               pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
+                               pl.rng_off.template as<Real>(), pl.proxy_up.template as<Real>());
             check(hipGetLastError(), "k_charge2proxy");
         }
     }
