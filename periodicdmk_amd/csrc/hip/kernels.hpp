@@ -1599,18 +1599,24 @@ __global__ __launch_bounds__(64) void k_nufft_interp(const double *__restrict__ 
 // psi(x_u) = e^{i b.x_u} sum_m w_m S(m) e^{i m.theta_u},  S(m) = sum_l q_l e^{-i b.x_l}
 // e^{-i m.theta_l}, over the full mode box m = (i, j, k) - (ms/2, mt/2, mu/2), index
 // i + ms (j + mt k); w_m = 0 outside the window (cpp/src/dmk/pbc.cpp:249-355).  FP64.
-// Sp[y][m]: partial structure factors of the points of workgroup column y; per tile of tp
-// points the per-axis factors e^{-i m_d theta_d} of every point are tabulated in LDS by
-// recurrence (tp (ms + mt + mu + 1) complex values).
+// Sp[y][m]: partial structure factors of the points of workgroup column y.  Per tile of tp
+// points the per-axis factors e^{-i m_d theta_d} are tabulated in LDS by recurrence
+// (tp (ms + mt + mu + 2) complex values); a thread owns QP_CHUNK consecutive i of one (j, k)
+// line and walks them by multiplying with e^{-i theta_1}, so a (point, mode) pair costs one
+// complex multiply-add.
+constexpr int QP_CHUNK = 8;
 __global__ __launch_bounds__(256) void k_qp_structure(const double *__restrict__ theta, const double *__restrict__ bx,
                                                        const double *__restrict__ q, int n, int ms, int mt, int mu, int tp,
                                                        double2 *__restrict__ Sp) {
     extern __shared__ double2 qp_lds[];
-    double2 *E1 = qp_lds, *E2 = E1 + tp * ms, *E3 = E2 + tp * mt, *Q = E3 + tp * mu;
-    const int M = ms * mt * mu, m = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = m < M;
-    const int i = active ? m % ms : 0, j = active ? (m / ms) % mt : 0, k = active ? m / (ms * mt) : 0;
-    double2 acc = make_double2(0, 0);
+    double2 *E1 = qp_lds, *E2 = E1 + tp * ms, *E3 = E2 + tp * mt, *Q = E3 + tp * mu, *Est = Q + tp;
+    const int M = ms * mt * mu, nci = (ms + QP_CHUNK - 1) / QP_CHUNK;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = c < nci * mt * mu;
+    const int i0 = active ? QP_CHUNK * (c % nci) : 0, j = active ? (c / nci) % mt : 0, k = active ? c / (nci * mt) : 0;
+    double2 acc[QP_CHUNK];
+#pragma unroll
+    for (int r = 0; r < QP_CHUNK; ++r) acc[r] = make_double2(0, 0);
     for (int p0 = blockIdx.y * tp; p0 < n; p0 += gridDim.y * tp) {
         __syncthreads();
         for (int t = threadIdx.x; t < tp * 4; t += blockDim.x) {
@@ -1632,7 +1638,8 @@ __global__ __launch_bounds__(256) void k_qp_structure(const double *__restrict__
             sincos(th, &s1, &c1);
             sincos(th * (cnt / 2), &sl, &cl);
             const double2 step = make_double2(c1, -s1);  // e^{-i th}
-            double2 cur = make_double2(cl, sl);          // e^{-i (-(cnt/2)) th}
+            if (ax == 0) Est[pp] = step;
+            double2 cur = make_double2(cl, sl);  // e^{-i (-(cnt/2)) th}
             for (int mm = 0; mm < cnt; ++mm) {
                 row[mm] = cur;
                 cur = cmul(cur, step);
@@ -1642,14 +1649,23 @@ __global__ __launch_bounds__(256) void k_qp_structure(const double *__restrict__
         if (active) {
             const int np = min(tp, n - p0);
             for (int pp = 0; pp < np; ++pp) {
-                const double2 e = cmul(cmul(E1[pp * ms + i], E2[pp * mt + j]), E3[pp * mu + k]);
-                const double2 qq = Q[pp];
-                acc.x = fma(qq.x, e.x, fma(-qq.y, e.y, acc.x));
-                acc.y = fma(qq.x, e.y, fma(qq.y, e.x, acc.y));
+                const double2 st = Est[pp];
+                double2 cur = cmul(cmul(cmul(E1[pp * ms + i0], E2[pp * mt + j]), E3[pp * mu + k]), Q[pp]);
+#pragma unroll
+                for (int r = 0; r < QP_CHUNK; ++r) {
+                    acc[r].x += cur.x;
+                    acc[r].y += cur.y;
+                    cur = cmul(cur, st);
+                }
             }
         }
     }
-    if (active) Sp[(size_t)blockIdx.y * M + m] = acc;
+    if (active) {
+        const int m0 = i0 + ms * (j + mt * k);
+#pragma unroll
+        for (int r = 0; r < QP_CHUNK; ++r)
+            if (i0 + r < ms) Sp[(size_t)blockIdx.y * M + m0 + r] = acc[r];
+    }
 }
 
 // S[m] = w[m] sum_y Sp[y][m], partials added in column order (deterministic).

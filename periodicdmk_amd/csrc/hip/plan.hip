@@ -949,9 +949,9 @@ void QpFarField::setup(const std::vector<double> &theta_host, const std::vector<
     w.upload(w_full);
     // points per LDS tile: up to 16 within 48 KB of tables; point columns: enough workgroups
     // to fill the GPU, partials M complex each
-    tp = std::max(1, std::min(16, (int)((48 << 10) / (sizeof(double2) * (size_t)(ms + mt + mu + 1)))));
-    const int mblk = (M + 255) / 256;
-    nblk = std::max(1, std::min((n + tp - 1) / tp, std::max(1, 4096 / mblk)));
+    tp = std::max(1, std::min(16, (int)((48 << 10) / (sizeof(double2) * (size_t)(ms + mt + mu + 2)))));
+    const int cblk = (((ms + QP_CHUNK - 1) / QP_CHUNK) * mt * mu + 255) / 256;
+    nblk = std::max(1, std::min((n + tp - 1) / tp, std::max(1, 4096 / cblk)));
     Sp.alloc(sizeof(double2) * (size_t)nblk * M);
     S.alloc(sizeof(double2) * (size_t)M);
 }
@@ -959,8 +959,9 @@ void QpFarField::setup(const std::vector<double> &theta_host, const std::vector<
 template <class R>
 void QpFarField::run(const double *d_q, R *d_out, hipStream_t s) const {
     if (n == 0) return;
-    const size_t lds = sizeof(double2) * (size_t)tp * (ms + mt + mu + 1);
-    hipLaunchKernelGGL(k_qp_structure, dim3((M + 255) / 256, nblk), dim3(256), lds, s, theta.as<double>(), bx.as<double>(),
+    const size_t lds = sizeof(double2) * (size_t)tp * (ms + mt + mu + 2);
+    const int nch = ((ms + QP_CHUNK - 1) / QP_CHUNK) * mt * mu;
+    hipLaunchKernelGGL(k_qp_structure, dim3((nch + 255) / 256, nblk), dim3(256), lds, s, theta.as<double>(), bx.as<double>(),
                        d_q, n, ms, mt, mu, tp, Sp.as<double2>());
     check(hipGetLastError(), "k_qp_structure");
     hipLaunchKernelGGL(k_qp_reduce, dim3((M + 255) / 256), dim3(256), 0, s, Sp.as<double2>(), nblk, w.as<double>(), M,
