@@ -1682,8 +1682,10 @@ __global__ __launch_bounds__(256) void k_qp_reduce(const double2 *__restrict__ S
     S[m] = make_double2(a.x * w[m], a.y * w[m]);
 }
 
-// out[2u], out[2u + 1] += Re, Im psi(x_u): nested sums over the axes, each a recurrence in
-// e^{i theta_d}, with the weighted structure factors read at wave-uniform addresses.
+// out[2u], out[2u + 1] += Re, Im psi(x_u).  The i axis is taken in chunks of QP_CHUNK: the
+// chunk's factors e^{i m_1 theta_1} sit in registers and serve every (j, k) line, whose
+// factors follow recurrences in e^{i theta_2}, e^{i theta_3}; the weighted structure factors
+// are read at wave-uniform addresses.
 template <class R>
 __global__ __launch_bounds__(256) void k_qp_eval(const double *__restrict__ theta, const double *__restrict__ bx, int n,
                                                   int ms, int mt, int mu, const double2 *__restrict__ S, R *__restrict__ out) {
@@ -1699,27 +1701,39 @@ __global__ __launch_bounds__(256) void k_qp_eval(const double *__restrict__ thet
         sincos(-th * (cnt[d] / 2), &sn, &cs);
         st[d] = make_double2(cs, sn);  // e^{i (-(cnt/2)) th}
     }
-    double2 acc = make_double2(0, 0), c3 = st[2];
-    for (int k = 0; k < mu; ++k) {
-        double2 acc2 = make_double2(0, 0), c2 = st[1];
-        for (int j = 0; j < mt; ++j) {
-            const double2 *row = S + ((size_t)k * mt + j) * ms;
-            double2 acc1 = make_double2(0, 0), c1 = st[0];
-            for (int i = 0; i < ms; ++i) {
-                const double2 v = row[i];
-                acc1.x = fma(v.x, c1.x, fma(-v.y, c1.y, acc1.x));
-                acc1.y = fma(v.x, c1.y, fma(v.y, c1.x, acc1.y));
-                c1 = cmul(c1, e[0]);
-            }
-            const double2 t2 = cmul(acc1, c2);
-            acc2.x += t2.x;
-            acc2.y += t2.y;
-            c2 = cmul(c2, e[1]);
+    double2 acc = make_double2(0, 0), c1next = st[0];
+    for (int i0 = 0; i0 < ms; i0 += QP_CHUNK) {
+        double2 c1[QP_CHUNK];
+#pragma unroll
+        for (int r = 0; r < QP_CHUNK; ++r) {
+            c1[r] = c1next;
+            c1next = cmul(c1next, e[0]);
         }
-        const double2 t3 = cmul(acc2, c3);
-        acc.x += t3.x;
-        acc.y += t3.y;
-        c3 = cmul(c3, e[2]);
+        const int nr = min(QP_CHUNK, ms - i0);
+        double2 c3 = st[2];
+        for (int k = 0; k < mu; ++k) {
+            double2 acc2 = make_double2(0, 0), c2 = st[1];
+            for (int j = 0; j < mt; ++j) {
+                const double2 *row = S + ((size_t)k * mt + j) * ms + i0;
+                double2 l = make_double2(0, 0);
+#pragma unroll
+                for (int r = 0; r < QP_CHUNK; ++r) {
+                    if (r < nr) {
+                        const double2 v = row[r];
+                        l.x = fma(v.x, c1[r].x, fma(-v.y, c1[r].y, l.x));
+                        l.y = fma(v.x, c1[r].y, fma(v.y, c1[r].x, l.y));
+                    }
+                }
+                const double2 t2 = cmul(l, c2);
+                acc2.x += t2.x;
+                acc2.y += t2.y;
+                c2 = cmul(c2, e[1]);
+            }
+            const double2 t3 = cmul(acc2, c3);
+            acc.x += t3.x;
+            acc.y += t3.y;
+            c3 = cmul(c3, e[2]);
+        }
     }
     double sn, cs;
     sincos(bx[u], &sn, &cs);
