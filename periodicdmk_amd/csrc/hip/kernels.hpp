@@ -345,62 +345,68 @@ template <class R, int P, int NPW> struct Pw2pCfg {
 // output planes c' in [slice kc, slice kc + kc), kc <= T3Cfg::KC.
 template <class R, int P> struct T3Cfg {
     static constexpr int PP = P * P, TS = OddStride<P>::v;
-    static constexpr int VW = 16 / (int)sizeof(R);           // matrix entries per 128-bit LDS read
-    static constexpr int PR = (P + VW - 1) / VW * VW;        // padded matrix row
-    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * P * PR * (int)sizeof(R));
+    // output planes per workgroup (the budget keeps 2 p^2 values spare, so FP64 at p = 28
+    // fits three workgroups per CU)
+    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * PP * (int)sizeof(R));
     // at most 4 waves below p = 38: evaluate() runs these beside the near field, whose
     // retiring 4-wave workgroups free 4 wave slots at a time
     static constexpr int threads = PP <= 256 ? (PP + 63) / 64 * 64 : (P >= 38 ? 512 : 256);
+    // the y and x steps have KC P columns of P outputs; when that leaves more than half the
+    // threads idle, each column's outputs are split between SG wavefronts
+    static constexpr int SG = KC * P * 2 <= threads ? 2 : 1;
+    static constexpr int PS = (P + SG - 1) / SG;  // outputs per thread in the y and x steps
+    static constexpr int slots(int nk) { return (nk * P + 63) / 64 * 64 * SG; }  // y / x work slots
 };
+
+// Work slot w of the y and x steps -> (column, segment): consecutive lanes take consecutive
+// columns, and a wavefront's lanes share one segment (so its matrix rows are wave-uniform).
+template <int SG> __device__ __forceinline__ void t3_slot(int w, int &col, int &sg) {
+    col = (w & 63) + 64 * (w / (64 * SG));
+    sg = __builtin_amdgcn_readfirstlane((w >> 6) % SG);  // wave-uniform (w = lane + multiple of 64)
+}
 
 template <class R, int P, bool TRANSPOSE>
 __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
     const R *__restrict__ in_base, R *__restrict__ out_base, const int *__restrict__ item_out,
     const int *__restrict__ item_add, const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
-    const int *__restrict__ in_code, const R *__restrict__ mats, int n_items, int kc) {
+    const int *__restrict__ in_code, const R *__restrict__ mats_rm, int n_items, int kc) {
     chain_priority();
-    constexpr int PP = P * P, TS = T3Cfg<R, P>::TS, KC = T3Cfg<R, P>::KC, NT = T3Cfg<R, P>::threads;
-    constexpr int VW = T3Cfg<R, P>::VW, PR = T3Cfg<R, P>::PR;
-    constexpr int XC = (KC * P + NT - 1) / NT;
-    typedef R vec_t __attribute__((ext_vector_type(VW)));
+    using Cfg = T3Cfg<R, P>;
+    constexpr int PP = P * P, TS = Cfg::TS, KC = Cfg::KC, NT = Cfg::threads, SG = Cfg::SG, PS = Cfg::PS;
+    constexpr int PR = P;  // row stride of the matrices
+    constexpr int XC = (Cfg::slots(KC) + NT - 1) / NT;
     __shared__ R T1[KC * PP];       // [k][b][a]
     __shared__ R T2[KC * P * TS];   // [k][b'][a] rows of TS
-    // U_s(r, c) = M_s[r + P c] (parent -> child) or M_s[c + P r] (child -> parent), stored
-    // row-major with rows padded to PR (zeros), so a row is read VW entries at a time
-    __shared__ __attribute__((aligned(16))) R Ur[2 * P * PR];
+    // U_s(r, c) = M_s[r + P c] (parent -> child) or M_s[c + P r] (child -> parent), read
+    // row-major from mats_rm[(transposed) 2 + s][r][c] at wave-uniform addresses: scalar
+    // loads into SGPRs (a broadcast LDS read of 16 bytes would feed only 2 FP64 / 4 FP32 FMAs
+    // per lane)
     const int it = blockIdx.x;
     if (it >= n_items) return;
     const int k0 = blockIdx.y * kc;
     if (k0 >= P) return;
-    for (int e = threadIdx.x; e < 2 * P * PR; e += NT) {
-        const int sm = e / (P * PR), r = (e / PR) % P, c = e % PR;
-        Ur[e] = c < P ? mats[sm * PP + (TRANSPOSE ? c + P * r : r + P * c)] : R(0);
-    }
-    const int nk = min(kc, P - k0);
+    const R *Ubase = mats_rm + (TRANSPOSE ? 2 : 0) * PP;
+    const int nk = min(kc, P - k0), nslot = Cfg::slots(nk);
     R *out = out_base + (size_t)item_out[it] * P * PP;
     const bool add = item_add[it] != 0;
     const int e0 = item_ptr[it], e1 = item_ptr[it + 1];
     // s += sum_c U(r, c) x[c] for the row of U starting at urow
     auto row_dot = [](const R *urow, const R (&x)[P], R s) {
 #pragma unroll
-        for (int c0 = 0; c0 < PR; c0 += VW) {
-            const vec_t u = *reinterpret_cast<const vec_t *>(urow + c0);
-#pragma unroll
-            for (int v = 0; v < VW; ++v)
-                if (c0 + v < P) s = fma(u[v], x[c0 + v], s);
-        }
+        for (int c = 0; c < P; ++c) s = fma(urow[c], x[c], s);
         return s;
     };
-    // x-step accumulators: thread column (k, b'), outputs a' -- persistent over children
-    R acc[XC][P];
+    // x-step accumulators: work slot (column (k, b'), segment of a') -- persistent over children
+    R acc[XC][PS];
 #pragma unroll
     for (int q = 0; q < XC; ++q)
 #pragma unroll
-        for (int a = 0; a < P; ++a) acc[q][a] = R(0);
+        for (int a = 0; a < PS; ++a) acc[q][a] = R(0);
     for (int e = e0; e < e1; ++e) {
         const R *in = in_base + (size_t)in_slot[e] * P * PP;
         const int code = in_code[e];
-        const R *U0 = Ur + (code & 1) * P * PR, *U1 = Ur + ((code >> 1) & 1) * P * PR, *U2 = Ur + ((code >> 2) & 1) * P * PR;
+        const R *U0 = Ubase + (code & 1) * P * PR, *U1 = Ubase + ((code >> 1) & 1) * P * PR,
+                *U2 = Ubase + ((code >> 2) & 1) * P * PR;
         __syncthreads();
         // z: T1[k][b][a] = sum_c U2(k0 + k, c) in[c][b][a]
         for (int j = threadIdx.x; j < PP; j += NT) {
@@ -410,35 +416,44 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
             for (int k = 0; k < nk; ++k) T1[k * PP + j] = row_dot(U2 + (k0 + k) * PR, x, R(0));
         }
         __syncthreads();
-        // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]
-        for (int w = threadIdx.x; w < nk * P; w += NT) {
-            const int k = w / P, a = w - k * P;
+        // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]; slot = (column (k, a), segment of b')
+        for (int w = threadIdx.x; w < nslot; w += NT) {
+            int col, sg;
+            t3_slot<SG>(w, col, sg);
+            if (col >= nk * P) continue;
+            const int k = col / P, a = col - k * P;
             R x[P];
 #pragma unroll
             for (int b = 0; b < P; ++b) x[b] = T1[k * PP + b * P + a];
-            for (int b2 = 0; b2 < P; ++b2) T2[(k * P + b2) * TS + a] = row_dot(U1 + b2 * PR, x, R(0));
+            for (int b2 = sg * PS; b2 < min(P, sg * PS + PS); ++b2) T2[(k * P + b2) * TS + a] = row_dot(U1 + b2 * PR, x, R(0));
         }
         __syncthreads();
-        // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a], column (k, b')
+        // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a]; slot = (column (k, b'), segment of a')
 #pragma unroll
         for (int q = 0; q < XC; ++q) {
             const int w = threadIdx.x + q * NT;
-            if (w < nk * P) {
+            int col, sg;
+            t3_slot<SG>(w, col, sg);
+            if (w < nslot && col < nk * P) {
                 R x[P];
 #pragma unroll
-                for (int a = 0; a < P; ++a) x[a] = T2[w * TS + a];
+                for (int a = 0; a < P; ++a) x[a] = T2[col * TS + a];
 #pragma unroll
-                for (int a2 = 0; a2 < P; ++a2) acc[q][a2] = row_dot(U0 + a2 * PR, x, acc[q][a2]);
+                for (int i = 0; i < PS; ++i)
+                    if (sg * PS + i < P) acc[q][i] = row_dot(U0 + (sg * PS + i) * PR, x, acc[q][i]);
             }
         }
     }
 #pragma unroll
     for (int q = 0; q < XC; ++q) {
         const int w = threadIdx.x + q * NT;
-        if (w < nk * P) {
-            R *dst = out + (size_t)k0 * PP + (size_t)w * P;  // [c' = k0 + k][b'][a']
+        int col, sg;
+        t3_slot<SG>(w, col, sg);
+        if (w < nslot && col < nk * P) {
+            R *dst = out + (size_t)k0 * PP + (size_t)col * P + sg * PS;  // [c' = k0 + k][b'][a']
 #pragma unroll
-            for (int a2 = 0; a2 < P; ++a2) dst[a2] = add ? dst[a2] + acc[q][a2] : acc[q][a2];
+            for (int i = 0; i < PS; ++i)
+                if (sg * PS + i < P) dst[i] = add ? dst[i] + acc[q][i] : acc[q][i];
         }
     }
 }

@@ -351,10 +351,16 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     }
     c2p_scratch.alloc(sizeof(Real) * P3 * (size_t)std::max(1, c2p_max));
     {
-        std::vector<Real> m(2 * (size_t)p * p);
-        for (int s = 0; s < 2; ++s)
-            for (int i = 0; i < p * p; ++i) m[(size_t)s * p * p + i] = (Real)T.p2c[s][i];
-        mats.upload(m);
+        // parent <-> child matrices, row-major U(r, c) = M_s[r + p c] (parent -> child) and
+        // M_s[c + p r] (child -> parent), M_s = p2c on the minus / plus side
+        std::vector<Real> rm(4 * (size_t)p * p);
+        for (int tr = 0; tr < 2; ++tr)
+            for (int sd = 0; sd < 2; ++sd)
+                for (int r = 0; r < p; ++r)
+                    for (int c = 0; c < p; ++c)
+                        rm[(size_t)(2 * tr + sd) * p * p + (size_t)r * p + c] =
+                            (Real)T.p2c[sd][tr ? c + p * r : r + p * c];
+        mats_rm.upload(rm);
     }
 
     // ---------------- plane-wave tables
@@ -691,7 +697,8 @@ namespace {
 // c2p / p2c of one level: output planes are split into slices so that a level with few
 // boxes still spreads over the whole GPU (about 1024 workgroups or more)
 template <class Real, int P, bool TRANSPOSE>
-void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launch &L, const Real *mats, hipStream_t s) {
+void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launch &L, const Real *mats_rm,
+                    hipStream_t s) {
     if (L.n_items == 0) return;
     constexpr int KC = T3Cfg<Real, P>::KC;
     int kc = KC;
@@ -699,7 +706,7 @@ void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launc
     const dim3 grid(L.n_items, (P + kc - 1) / kc);
     hipLaunchKernelGGL((k_tensor3<Real, P, TRANSPOSE>), grid, dim3(T3Cfg<Real, P>::threads), 0, s, in, out,
                        L.out.template as<int>(), L.add.template as<int>(), L.ptr.template as<int>(),
-                       L.in_slot.template as<int>(), L.in_code.template as<int>(), mats, L.n_items, kc);
+                       L.in_slot.template as<int>(), L.in_code.template as<int>(), mats_rm, L.n_items, kc);
     check(hipGetLastError(), "k_tensor3");
 }
 
@@ -728,7 +735,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     }
     for (auto &C : pl.c2p_levels) {
         launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(), C.children,
-                                      pl.mats.template as<Real>(), s);
+                                      pl.mats_rm.template as<Real>(), s);
         hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256), dim3(256), 0, s,
                            pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(),
                            C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents);
@@ -759,7 +766,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         }
         auto &L = pl.p2c_levels[l];
         launch_tensor3<Real, P, false>(pl.proxy_down.template as<Real>(), pl.proxy_down.template as<Real>(), L,
-                                       pl.mats.template as<Real>(), s);
+                                       pl.mats_rm.template as<Real>(), s);
     }
     if (pl.n_leaf_eval > 0) {
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {

@@ -112,7 +112,7 @@ struct Plan {
     struct C2pLevel { Tensor3Launch children; int n_parents = 0; DevBuf parent_slot, child_ptr; };
     std::vector<C2pLevel> c2p_levels;
     DevBuf c2p_scratch;
-    DevBuf mats;                            // p2c minus / plus (p x p each)
+    DevBuf mats_rm;                         // parent <-> child matrices: [transposed][minus / plus][p][p]
     // plane waves
     DevBuf pw_out, poly2pw, pw2poly, shift0;
     struct EvalLevel { int level, n_items; DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten; };

@@ -119,3 +119,18 @@ def test_qp_deterministic_and_large(pdmk, cuda):
     ab = s.evaluate_complex(0.5 * q1 - 2.0 * q2)
     s.close()
     assert rel(ab, 0.5 * a - 2.0 * b) < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-11), ("float", 1e-5)])
+def test_qp_reciprocal_lattice_bloch_is_periodic(pdmk, cuda, precision, tol):
+    """b = 2 pi e_1 on the unit cube gives the periodic potentials (tests/test_oracle_qp.py)."""
+    cell = W.cubic_cell()
+    pos = W.uniform_positions(3000, 9, cell)
+    q = W.neutral_uniform_charges(3000, 10)
+    a = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=40, precision=precision,
+                        bloch=np.array([2 * np.pi, 0.0, 0.0]))
+    b = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=40, precision=precision)
+    c, p = a.evaluate_complex(q), b.evaluate(q).astype(np.float64)
+    a.close(); b.close()
+    assert np.abs(c.imag).max() < tol * np.abs(p).max()
+    assert rel(c.real, p) < tol

@@ -118,3 +118,19 @@ def test_oracle_qp_stages_add_up(oracle_mod):
     whole = o.evaluate_complex(q, b)
     o.close()
     assert rel(parts, whole) < 1e-13
+
+
+def test_oracle_qp_reciprocal_lattice_bloch_is_periodic(oracle_mod):
+    """b = 2 pi e_1 on the unit cube: every image phase e^{i b.R} is 1 and the far-field
+    wave vectors K = 2 pi (m + e_1) run over the periodic set (K = 0 dropped), so the
+    quasi-periodic potential is the periodic one -- a check of the image phases, the
+    shifted mode box and the e^{-+i b.x} factors together."""
+    cell = W.cubic_cell()
+    pos = W.uniform_positions(60, 9, cell)
+    q = W.neutral_uniform_charges(60, 10)
+    o = oracle_mod.OracleTree(cell, pos, 6, 10, 4, precision="double")
+    c = o.evaluate_complex(q, np.array([2 * np.pi, 0.0, 0.0]))
+    p = o.evaluate(q)
+    o.close()
+    assert np.abs(c.imag).max() < 1e-11 * np.abs(p).max()
+    assert rel(c.real, p) < 1e-11
