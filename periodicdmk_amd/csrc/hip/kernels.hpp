@@ -330,11 +330,23 @@ template <class R, int P, int NPW> struct P2pwCfg {
     static constexpr int H = (NPW + 1) / 2;
     static constexpr int K3 = tf_slices(H, (P * P + NPW * OddStride<P>::v) * (int)sizeof(Cx<R>));
     static constexpr int slices = (H + K3 - 1) / K3;
+    // the y and x steps have K3 P and K3 n columns of n outputs for 256 threads; the outputs
+    // of a column are split over SGY / SGX wavefronts when that fills more of them
+    static constexpr int NT = 256;
+    static constexpr int cap(int v) { return v < 1 ? 1 : (v > 8 ? 8 : v); }
+    static constexpr int SGY = cap(NT / ((K3 * P + 63) / 64 * 64)), SGX = cap(NT / ((K3 * NPW + 63) / 64 * 64));
+    static constexpr int PSY = (NPW + SGY - 1) / SGY, PSX = (NPW + SGX - 1) / SGX;
 };
 template <class R, int P, int NPW> struct Pw2pCfg {
     static constexpr int H = (NPW + 1) / 2;
     static constexpr int AC = tf_slices(P, (H * NPW + H * OddStride<P>::v) * (int)sizeof(Cx<R>));
     static constexpr int slices = (P + AC - 1) / AC;
+    // the y and z steps have AC H and AC P columns of P outputs for 256 threads; the outputs of
+    // a column are split over SGY / SGZ wavefronts when that fills more of them
+    static constexpr int NT = 256;
+    static constexpr int cap(int v) { return v < 1 ? 1 : (v > 8 ? 8 : v); }
+    static constexpr int SGY = cap(NT / ((AC * H + 63) / 64 * 64)), SGZ = cap(NT / ((AC * P + 63) / 64 * 64));
+    static constexpr int PSY = (P + SGY - 1) / SGY, PSZ = (P + SGZ - 1) / SGZ;
 };
 
 // 3-mode tensor transform out(a',b',c') (+)= sum_children U0(a',a) U1(b',b) U2(c',c) in(a,b,c)
@@ -510,13 +522,19 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
             }
         }
         __syncthreads();
-        // y: T2[k][k2][a] = sum_b W(k2, b) T1[k][b][a]
-        for (int w = threadIdx.x; w < nk * P; w += blockDim.x) {
+        // y: T2[k][k2][a] = sum_b W(k2, b) T1[k][b][a]; slot = (column (k, a), segment of k2,
+        // uniform across a wavefront)
+        constexpr int SGY = P2pwCfg<R, P, NPW>::SGY, PSY = P2pwCfg<R, P, NPW>::PSY;
+        const int ny = (nk * P + 63) / 64 * 64 * SGY;
+        for (int t = threadIdx.x; t < ny; t += blockDim.x) {
+            int w, sg;
+            t3_slot<SGY>(t, w, sg);
+            if (w >= nk * P) continue;
             const int k = w / P, a = w - k * P;
             Cx<R> v[P];
 #pragma unroll
             for (int b = 0; b < P; ++b) v[b] = T1[k * PP + b * P + a];
-            for (int k2 = 0; k2 < NPW; ++k2) {
+            for (int k2 = sg * PSY; k2 < min(NPW, sg * PSY + PSY); ++k2) {
                 R sr = 0, si = 0;
 #pragma unroll
                 for (int b = 0; b < P; ++b) {
@@ -528,13 +546,19 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
             }
         }
         __syncthreads();
-        // x: pw[k1][k30 + k][k2] = sum_a W(k1, a) T2[k][k2][a], column w = k n + k2
-        for (int w = threadIdx.x; w < nk * NPW; w += blockDim.x) {
+        // x: pw[k1][k30 + k][k2] = sum_a W(k1, a) T2[k][k2][a]; slot = (column w = k n + k2,
+        // segment of k1, uniform across a wavefront)
+        constexpr int SGX = P2pwCfg<R, P, NPW>::SGX, PSX = P2pwCfg<R, P, NPW>::PSX;
+        const int nx = (nk * NPW + 63) / 64 * 64 * SGX;
+        for (int t = threadIdx.x; t < nx; t += blockDim.x) {
+            int w, sg;
+            t3_slot<SGX>(t, w, sg);
+            if (w >= nk * NPW) continue;
             Cx<R> v[P];
 #pragma unroll
             for (int a = 0; a < P; ++a) v[a] = T2[w * TS + a];
             Cx<R> *dst = pw + (size_t)k30 * NPW + w;
-            for (int k1 = 0; k1 < NPW; ++k1) {
+            for (int k1 = sg * PSX; k1 < min(NPW, sg * PSX + PSX); ++k1) {
                 R sr = 0, si = 0;
 #pragma unroll
                 for (int a = 0; a < P; ++a) {
@@ -614,43 +638,62 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
                 if (a < na) U1[a * COL + col] = acc[a];
         }
         __syncthreads();
-        // y: U2[a][k3][b] = h(k3) sum_k2 V(k2, b) U1[a][k3][k2], column w = a H + k3
-        for (int w = threadIdx.x; w < na * H; w += blockDim.x) {
-            const int k3 = w % H;
-            Cx<R> acc[P];
+        // y: U2[a][k3][b] = h(k3) sum_k2 V(k2, b) U1[a][k3][k2]; slot = (column w = a H + k3,
+        // segment of b, uniform across a wavefront)
+        constexpr int SGY = Pw2pCfg<R, P, NPW>::SGY, PSY = Pw2pCfg<R, P, NPW>::PSY;
+        const int ny = (na * H + 63) / 64 * 64 * SGY;
+        for (int t = threadIdx.x; t < ny; t += blockDim.x) {
+            int w, sg;
+            t3_slot<SGY>(t, w, sg);
+            if (w >= na * H) continue;
+            const int k3 = w % H, b0 = sg * PSY;
+            Cx<R> acc[PSY];
 #pragma unroll
-            for (int b = 0; b < P; ++b) acc[b] = {R(0), R(0)};
+            for (int b = 0; b < PSY; ++b) acc[b] = {R(0), R(0)};
             for (int k2 = 0; k2 < NPW; ++k2) {
                 const Cx<R> x = U1[w * NPW + k2];
 #pragma unroll
-                for (int b = 0; b < P; ++b) {
-                    const Cx<R> m = V[k2 + NPW * b];
-                    acc[b].re = fma(m.re, x.re, fma(-m.im, x.im, acc[b].re));
-                    acc[b].im = fma(m.re, x.im, fma(m.im, x.re, acc[b].im));
+                for (int b = 0; b < PSY; ++b) {
+                    if (b0 + b < P) {
+                        const Cx<R> m = V[k2 + NPW * (b0 + b)];
+                        acc[b].re = fma(m.re, x.re, fma(-m.im, x.im, acc[b].re));
+                        acc[b].im = fma(m.re, x.im, fma(m.im, x.re, acc[b].im));
+                    }
                 }
             }
             const R hw = (k3 >= half) ? R(0.5) : R(1);
 #pragma unroll
-            for (int b = 0; b < P; ++b) U2[w * TS2 + b] = {acc[b].re * hw, acc[b].im * hw};
+            for (int b = 0; b < PSY; ++b)
+                if (b0 + b < P) U2[w * TS2 + b0 + b] = {acc[b].re * hw, acc[b].im * hw};
         }
         __syncthreads();
-        // z: down[c][b][a0 + a] (+)= 2 Re sum_k3 V(k3, c) U2[a][k3][b], column w = b na + a
-        for (int w = threadIdx.x; w < na * P; w += blockDim.x) {
-            const int b = w / na, a = w - b * na;
-            R acc[P];
+        // z: down[c][b][a0 + a] (+)= 2 Re sum_k3 V(k3, c) U2[a][k3][b]; slot = (column w = b na + a,
+        // segment of c, uniform across a wavefront)
+        constexpr int SGZ = Pw2pCfg<R, P, NPW>::SGZ, PSZ = Pw2pCfg<R, P, NPW>::PSZ;
+        const int nz = (na * P + 63) / 64 * 64 * SGZ;
+        for (int t = threadIdx.x; t < nz; t += blockDim.x) {
+            int w, sg;
+            t3_slot<SGZ>(t, w, sg);
+            if (w >= na * P) continue;
+            const int b = w / na, a = w - b * na, c0 = sg * PSZ;
+            R acc[PSZ];
 #pragma unroll
-            for (int c = 0; c < P; ++c) acc[c] = R(0);
+            for (int c = 0; c < PSZ; ++c) acc[c] = R(0);
             for (int k3 = 0; k3 < H; ++k3) {
                 const Cx<R> x = U2[(a * H + k3) * TS2 + b];
 #pragma unroll
-                for (int c = 0; c < P; ++c) {
-                    const Cx<R> m = V[k3 + NPW * c];
-                    acc[c] = fma(m.re, x.re, fma(-m.im, x.im, acc[c]));
+                for (int c = 0; c < PSZ; ++c) {
+                    if (c0 + c < P) {
+                        const Cx<R> m = V[k3 + NPW * (c0 + c)];
+                        acc[c] = fma(m.re, x.re, fma(-m.im, x.im, acc[c]));
+                    }
                 }
             }
             R *dst = down + (size_t)b * P + a0 + a;
 #pragma unroll
-            for (int c = 0; c < P; ++c) dst[(size_t)c * PP] = (zero ? R(0) : dst[(size_t)c * PP]) + R(2) * acc[c];
+            for (int c = 0; c < PSZ; ++c)
+                if (c0 + c < P)
+                    dst[(size_t)(c0 + c) * PP] = (zero ? R(0) : dst[(size_t)(c0 + c) * PP]) + R(2) * acc[c];
         }
         __syncthreads();
     }
