@@ -65,6 +65,16 @@ int pdmk_evaluate_complex_float(void* h, int N, const double* positions_3xN, con
 /// pdmk_gpu_evaluate_stages (15 = everything).
 int pdmk_gpu_evaluate_complex(void* h, const double* d_charges, void* d_pot, int stages);
 
+/// FP64 Ewald potentials at M of the N points only, computed on the GPU: pot_out[t] is the
+/// potential at point target_indices[t] (an index into positions_3xN / charges, which are laid
+/// out as in pdmk_ewald_nufft), equal to pdmk_ewald_nufft's at that point.  Replaces
+/// pdmk::ewald_nufft_targets (cpp/include/pdmk/ewald_nufft.hpp:42-48,
+/// cpp/src/ewald_nufft.cpp:663-759), which the reference has in C++ only: an accuracy
+/// reference at large N.  -1 on a NULL argument, N <= 0, M < 0, s <= 0 or an index outside
+/// [0, N).
+int pdmk_gpu_ewald_targets(const double* cell_3x3, int N, const double* positions_3xN, const double* charges,
+                           int M, const int* target_indices, double s, double* pot_out);
+
 /// Tree and work statistics: out[0..15] = {n_src, nsall, nboxes, n_levels, p, n_pw,
 /// ms, mt, mu, n_leaves, n_list1, n_direct_leaves, n_far_modes, n_digits,
 /// is_float, n_nf_steps}.
@@ -73,9 +83,10 @@ int pdmk_gpu_stats(void* h, long long* out16);
 /// Construction of the handle: out[0..3] = {total ms, tree ms (geometry, images, tree),
 /// far-field plan ms, 1 if the tree's per-point work (images, binning, distribution, sort)
 /// ran on the GPU else 0, boxes folded onto a periodic counterpart (their proxies and
-/// outgoing plane waves are not computed), proxy slots computed}.  No reference counterpart
-/// (measurement hook).
-int pdmk_gpu_build_info(void* h, double* out6);
+/// outgoing plane waves are not computed), proxy slots computed, 1 if the outgoing plane
+/// waves are held one level at a time (when all of them do not fit in device memory) else 0}.
+/// No reference counterpart (measurement hook).
+int pdmk_gpu_build_info(void* h, double* out7);
 
 /// Algorithmic work per evaluate(): out[0..7] = {near-field pairs in support,
 /// near-field candidate pairs, near-field flops (in-support pairs x (2*NC+12)),

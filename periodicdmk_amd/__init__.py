@@ -44,6 +44,7 @@ def lib():
     fp = C.POINTER(C.c_float)
     sig = {
         "pdmk_ewald_nufft": (C.c_int, [dp, C.c_int, dp, dp, C.c_double, dp, dp]),
+        "pdmk_gpu_ewald_targets": (C.c_int, [dp, C.c_int, dp, dp, C.c_int, ip, C.c_double, dp]),
         "pdmk_optimize_cubic_cover": (C.c_int, [dp, C.c_int, ip, ip, ip, dp, dp, dp, dp, dp]),
         "pdmk_create_double": (vp, [dp, C.c_int, dp, C.c_int, C.c_int, C.c_int]),
         "pdmk_evaluate_double": (C.c_int, [vp, C.c_int, dp, dp, dp]),
@@ -147,6 +148,21 @@ def ewald(cell, positions, charges, s: float = 5.0):
     if lib().pdmk_ewald_nufft(_d(cellc), pos.shape[0], _d(pos), _d(q), float(s), _d(pot), C.byref(e)) != 0:
         raise RuntimeError("pdmk_ewald_nufft failed")
     return pot, e.value
+
+
+def ewald_targets(cell, positions, charges, targets, s: float = 5.0):
+    """FP64 Ewald potentials on the GPU at the points ``targets`` (indices into ``positions``)
+    only (pdmk_gpu_ewald_targets); equal to ``ewald(...)[0][targets]``."""
+    cellc = _cell_colmajor(cell)
+    pos = _positions(positions)
+    q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+    if q.shape != (pos.shape[0],):
+        raise ValueError("charges must have shape (N,)")
+    t = np.ascontiguousarray(np.asarray(targets, dtype=np.int32).ravel())
+    pot = np.zeros(t.size)
+    if lib().pdmk_gpu_ewald_targets(_d(cellc), pos.shape[0], _d(pos), _d(q), t.size, _i(t), float(s), _d(pot)) != 0:
+        raise RuntimeError("pdmk_gpu_ewald_targets failed")
+    return pot
 
 
 def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
@@ -302,10 +318,10 @@ class PDMKSolver:
 
     def build_info(self):
         """Construction times (ms) and whether the tree's per-point work ran on the GPU."""
-        o = np.zeros(6)
+        o = np.zeros(7)
         lib().pdmk_gpu_build_info(self._h, _d(o))
         return {"build_ms": float(o[0]), "tree_ms": float(o[1]), "far_plan_ms": float(o[2]), "tree_on_gpu": bool(o[3]),
-                "folded_boxes": int(o[4]), "proxy_slots": int(o[5])}
+                "folded_boxes": int(o[4]), "proxy_slots": int(o[5]), "pw_by_level": bool(o[6])}
 
     def work(self):
         o = np.zeros(8)

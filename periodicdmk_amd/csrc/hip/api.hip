@@ -223,6 +223,19 @@ int pdmk_ewald_nufft(const double *cell, int N, const double *pos, const double 
     }
 }
 
+int pdmk_gpu_ewald_targets(const double *cell, int N, const double *pos, const double *charges, int M,
+                           const int *target_indices, double s, double *pot_out) {
+    try {
+        if (!cell || !pos || !charges || (M > 0 && (!target_indices || !pot_out)))
+            throw std::invalid_argument("null argument");
+        pdmk::gpu::ewald_sum_targets(cell, N, pos, charges, M, target_indices, s, pot_out);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_gpu_ewald_targets: %s\n", e.what());
+        return -1;
+    }
+}
+
 int pdmk_optimize_cubic_cover(const double *cell, int nz, int *m, int *n, int *p, double *e1, double *e2, double *e3,
                               double *s, double *cost) {
     if (nz <= 0) return 1;
@@ -394,7 +407,8 @@ int pdmk_gpu_build_info(void *h, double *o) {
     const bool gpu = f ? tree_of<float>(h)->plan().tree_on_gpu : tree_of<double>(h)->plan().tree_on_gpu;
     const int folded = f ? tree_of<float>(h)->plan().n_folded : tree_of<double>(h)->plan().n_folded;
     const int slots = f ? tree_of<float>(h)->plan().n_up : tree_of<double>(h)->plan().n_up;
-    const double v[6] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0, (double)folded, (double)slots};
+    const bool by_level = f ? tree_of<float>(h)->plan().pw_by_level : tree_of<double>(h)->plan().pw_by_level;
+    const double v[7] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0, (double)folded, (double)slots, by_level ? 1.0 : 0.0};
     std::memcpy(o, v, sizeof(v));
     return 0;
 }

@@ -61,6 +61,41 @@ __global__ __launch_bounds__(EW_THREADS) void k_ewald_real(const double4 *__rest
     if (active) out[t] = acc;
 }
 
+// Real-space sum at one target per workgroup (ewald_nufft_targets): the workgroup's threads
+// stride over the sources of the target bin's neighbour entries; the partial sums are added
+// in a fixed order (lanes by xor shuffles, then the wavefronts in index order).
+__global__ __launch_bounds__(EW_THREADS) void k_ewald_real_targets(
+    const double4 *__restrict__ pts, const int *__restrict__ bin_start, const int *__restrict__ nb_ptr,
+    const int *__restrict__ nb_bin, const double4 *__restrict__ nb_shift, const int *__restrict__ trg_sorted,
+    const int *__restrict__ trg_bin, double alpha, double rc2, double *__restrict__ out) {
+    __shared__ double part[EW_THREADS / 64];
+    const int t = blockIdx.x, b = trg_bin[t];
+    const double4 me = pts[trg_sorted[t]];
+    double acc = 0;
+    for (int e = nb_ptr[b]; e < nb_ptr[b + 1]; ++e) {
+        const int sb = nb_bin[e];
+        const double4 sh = nb_shift[e];
+        const double tx = me.x - sh.x, ty = me.y - sh.y, tz = me.z - sh.z;
+        for (int j = bin_start[sb] + (int)threadIdx.x; j < bin_start[sb + 1]; j += EW_THREADS) {
+            const double4 q = pts[j];
+            const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (r2 < rc2 && r2 > 1e-30) {
+                const double r = sqrt(r2);
+                acc += q.w * erfc(alpha * r) / r;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0;
+        for (int w = 0; w < EW_THREADS / 64; ++w) v += part[w];
+        out[t] = v;
+    }
+}
+
 // phi[u] = real[sorted(u)] + far[u] - 2 alpha/sqrt(pi) q_u + shift
 __global__ void k_ewald_finalize(const double *__restrict__ real_sorted, const int *__restrict__ sorted_of,
                                  const double *__restrict__ far, const double *__restrict__ q, double self, double shift,
@@ -113,16 +148,12 @@ double ewald_gpu_alpha(int n, double vol) {
     return std::pow(500.0 * n / (vol * vol), 1.0 / 6.0);
 }
 
-void ewald_sum(const double *cell, int n, const double *pos, const double *q, double s, double *pot, double *energy) {
-    if (n <= 0) throw std::invalid_argument("N must be positive");
-    if (!(s > 0)) throw std::invalid_argument("s must be positive");
-    const Lat L = make_lat(cell);
-    const double alpha_ref = ewald_reference_alpha(n, L.vol, s);
-    const double alpha = ewald_gpu_alpha(n, L.vol);
-    const double rcut = s / alpha, kmax = 2 * s * alpha;
+namespace {
 
-    // wrapped fractional and Cartesian coordinates
-    std::vector<double> frac(3 * (size_t)n), theta(3 * (size_t)n);
+// Positions wrapped into the cell: fractional coordinates in [0, 1) and theta = 2 pi frac.
+void wrap_points(const Lat &L, int n, const double *pos, std::vector<double> &frac, std::vector<double> &theta) {
+    frac.resize(3 * (size_t)n);
+    theta.resize(3 * (size_t)n);
     for (int i = 0; i < n; ++i)
         for (int d = 0; d < 3; ++d) {
             double f = L.inv[d][0] * pos[3 * (size_t)i] + L.inv[d][1] * pos[3 * (size_t)i + 1] +
@@ -132,106 +163,153 @@ void ewald_sum(const double *cell, int n, const double *pos, const double *q, do
             frac[3 * (size_t)i + d] = f;
             theta[3 * (size_t)i + d] = 2 * M_PI * f;
         }
+}
 
-    // ---- real space: bins of side >= r_cut along each height, counting sort
-    int nb[3], nimg[3];
+// Real-space bins of side >= r_cut along each height (nimg images per axis when one bin
+// spans the cell): points counting-sorted by bin, and per bin the (source bin, image shift)
+// entries with offsets -nimg..nimg per axis, wrapped.
+struct Bins {
+    int nb[3], nimg[3], nbins = 0, max_occ = 0;
+    std::vector<int> bin_of, bstart, sorted_of, nptr, nbin;
+    std::vector<double4> pts, nshift;
+};
+
+void bin_shape(const Lat &L, double rcut, int nb[3], int nimg[3]) {
     for (int d = 0; d < 3; ++d) {
         nb[d] = std::max(1, (int)std::floor(L.h[d] / rcut));
         nimg[d] = nb[d] == 1 ? (int)std::ceil(rcut / L.h[d]) : 1;
     }
-    const int nbins = nb[0] * nb[1] * nb[2];
-    std::vector<int> bin_of(n), bstart(nbins + 1, 0), sorted_of(n);
-    for (int i = 0; i < n; ++i) {
-        int c[3];
-        for (int d = 0; d < 3; ++d) c[d] = std::min(nb[d] - 1, (int)(frac[3 * (size_t)i + d] * nb[d]));
-        bin_of[i] = (c[2] * nb[1] + c[1]) * nb[0] + c[0];
-        bstart[bin_of[i] + 1]++;
-    }
-    for (int b = 0; b < nbins; ++b) bstart[b + 1] += bstart[b];
-    std::vector<double4> pts(n);
-    {
-        std::vector<int> fill(bstart.begin(), bstart.end() - 1);
-        for (int i = 0; i < n; ++i) {
-            const int k = fill[bin_of[i]]++;
-            sorted_of[i] = k;
-            double x[3];
-            for (int dd = 0; dd < 3; ++dd)
-                x[dd] = L.a[0][dd] * frac[3 * (size_t)i] + L.a[1][dd] * frac[3 * (size_t)i + 1] +
-                        L.a[2][dd] * frac[3 * (size_t)i + 2];
-            pts[k] = make_double4(x[0], x[1], x[2], q[i]);
-        }
-    }
-    // neighbour (bin, image shift) entries: offsets -nimg..nimg per axis, wrapped
-    std::vector<int> nptr{0}, nbin;
-    std::vector<double4> nshift;
-    int max_occ = 0;
-    for (int b = 0; b < nbins; ++b) {
-        max_occ = std::max(max_occ, bstart[b + 1] - bstart[b]);
-        const int c[3] = {b % nb[0], (b / nb[0]) % nb[1], b / (nb[0] * nb[1])};
-        for (int oz = -nimg[2]; oz <= nimg[2]; ++oz)
-            for (int oy = -nimg[1]; oy <= nimg[1]; ++oy)
-                for (int ox = -nimg[0]; ox <= nimg[0]; ++ox) {
-                    const int o[3] = {ox, oy, oz};
-                    int cc[3], sh[3];
-                    for (int d = 0; d < 3; ++d) {
-                        cc[d] = c[d] + o[d];
-                        sh[d] = 0;
-                        while (cc[d] < 0) { cc[d] += nb[d]; --sh[d]; }
-                        while (cc[d] >= nb[d]) { cc[d] -= nb[d]; ++sh[d]; }
-                    }
-                    const int sb = (cc[2] * nb[1] + cc[1]) * nb[0] + cc[0];
-                    if (bstart[sb + 1] == bstart[sb]) continue;
-                    double v[3];
-                    for (int d = 0; d < 3; ++d) v[d] = sh[0] * L.a[0][d] + sh[1] * L.a[1][d] + sh[2] * L.a[2][d];
-                    nbin.push_back(sb);
-                    nshift.push_back(make_double4(v[0], v[1], v[2], 0));
-                }
-        nptr.push_back((int)nbin.size());
-    }
+}
 
-    // ---- reciprocal: k = 2 pi A^{-T} m, |k| <= k_max, m != 0; half space with weight 2 w
+int bin_index(const int nb[3], const double *f) {
+    int c[3];
+    for (int d = 0; d < 3; ++d) c[d] = std::min(nb[d] - 1, (int)(f[d] * nb[d]));
+    return (c[2] * nb[1] + c[1]) * nb[0] + c[0];
+}
+
+// The entries of bin b: calls visit(source bin, shift d0, d1, d2) in a fixed order.
+template <class F>
+void for_each_neighbour(const Lat &L, const int nb[3], const int nimg[3], int b, F &&visit) {
+    const int c[3] = {b % nb[0], (b / nb[0]) % nb[1], b / (nb[0] * nb[1])};
+    for (int oz = -nimg[2]; oz <= nimg[2]; ++oz)
+        for (int oy = -nimg[1]; oy <= nimg[1]; ++oy)
+            for (int ox = -nimg[0]; ox <= nimg[0]; ++ox) {
+                const int o[3] = {ox, oy, oz};
+                int cc[3], sh[3];
+                for (int d = 0; d < 3; ++d) {
+                    cc[d] = c[d] + o[d];
+                    sh[d] = 0;
+                    while (cc[d] < 0) { cc[d] += nb[d]; --sh[d]; }
+                    while (cc[d] >= nb[d]) { cc[d] -= nb[d]; ++sh[d]; }
+                }
+                double v[3];
+                for (int d = 0; d < 3; ++d) v[d] = sh[0] * L.a[0][d] + sh[1] * L.a[1][d] + sh[2] * L.a[2][d];
+                visit((cc[2] * nb[1] + cc[1]) * nb[0] + cc[0], v);
+            }
+}
+
+Bins make_bins(const Lat &L, const std::vector<double> &frac, const double *q, int n, double rcut) {
+    Bins B;
+    bin_shape(L, rcut, B.nb, B.nimg);
+    B.nbins = B.nb[0] * B.nb[1] * B.nb[2];
+    B.bin_of.resize(n);
+    B.bstart.assign(B.nbins + 1, 0);
+    B.sorted_of.resize(n);
+    for (int i = 0; i < n; ++i) {
+        B.bin_of[i] = bin_index(B.nb, &frac[3 * (size_t)i]);
+        B.bstart[B.bin_of[i] + 1]++;
+    }
+    for (int b = 0; b < B.nbins; ++b) B.bstart[b + 1] += B.bstart[b];
+    B.pts.resize(n);
+    std::vector<int> fill(B.bstart.begin(), B.bstart.end() - 1);
+    for (int i = 0; i < n; ++i) {
+        const int k = fill[B.bin_of[i]]++;
+        B.sorted_of[i] = k;
+        double x[3];
+        for (int dd = 0; dd < 3; ++dd)
+            x[dd] = L.a[0][dd] * frac[3 * (size_t)i] + L.a[1][dd] * frac[3 * (size_t)i + 1] +
+                    L.a[2][dd] * frac[3 * (size_t)i + 2];
+        B.pts[k] = make_double4(x[0], x[1], x[2], q[i]);
+    }
+    B.nptr.push_back(0);
+    for (int b = 0; b < B.nbins; ++b) {
+        B.max_occ = std::max(B.max_occ, B.bstart[b + 1] - B.bstart[b]);
+        for_each_neighbour(L, B.nb, B.nimg, b, [&](int sb, const double *v) {
+            if (B.bstart[sb + 1] == B.bstart[sb]) return;
+            B.nbin.push_back(sb);
+            B.nshift.push_back(make_double4(v[0], v[1], v[2], 0));
+        });
+        B.nptr.push_back((int)B.nbin.size());
+    }
+    return B;
+}
+
+// Reciprocal modes k = 2 pi A^{-T} m, 0 < |k| <= k_max, in the half space (k > 0, or k = 0
+// and j > 0, or k = j = 0 and i > 0) with weight 2 (4 pi/V) e^{-k^2/4 alpha^2}/k^2.
+struct Modes {
     int mmax[3];
-    for (int d = 0; d < 3; ++d) mmax[d] = (int)std::ceil(kmax * L.h[d] / (2 * M_PI));
     std::vector<int> mijk;
     std::vector<double> mw;
+};
+
+Modes make_modes(const Lat &L, double alpha, double kmax) {
+    Modes M;
+    for (int d = 0; d < 3; ++d) M.mmax[d] = (int)std::ceil(kmax * L.h[d] / (2 * M_PI));
     const double pref = 4 * M_PI / L.vol, inv4a2 = 1 / (4 * alpha * alpha), kmax2 = kmax * kmax;
-    for (int k = 0; k <= mmax[2]; ++k)
-        for (int j = (k == 0 ? 0 : -mmax[1]); j <= mmax[1]; ++j)
-            for (int i = ((k == 0 && j == 0) ? 1 : -mmax[0]); i <= mmax[0]; ++i) {
+    for (int k = 0; k <= M.mmax[2]; ++k)
+        for (int j = (k == 0 ? 0 : -M.mmax[1]); j <= M.mmax[1]; ++j)
+            for (int i = ((k == 0 && j == 0) ? 1 : -M.mmax[0]); i <= M.mmax[0]; ++i) {
                 double kv[3];
                 for (int d = 0; d < 3; ++d) kv[d] = 2 * M_PI * (i * L.inv[0][d] + j * L.inv[1][d] + k * L.inv[2][d]);
                 const double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
                 if (k2 > kmax2) continue;
-                mijk.push_back(i);
-                mijk.push_back(j);
-                mijk.push_back(k);
-                mw.push_back(2 * pref * std::exp(-k2 * inv4a2) / k2);
+                M.mijk.push_back(i);
+                M.mijk.push_back(j);
+                M.mijk.push_back(k);
+                M.mw.push_back(2 * pref * std::exp(-k2 * inv4a2) / k2);
             }
+    return M;
+}
 
-    // ---- device work
-    hipStream_t st = nullptr;
-    check(hipStreamCreate(&st), "hipStreamCreate");
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() {
-            if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
-        }
-    } guard{st};
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    StreamGuard() { check(hipStreamCreate(&s), "hipStreamCreate"); }
+    ~StreamGuard() {
+        if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+    }
+};
+
+}  // namespace
+
+void ewald_sum(const double *cell, int n, const double *pos, const double *q, double s, double *pot, double *energy) {
+    if (n <= 0) throw std::invalid_argument("N must be positive");
+    if (!(s > 0)) throw std::invalid_argument("s must be positive");
+    const Lat L = make_lat(cell);
+    const double alpha_ref = ewald_reference_alpha(n, L.vol, s);
+    const double alpha = ewald_gpu_alpha(n, L.vol);
+    const double rcut = s / alpha, kmax = 2 * s * alpha;
+    std::vector<double> frac, theta;
+    wrap_points(L, n, pos, frac, theta);
+    const Bins B = make_bins(L, frac, q, n, rcut);
+    const Modes M = make_modes(L, alpha, kmax);
+
+    StreamGuard guard;
+    const hipStream_t st = guard.s;
     DevBuf d_pts, d_bstart, d_nptr, d_nbin, d_nshift, d_real, d_far, d_q, d_sorted, d_phi;
-    d_pts.upload(pts);
-    d_bstart.upload(bstart);
-    d_nptr.upload(nptr);
-    d_nbin.upload(nbin);
-    d_nshift.upload(nshift);
-    d_sorted.upload(sorted_of);
+    d_pts.upload(B.pts);
+    d_bstart.upload(B.bstart);
+    d_nptr.upload(B.nptr);
+    d_nbin.upload(B.nbin);
+    d_nshift.upload(B.nshift);
+    d_sorted.upload(B.sorted_of);
     d_q.upload(std::vector<double>(q, q + n));
     d_real.alloc(sizeof(double) * (size_t)n);
     d_far.alloc(sizeof(double) * (size_t)n);
     d_phi.alloc(sizeof(double) * (size_t)n);
     FourierSum fs;
-    fs.setup(theta, n, 2 * mmax[0] + 1, 2 * mmax[1] + 1, 2 * mmax[2] + 1, mijk, mw);
+    fs.setup(theta, n, 2 * M.mmax[0] + 1, 2 * M.mmax[1] + 1, 2 * M.mmax[2] + 1, M.mijk, M.mw);
     fs.run(d_q.as<double>(), d_far.as<double>(), st);
-    const dim3 grid(nbins, (max_occ + EW_THREADS - 1) / EW_THREADS);
+    const dim3 grid(B.nbins, (B.max_occ + EW_THREADS - 1) / EW_THREADS);
     hipLaunchKernelGGL(k_ewald_real, grid, dim3(EW_THREADS), 0, st, d_pts.as<double4>(), d_bstart.as<int>(),
                        d_nptr.as<int>(), d_nbin.as<int>(), d_nshift.as<double4>(), alpha, rcut * rcut,
                        d_real.as<double>());
@@ -249,6 +327,93 @@ void ewald_sum(const double *cell, int n, const double *pos, const double *q, do
         for (int i = 0; i < n; ++i) e += q[i] * pot[i];
         *energy = 0.5 * e - M_PI * Q * Q / (2 * L.vol * alpha_ref * alpha_ref);
     }
+}
+
+double ewald_targets_alpha(const double *cell, int n, const double *pos, int m, const int *targets, double s) {
+    const Lat L = make_lat(cell);
+    std::vector<double> frac, theta;
+    wrap_points(L, n, pos, frac, theta);
+    double best = 0, best_cost = 0;
+    const double hmin = std::min(L.h[0], std::min(L.h[1], L.h[2]));
+    std::vector<long long> cnt;
+    for (int div = 1; div <= 32; ++div) {
+        const double rcut = hmin / div, alpha = s / rcut, kmax = 2 * s * alpha;
+        int nb[3], nimg[3];
+        bin_shape(L, rcut, nb, nimg);
+        const long long nbins = (long long)nb[0] * nb[1] * nb[2];
+        if (nbins > 4 * (long long)n + 64) break;
+        cnt.assign(nbins, 0);
+        for (int i = 0; i < n; ++i) cnt[bin_index(nb, &frac[3 * (size_t)i])]++;
+        // points the target workgroups visit, and (point, mode) products of the structure
+        // factors and the evaluation (half-space mode count from the volume of the k-ball)
+        double visits = 0;
+        for (int t = 0; t < m; ++t)
+            for_each_neighbour(L, nb, nimg, bin_index(nb, &frac[3 * (size_t)targets[t]]),
+                               [&](int sb, const double *) { visits += (double)cnt[sb]; });
+        const double modes = 0.5 * (4.0 / 3.0) * M_PI * kmax * kmax * kmax * L.vol / std::pow(2 * M_PI, 3);
+        const double cost = visits + 2.0 * modes * ((double)n + m);
+        if (best == 0 || cost < best_cost) {
+            best = alpha;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+void ewald_sum_targets(const double *cell, int n, const double *pos, const double *q, int m, const int *targets,
+                       double s, double *pot) {
+    if (n <= 0) throw std::invalid_argument("N must be positive");
+    if (m < 0) throw std::invalid_argument("M must be non-negative");
+    if (!(s > 0)) throw std::invalid_argument("s must be positive");
+    for (int t = 0; t < m; ++t)
+        if (targets[t] < 0 || targets[t] >= n) throw std::invalid_argument("target index out of range");
+    if (m == 0) return;
+    const Lat L = make_lat(cell);
+    const double alpha_ref = ewald_reference_alpha(n, L.vol, s);
+    const double alpha = ewald_targets_alpha(cell, n, pos, m, targets, s);
+    const double rcut = s / alpha, kmax = 2 * s * alpha;
+    std::vector<double> frac, theta;
+    wrap_points(L, n, pos, frac, theta);
+    const Bins B = make_bins(L, frac, q, n, rcut);
+    const Modes M = make_modes(L, alpha, kmax);
+    std::vector<int> t_sorted(m), t_bin(m);
+    std::vector<double> t_theta(3 * (size_t)m);
+    for (int t = 0; t < m; ++t) {
+        t_sorted[t] = B.sorted_of[targets[t]];
+        t_bin[t] = B.bin_of[targets[t]];
+        for (int d = 0; d < 3; ++d) t_theta[3 * (size_t)t + d] = theta[3 * (size_t)targets[t] + d];
+    }
+
+    StreamGuard guard;
+    const hipStream_t st = guard.s;
+    DevBuf d_pts, d_bstart, d_nptr, d_nbin, d_nshift, d_tsorted, d_tbin, d_ttheta, d_q, d_real, d_far;
+    d_pts.upload(B.pts);
+    d_bstart.upload(B.bstart);
+    d_nptr.upload(B.nptr);
+    d_nbin.upload(B.nbin);
+    d_nshift.upload(B.nshift);
+    d_tsorted.upload(t_sorted);
+    d_tbin.upload(t_bin);
+    d_ttheta.upload(t_theta);
+    d_q.upload(std::vector<double>(q, q + n));
+    d_real.alloc(sizeof(double) * (size_t)m);
+    d_far.alloc(sizeof(double) * (size_t)m);
+    FourierSum fs;
+    fs.setup(theta, n, 2 * M.mmax[0] + 1, 2 * M.mmax[1] + 1, 2 * M.mmax[2] + 1, M.mijk, M.mw);
+    fs.run_at(d_q.as<double>(), d_ttheta.as<double>(), m, d_far.as<double>(), st);
+    hipLaunchKernelGGL(k_ewald_real_targets, dim3(m), dim3(EW_THREADS), 0, st, d_pts.as<double4>(), d_bstart.as<int>(),
+                       d_nptr.as<int>(), d_nbin.as<int>(), d_nshift.as<double4>(), d_tsorted.as<int>(),
+                       d_tbin.as<int>(), alpha, rcut * rcut, d_real.as<double>());
+    check(hipGetLastError(), "k_ewald_real_targets");
+    std::vector<double> real(m), far(m);
+    check(hipMemcpyAsync(real.data(), d_real.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, st), "D2H");
+    check(hipMemcpyAsync(far.data(), d_far.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, st), "D2H");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    double Q = 0;
+    for (int i = 0; i < n; ++i) Q += q[i];
+    const double shift = M_PI * Q / L.vol * (1 / (alpha_ref * alpha_ref) - 1 / (alpha * alpha));
+    const double self = 2 * alpha / std::sqrt(M_PI);
+    for (int t = 0; t < m; ++t) pot[t] = real[t] + far[t] - self * q[targets[t]] + shift;
 }
 
 }  // namespace pdmk::gpu

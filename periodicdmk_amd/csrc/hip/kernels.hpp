@@ -489,10 +489,11 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // ------------------------------------------------------------------ plane waves
 // Plane-wave mode layout: m = k1 * (H n) + k3 * n + k2 ([x][z][y], y fastest), k3 < H.
 // Outgoing expansion pw(k1,k2,k3) = sum W(k1,a) W(k2,b) W(k3,c) C[c][b][a], W = poly2pw
-// (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3).
+// (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3).  Workgroup x reads proxy slot
+// slot0 + x and writes plane-wave slot x.
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
-                                                   const Cx<R> *__restrict__ W, int n_boxes) {
+                                                   const Cx<R> *__restrict__ W, int slot0, int n_boxes) {
     chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, TS = OddStride<P>::v;
     constexpr int K3 = P2pwCfg<R, P, NPW>::K3;
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
     __shared__ Cx<R> T2[K3 * NPW * TS];  // [k][k2][a] rows of TS
     const int bx = blockIdx.x;
     if (bx >= n_boxes) return;
-    const R *C = proxy + (size_t)bx * P * PP;
+    const R *C = proxy + (size_t)(slot0 + bx) * P * PP;
     Cx<R> *pw = pw_out + (size_t)bx * NPW * COL;
     {
         const int k30 = blockIdx.y * K3;  // this workgroup's slice of k3
@@ -572,14 +573,14 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
     }
 }
 
-// Incoming expansion of one level: pin[i][m] = lscale sum_e pw_out[slot_e][m] shift0[sten_e][m]
+// Incoming expansion of one level: pin[i][m] = lscale sum_e pw_out[slot_e - slot0][m] shift0[sten_e][m]
 // over box i's CSR range of (neighbour slot, stencil); shift0 holds the level-0 translation
 // times the radial Fourier factor, lscale = 2^level scales the latter to this level.
 template <class R>
 __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_out, const Cx<R> *__restrict__ shift0,
                                                    R lscale, const int *__restrict__ nbr_ptr,
                                                    const int *__restrict__ nbr_slot, const int *__restrict__ nbr_sten,
-                                                   int nm, int item0, Cx<R> *__restrict__ pin) {
+                                                   int slot0, int nm, int item0, Cx<R> *__restrict__ pin) {
     chain_priority();
     // x = target box (consecutive workgroups share neighbours; each XCD takes a contiguous,
     // spatially coherent range of boxes), y = mode chunk: the neighbours' chunk is read from
@@ -589,7 +590,7 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
     if (m >= nm) return;
     R sr = 0, si = 0;
     for (int e = nbr_ptr[it]; e < nbr_ptr[it + 1]; ++e) {
-        const Cx<R> a = pw_out[(size_t)nbr_slot[e] * nm + m];
+        const Cx<R> a = pw_out[(size_t)(nbr_slot[e] - slot0) * nm + m];
         const Cx<R> s = shift0[(size_t)nbr_sten[e] * nm + m];
         sr = fma(a.re, s.re, fma(-a.im, s.im, sr));
         si = fma(a.re, s.im, fma(a.im, s.re, si));

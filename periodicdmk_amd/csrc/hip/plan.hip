@@ -253,18 +253,23 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
 
     // ---------------- proxy slots
     // up_of: the slot a box's proxy and outgoing plane waves are computed in (-1 for folded
-    // boxes); up_src: the slot they are read from (the counterpart's for folded boxes)
-    std::vector<int> up_of(nboxes, -1), up_src(nboxes, -1), down_of(nboxes, -1);
-    for (int b = 0; b < nboxes; ++b) {
-        if (t.pwexp[b] && t.src_count[b] > 0 && fold[b] < 0) up_of[b] = n_up++;
+    // boxes), numbered level by level (box order within a level) so that the slots one level's
+    // shifts read form one contiguous range; up_src: the slot they are read from (the
+    // counterpart's, on the same level, for folded boxes)
+    std::vector<int> up_of(nboxes, -1), up_src(nboxes, -1), down_of(nboxes, -1), up_box;
+    for (int l = 0; l < nlevels; ++l)
+        for (int b : t.by_level[l])
+            if (t.pwexp[b] && t.src_count[b] > 0 && fold[b] < 0) {
+                up_of[b] = n_up++;
+                up_box.push_back(b);
+            }
+    for (int b = 0; b < nboxes; ++b)
         if ((t.pwexp[b] || t.tpeval[b]) && t.src_count[b] + t.trg_count[b] > 0) down_of[b] = n_down++;
-    }
     for (int b = 0; b < nboxes; ++b) up_src[b] = fold[b] < 0 ? up_of[b] : up_of[fold[b]];
     const size_t P3 = (size_t)p * p * p;
     const int H = (n_pw + 1) / 2, NM = n_pw * n_pw * H;
     proxy_up.alloc(sizeof(Real) * P3 * std::max(1, n_up));
     proxy_down.alloc(sizeof(Real) * P3 * std::max(1, n_down));
-    pw_out.alloc(2 * sizeof(Real) * (size_t)NM * std::max(1, n_up));
 
     // ---------------- upward: one charge->proxy group per proxy slot (possibly without sources)
     {
@@ -272,9 +277,8 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         for (const auto &gr : t.up_groups) grp_of[gr.box] = &gr;
         std::vector<int> slot(n_up), rng(n_up + 1, 0), rb, rcnt;
         std::vector<Real> geom(4 * (size_t)n_up), roff;
-        for (int b = 0; b < nboxes; ++b) {
-            const int k = up_of[b];
-            if (k < 0) continue;
+        for (int k = 0; k < n_up; ++k) {
+            const int b = up_box[k];
             slot[k] = k;
             for (int d = 0; d < 3; ++d) geom[4 * k + d] = (Real)t.center[3 * b + d];
             geom[4 * k + 3] = (Real)(2.0 / t.boxsize[t.level[b]]);
@@ -296,7 +300,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 }
             }
         }
-        // ranges were appended in box order == slot order
+        // ranges were appended in slot order
         for (int k = 0; k < n_up; ++k) rng[k + 1] += rng[k];
         up_slot.upload(slot);
         up_geom.upload(geom);
@@ -349,7 +353,6 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         c2p_max = std::max(c2p_max, (int)in.size());
         c2p_levels.push_back(std::move(C));
     }
-    c2p_scratch.alloc(sizeof(Real) * P3 * (size_t)std::max(1, c2p_max));
     {
         // parent <-> child matrices, row-major U(r, c) = M_s[r + p c] (parent -> child) and
         // M_s[c + p r] (child -> parent), M_s = p2c on the minus / plus side
@@ -412,6 +415,11 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         EvalLevel E;
         E.level = l;
         E.n_items = (int)down.size();
+        if (!nslot.empty()) {
+            const auto mm = std::minmax_element(nslot.begin(), nslot.end());
+            E.pw_lo = *mm.first;
+            E.pw_cnt = *mm.second - *mm.first + 1;
+        }
         E.down.upload(down);
         E.zero.upload(zero);
         E.nbr_ptr.upload(nptr);
@@ -654,6 +662,26 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         pin_batch = std::max(1, std::min(items, (int)(kPinBytes / ((size_t)NM * 2 * sizeof(Real)))));
         pin.alloc((size_t)pin_batch * NM * 2 * sizeof(Real));
     }
+    // ---------------- outgoing plane waves and the child -> parent scratch: all up slots when
+    // they fit in the free device memory (less 1 GiB), else one level's slot range at a time
+    // (PDMK_PW_LEVELS=1 / 0 forces either)
+    {
+        const size_t pw_slot = 2 * sizeof(Real) * (size_t)NM;
+        const size_t c2p_bytes = sizeof(Real) * P3 * (size_t)std::max(1, c2p_max);
+        int lvl_cnt = 1;
+        for (auto &E : eval_levels) lvl_cnt = std::max(lvl_cnt, E.pw_cnt);
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const size_t all_bytes = pw_slot * (size_t)std::max(1, n_up) + c2p_bytes;
+        pw_by_level = all_bytes + (size_t(1) << 30) > free_b;
+        if (const char *e = std::getenv("PDMK_PW_LEVELS")) pw_by_level = e[0] == '1';
+        if (pw_by_level) {
+            pw_out.alloc(std::max(pw_slot * (size_t)lvl_cnt, c2p_bytes));
+        } else {
+            pw_out.alloc(pw_slot * (size_t)std::max(1, n_up));
+            c2p_scratch.alloc(c2p_bytes);
+        }
+    }
 
     // ---------------- statistics and exact near-field pair count
     stats.n_src = n; stats.nsall = nsall; stats.nboxes = nboxes; stats.n_levels = nlevels; stats.p = p;
@@ -734,28 +762,31 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         }
     }
     for (auto &C : pl.c2p_levels) {
-        launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(), C.children,
+        launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_buffer(), C.children,
                                       pl.mats_rm.template as<Real>(), s);
         hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256), dim3(256), 0, s,
-                           pl.proxy_up.template as<Real>(), pl.c2p_scratch.template as<Real>(),
+                           pl.proxy_up.template as<Real>(), pl.c2p_buffer(),
                            C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents);
         check(hipGetLastError(), "k_sum_children");
     }
-    if (pl.n_up > 0) {
-        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(pl.n_up, P2pwCfg<Real, P, NPW>::slices), dim3(256), 0, s,
-                           pl.proxy_up.template as<Real>(),
-                           pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(), pl.n_up);
+    auto proxy2pw = [&](int slot0, int count) {
+        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(count, P2pwCfg<Real, P, NPW>::slices), dim3(256), 0, s,
+                           pl.proxy_up.template as<Real>(), pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(),
+                           slot0, count);
         check(hipGetLastError(), "k_proxy2pw");
-    }
+    };
+    if (pl.n_up > 0 && !pl.pw_by_level) proxy2pw(0, pl.n_up);
     if (pl.e_upward) check(hipEventRecord(pl.e_upward, s), "hipEventRecord");
     for (size_t l = 0; l < pl.eval_levels.size(); ++l) {
         auto &E = pl.eval_levels[l];
         const Real lscale = (Real)std::ldexp(1.0, E.level);
+        if (pl.pw_by_level && E.pw_cnt > 0) proxy2pw(E.pw_lo, E.pw_cnt);
+        const int slot0 = pl.pw_by_level ? E.pw_lo : 0;
         for (int i0 = 0; i0 < E.n_items; i0 += pl.pin_batch) {
             const int nb = std::min(pl.pin_batch, E.n_items - i0);
             hipLaunchKernelGGL(k_pw_shift<Real>, dim3(nb, (NM + 255) / 256), dim3(256), 0, s, pl.pw_out.template as<C>(),
                                pl.shift0.template as<C>(), lscale, E.nbr_ptr.template as<int>(),
-                               E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), NM, i0,
+                               E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0,
                                pl.pin.template as<C>());
             check(hipGetLastError(), "k_pw_shift");
             hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb, Pw2pCfg<Real, P, NPW>::slices), dim3(256), 0, s,
@@ -1065,9 +1096,13 @@ void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int 
 }
 
 void FourierSum::run(const double *d_q, double *d_out, hipStream_t s) const {
-    if (n == 0) return;
-    if (n_modes == 0) {
-        hipLaunchKernelGGL(k_fill<double>, dim3((n + 255) / 256), dim3(256), 0, s, d_out, 0.0, n);
+    run_at(d_q, theta.as<double>(), n, d_out, s);
+}
+
+void FourierSum::run_at(const double *d_q, const double *d_theta_eval, int n_eval, double *d_out, hipStream_t s) const {
+    if (n_eval <= 0) return;
+    if (n == 0 || n_modes == 0) {
+        hipLaunchKernelGGL(k_fill<double>, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_out, 0.0, n_eval);
         check(hipGetLastError(), "k_fill far");
         return;
     }
@@ -1080,7 +1115,7 @@ void FourierSum::run(const double *d_q, double *d_out, hipStream_t s) const {
     hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32), dim3(256), 0, s, Sp.as<double2>(), blocks, w.as<double>(),
                        pidx.as<int>(), n_modes, S.as<double2>());
     check(hipGetLastError(), "k_ff_reduce");
-    hipLaunchKernelGGL(k_ff_eval, dim3((n + 255) / 256), dim3(256), 0, s, theta.as<double>(), n, lines.as<int4>(),
+    hipLaunchKernelGGL(k_ff_eval, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_theta_eval, n_eval, lines.as<int4>(),
                        line_base.as<int>(), n_lines, S.as<double2>(), d_out);
     check(hipGetLastError(), "k_ff_eval");
 }

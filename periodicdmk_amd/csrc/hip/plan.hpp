@@ -59,6 +59,9 @@ struct FourierSum {
     void setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
                const std::vector<int> &mode_ijk, const std::vector<double> &mode_w);
     void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
+    // S(m) from the setup points, then out[t] at n_eval other points (theta_eval[3t + d],
+    // device, same convention); enqueue only
+    void run_at(const double *d_q, const double *d_theta_eval, int n_eval, double *d_out, hipStream_t s) const;
 };
 
 // The same Fourier sum by a non-uniform FFT (spreading with an exponential-of-semicircle
@@ -112,10 +115,17 @@ struct Plan {
     struct C2pLevel { Tensor3Launch children; int n_parents = 0; DevBuf parent_slot, child_ptr; };
     std::vector<C2pLevel> c2p_levels;
     DevBuf c2p_scratch;
+    // Outgoing plane waves of all up slots (pw_out[slot]), or, when that does not fit in the
+    // device memory, of one level's range at a time (pw_out[slot - pw_lo], formed just before
+    // the level's shifts); c2p_scratch then shares pw_out's memory (the upward pass is done
+    // before the first plane wave is formed, on the same stream).
+    bool pw_by_level = false;
+    Real *c2p_buffer() { return pw_by_level ? pw_out.template as<Real>() : c2p_scratch.template as<Real>(); }
     DevBuf mats_rm;                         // parent <-> child matrices: [transposed][minus / plus][p][p]
     // plane waves
     DevBuf pw_out, poly2pw, pw2poly, shift0;
-    struct EvalLevel { int level, n_items; DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten; };
+    // pw_lo, pw_cnt: the range of up slots the level's shifts read
+    struct EvalLevel { int level, n_items, pw_lo = 0, pw_cnt = 0; DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten; };
     std::vector<EvalLevel> eval_levels;
     std::vector<Tensor3Launch> p2c_levels;  // one per level (may be empty)
     DevBuf proxy_down;

@@ -59,6 +59,53 @@ def test_larger_system_vs_cell_ewald(pdmk, oracle_mod, cuda):
     assert rel(pot[tg], ref) < 1e-10
 
 
+@pytest.mark.parametrize("name", ["triclinic", "non_neutral", "clustered"])
+def test_targets_equal_full_sum(pdmk, cuda, name):
+    """pdmk_gpu_ewald_targets (ewald_nufft_targets, cpp/src/ewald_nufft.cpp:663-759) returns
+    pdmk_ewald_nufft's potentials at the chosen points, although it picks its own splitting
+    parameter (the sum does not depend on it)."""
+    n = 6000
+    cell = W.triclinic_cell()
+    pos = (W.gaussian_mixture(n, 8, 0.03, 7, cell) if name == "clustered" else W.uniform_positions(n, 5, cell))
+    q = W.neutral_uniform_charges(n, 6) + (0.01 if name == "non_neutral" else 0.0)
+    tg = np.random.default_rng(2).choice(n, 150, replace=False).astype(np.int32)
+    tg[-1] = tg[0]  # a repeated target
+    full, _ = pdmk.ewald(cell, pos, q, 5.5)
+    sub = pdmk.ewald_targets(cell, pos, q, tg, 5.5)
+    assert rel(sub, full[tg]) < 1e-11, rel(sub, full[tg])
+    assert sub[-1] == sub[0]
+
+
+def test_targets_vs_cell_ewald_clustered(pdmk, oracle_mod, cuda):
+    """At N = 2e5 clustered points the subset sum agrees with the oracle's cell-list Ewald."""
+    n = 200_000
+    cell = W.triclinic_cell()
+    pos = W.gaussian_mixture(n, 64, 0.02, 21, cell)
+    q = W.pm1_charges(n, 22)
+    tg = np.sort(np.random.default_rng(3).choice(n, 40, replace=False)).astype(np.int32)
+    # smallest splitting parameter the oracle allows: few reciprocal modes over the 2e5 charges
+    ref = oracle_mod.ewald_cells(cell, pos, q, s=5.5, targets=tg, alpha=1e-9)
+    assert rel(pdmk.ewald_targets(cell, pos, q, tg, 5.5), ref) < 1e-10
+
+
+def test_targets_madelung(pdmk, cuda):
+    pos = np.array([[0, 0, 0], [.5, .5, 0], [.5, 0, .5], [0, .5, .5], [.5, 0, 0], [0, .5, 0], [0, 0, .5], [.5, .5, .5]])
+    q = np.array([1, 1, 1, 1, -1, -1, -1, -1.0])
+    pot = pdmk.ewald_targets(np.eye(3), pos, q, [0, 7], 5.0)
+    np.testing.assert_allclose(-pot * q[[0, 7]] / 2, M_REF, rtol=1e-8)
+
+
+def test_targets_errors(pdmk, cuda):
+    pos = np.random.default_rng(0).random((10, 3))
+    q = np.ones(10)
+    assert pdmk.ewald_targets(np.eye(3), pos, q, np.zeros(0, dtype=np.int32)).shape == (0,)
+    for bad in ([10], [-1]):
+        with pytest.raises(RuntimeError):
+            pdmk.ewald_targets(np.eye(3), pos, q, bad)
+    with pytest.raises(RuntimeError):
+        pdmk.ewald_targets(np.eye(3), pos, q, [0], s=0.0)
+
+
 def test_errors(pdmk, cuda):
     pos = np.zeros((4, 3))
     q = np.ones(4)

@@ -1,0 +1,102 @@
+"""The largest workload and the memory layout it needs.
+
+The clustered triclinic workload: N = 4,000,000 points in 64 isotropic Gaussian clusters
+(standard deviation 0.02, centres uniform in fractional coordinates, splitmix64 seed 21), cell
+columns a1=(1,0,0), a2=(0.3,1,0), a3=(0.2,0.25,1), charges +/-1 in equal numbers (seed 22),
+n_per_leaf = 300, nz = 4.  Its tree has 333,928 boxes and 124,287 proxy slots; at 12 digits
+the outgoing plane waves of all slots alone would take 168 GB, so the plan holds them one
+level at a time.  The potentials are compared with the FP64 Ewald sum at 1000 sampled points
+(pdmk_gpu_ewald_targets, pinned against the oracle's Ewald sums in test_gpu_ewald.py), to
+10^(2 - n_digits) in relative L2 norm; the reference's own end-to-end tests allow 1e-4 at 6
+digits (cpp/tests/test_periodic_dmk.cpp:42, 67).
+
+The per-level layout runs the same kernels on the same values, so on smaller inputs it must
+give exactly the potentials of the all-slots layout (PDMK_PW_LEVELS forces either).
+"""
+import numpy as np
+import pytest
+
+from periodicdmk_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def build(pdmk, monkeypatch, cell, pos, digits, npl, precision, by_level, bloch=None):
+    monkeypatch.setenv("PDMK_PW_LEVELS", "1" if by_level else "0")
+    try:
+        s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=npl, nz=4, precision=precision, bloch=bloch)
+    finally:
+        monkeypatch.delenv("PDMK_PW_LEVELS")
+    assert s.build_info()["pw_by_level"] == by_level
+    return s
+
+
+SMALL = {
+    "cubic_6_double": (W.cubic_cell(), W.uniform_positions(20_000, 3, W.cubic_cell()), 40, 6, "double"),
+    "clustered_9_double": (W.triclinic_cell(), W.gaussian_mixture(30_000, 8, 0.02, 21, W.triclinic_cell()), 30, 9,
+                           "double"),
+    "clustered_6_float": (W.triclinic_cell(), W.gaussian_mixture(30_000, 8, 0.02, 21, W.triclinic_cell()), 30, 6,
+                          "float"),
+}
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_plane_waves_by_level_identical(pdmk, cuda, monkeypatch, name):
+    cell, pos, npl, digits, precision = SMALL[name]
+    q = W.neutral_uniform_charges(pos.shape[0], 8)
+    out = []
+    for by_level in (False, True):
+        s = build(pdmk, monkeypatch, cell, pos, digits, npl, precision, by_level)
+        out.append(s.evaluate(q))
+        s.close()
+    np.testing.assert_array_equal(out[1], out[0])
+
+
+def test_plane_waves_by_level_identical_quasi_periodic(pdmk, cuda, monkeypatch):
+    cell = W.triclinic_cell()
+    pos = W.gaussian_mixture(20_000, 8, 0.03, 4, cell)
+    q = W.neutral_uniform_charges(pos.shape[0], 9)
+    out = []
+    for by_level in (False, True):
+        s = build(pdmk, monkeypatch, cell, pos, 6, 40, "double", by_level, bloch=(0.3, -0.2, 0.1))
+        out.append(s.evaluate_complex(q))
+        s.close()
+    np.testing.assert_array_equal(out[1], out[0])
+
+
+@pytest.fixture(scope="module")
+def clustered_4m():
+    n = 4_000_000
+    cell = W.triclinic_cell()
+    pos = W.gaussian_mixture(n, 64, 0.02, 21, cell)
+    q = W.pm1_charges(n, 22)
+    tg = np.sort(np.random.default_rng(5).choice(n, 1000, replace=False)).astype(np.int32)
+    return cell, pos, q, tg
+
+
+@pytest.fixture(scope="module")
+def clustered_4m_ewald(pdmk, clustered_4m):
+    cell, pos, q, tg = clustered_4m
+    return pdmk.ewald_targets(cell, pos, q, tg, 5.5)
+
+
+@pytest.mark.parametrize("digits,precision", [(6, "float"), (9, "double"), (12, "double")])
+def test_clustered_triclinic_4m(pdmk, cuda, clustered_4m, clustered_4m_ewald, digits, precision):
+    cell, pos, q, tg = clustered_4m
+    s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=300, nz=4, precision=precision)
+    try:
+        st = s.stats()
+        assert (st["nsall"], st["nboxes"], s.build_info()["proxy_slots"]) == (23_423_948, 333_928, 124_287)
+        pot = s.evaluate(q).astype(np.float64)
+        err = rel(pot[tg], clustered_4m_ewald)
+        print(f"N=4e6 clustered triclinic, {digits} digits {precision}: rel-L2 vs Ewald {err:.2e}, "
+              f"plane waves by level: {s.build_info()['pw_by_level']}", flush=True)
+        assert err < 10.0 ** (2 - digits), err
+        # the same charges again give the same potentials
+        np.testing.assert_array_equal(s.evaluate(q).astype(np.float64), pot)
+    finally:
+        s.close()
