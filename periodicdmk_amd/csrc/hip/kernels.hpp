@@ -318,9 +318,17 @@ constexpr int kTfLdsBytes = 64 * 1024;
 
 template <int N> struct OddStride { static constexpr int v = (N % 2) ? N : N + 1; };
 
+// LDS budget of a tensor-transform workgroup: 64 KiB, or all 160 KiB of a CU for FP64 at
+// p >= 38 (12 digits), whose registers allow one such workgroup per CU anyway: 6 output planes
+// per workgroup instead of 1, so each input is read 7 times instead of 38 (measured on the
+// N=4e6 clustered workload: child -> parent 431 -> 128 ms, parent -> child 162 -> 73 ms per
+// evaluate).  The proxy <-> plane-wave transforms keep 64 KiB: with 160 KiB they hold one
+// wavefront per SIMD and ran 1.7x slower.
+constexpr int tf_lds_budget(int p, int real_bytes) { return p >= 38 && real_bytes == 8 ? 160 * 1024 : kTfLdsBytes; }
+
 // slices per pass so that `reserved` + `per_slice` bytes x slices stays within the LDS budget
-constexpr int tf_slices(int total, int per_slice, int reserved = 0) {
-    const int k = (kTfLdsBytes - reserved) / per_slice;
+constexpr int tf_slices(int total, int per_slice, int reserved = 0, int budget = kTfLdsBytes) {
+    const int k = (budget - reserved) / per_slice;
     return k < 1 ? 1 : (k > total ? total : k);
 }
 
@@ -354,12 +362,15 @@ template <class R, int P, int NPW> struct Pw2pCfg {
 // by bit d of the child code, transposed for child -> parent.  Item i writes out slot
 // item_out[i] from the CSR range item_ptr[i]..item_ptr[i+1] of (in slot, child code);
 // item_add[i] = accumulate into the existing output.  Workgroup (item, slice) computes the
-// output planes c' in [slice kc, slice kc + kc), kc <= T3Cfg::KC.
+// output planes c' in [slice kc, slice kc + kc), kc <= T3Cfg::KC; the grid is 1-D, workgroup
+// (item, slice) at xcd_block index item * slices + slice, so an item's slices run together
+// under one L2 and its input is fetched from memory about once.
 template <class R, int P> struct T3Cfg {
     static constexpr int PP = P * P, TS = OddStride<P>::v;
     // output planes per workgroup (the budget keeps 2 p^2 values spare, so FP64 at p = 28
     // fits three workgroups per CU)
-    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * PP * (int)sizeof(R));
+    static constexpr int KC = tf_slices(P, (PP + P * TS) * (int)sizeof(R), 2 * PP * (int)sizeof(R),
+                                        tf_lds_budget(P, (int)sizeof(R)));
     // at most 4 waves below p = 38: evaluate() runs these beside the near field, whose
     // retiring 4-wave workgroups free 4 wave slots at a time
     static constexpr int threads = PP <= 256 ? (PP + 63) / 64 * 64 : (P >= 38 ? 512 : 256);
@@ -393,10 +404,10 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
     // row-major from mats_rm[(transposed) 2 + s][r][c] at wave-uniform addresses: scalar
     // loads into SGPRs (a broadcast LDS read of 16 bytes would feed only 2 FP64 / 4 FP32 FMAs
     // per lane)
-    const int it = blockIdx.x;
+    const int ns = (P + kc - 1) / kc, lg = xcd_block(blockIdx.x, gridDim.x);
+    const int it = lg / ns;
     if (it >= n_items) return;
-    const int k0 = blockIdx.y * kc;
-    if (k0 >= P) return;
+    const int k0 = (lg - it * ns) * kc;
     const R *Ubase = mats_rm + (TRANSPOSE ? 2 : 0) * PP;
     const int nk = min(kc, P - k0), nslot = Cfg::slots(nk);
     R *out = out_base + (size_t)item_out[it] * P * PP;
@@ -489,8 +500,9 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // ------------------------------------------------------------------ plane waves
 // Plane-wave mode layout: m = k1 * (H n) + k3 * n + k2 ([x][z][y], y fastest), k3 < H.
 // Outgoing expansion pw(k1,k2,k3) = sum W(k1,a) W(k2,b) W(k3,c) C[c][b][a], W = poly2pw
-// (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3).  Workgroup x reads proxy slot
-// slot0 + x and writes plane-wave slot x.
+// (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3), 1-D grid, (box, slice) at
+// xcd_block index box * slices + slice (a box's slices share one L2).  Box x reads proxy
+// slot slot0 + x and writes plane-wave slot x.
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
                                                    const Cx<R> *__restrict__ W, int slot0, int n_boxes) {
@@ -499,12 +511,13 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
     constexpr int K3 = P2pwCfg<R, P, NPW>::K3;
     __shared__ Cx<R> T1[K3 * PP];        // [k][b][a]
     __shared__ Cx<R> T2[K3 * NPW * TS];  // [k][k2][a] rows of TS
-    const int bx = blockIdx.x;
+    constexpr int NS = P2pwCfg<R, P, NPW>::slices;
+    const int lg = xcd_block(blockIdx.x, gridDim.x), bx = lg / NS, slice = lg - bx * NS;
     if (bx >= n_boxes) return;
     const R *C = proxy + (size_t)(slot0 + bx) * P * PP;
     Cx<R> *pw = pw_out + (size_t)bx * NPW * COL;
     {
-        const int k30 = blockIdx.y * K3;  // this workgroup's slice of k3
+        const int k30 = slice * K3;  // this workgroup's slice of k3
         const int nk = min(K3, H - k30);
         // z: T1[k][b][a] = sum_c W(k30 + k, c) C[c][b][a]
         for (int j = threadIdx.x; j < PP; j += blockDim.x) {
@@ -599,8 +612,8 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
 }
 
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
-// (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  Grid x =
-// box, grid y = slices of a (strided when grid y has fewer workgroups than slices).
+// (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  One
+// workgroup per (box, slice of a).
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
@@ -610,13 +623,15 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
     constexpr int AC = Pw2pCfg<R, P, NPW>::AC;
     __shared__ Cx<R> U1[AC * COL];       // [a][k3][k2]
     __shared__ Cx<R> U2[AC * H * TS2];   // [a][k3][b] rows of TS2
-    const int it = blockIdx.x;
+    // 1-D grid: (box, slice of a) at xcd_block index box * slices + slice
+    constexpr int NS = Pw2pCfg<R, P, NPW>::slices;
+    const int lg = xcd_block(blockIdx.x, gridDim.x), it = lg / NS, slice = lg - it * NS;
     if (it >= n_items) return;
     const Cx<R> *X = pin + (size_t)it * NPW * COL;
     R *down = down_base + (size_t)box_down[item0 + it] * P * PP;
     const bool zero = box_zero[item0 + it] != 0;
-    // slices of a, strided over the workgroups of grid y
-    for (int a0 = blockIdx.y * AC; a0 < P; a0 += gridDim.y * AC) {
+    {
+        const int a0 = slice * AC;
         const int na = min(AC, P - a0);
         // x: U1[a][k3][k2] = sum_k1 V(k1, a0 + a) pin[k1][k3][k2], column = (k3, k2)
         for (int col = threadIdx.x; col < COL; col += blockDim.x) {

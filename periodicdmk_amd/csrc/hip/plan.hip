@@ -731,7 +731,7 @@ void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launc
     constexpr int KC = T3Cfg<Real, P>::KC;
     int kc = KC;
     while (kc > 1 && (long)L.n_items * ((P + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
-    const dim3 grid(L.n_items, (P + kc - 1) / kc);
+    const dim3 grid(L.n_items * ((P + kc - 1) / kc));  // (item, slice) pairs
     hipLaunchKernelGGL((k_tensor3<Real, P, TRANSPOSE>), grid, dim3(T3Cfg<Real, P>::threads), 0, s, in, out,
                        L.out.template as<int>(), L.add.template as<int>(), L.ptr.template as<int>(),
                        L.in_slot.template as<int>(), L.in_code.template as<int>(), mats_rm, L.n_items, kc);
@@ -770,7 +770,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         check(hipGetLastError(), "k_sum_children");
     }
     auto proxy2pw = [&](int slot0, int count) {
-        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(count, P2pwCfg<Real, P, NPW>::slices), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(count * P2pwCfg<Real, P, NPW>::slices), dim3(256), 0, s,
                            pl.proxy_up.template as<Real>(), pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(),
                            slot0, count);
         check(hipGetLastError(), "k_proxy2pw");
@@ -789,7 +789,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
                                E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0,
                                pl.pin.template as<C>());
             check(hipGetLastError(), "k_pw_shift");
-            hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb, Pw2pCfg<Real, P, NPW>::slices), dim3(256), 0, s,
+            hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb * Pw2pCfg<Real, P, NPW>::slices), dim3(256), 0, s,
                                pl.pin.template as<C>(),
                                pl.pw2poly.template as<C>(), E.down.template as<int>(), E.zero.template as<int>(), i0, nb,
                                pl.proxy_down.template as<Real>());

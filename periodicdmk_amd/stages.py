@@ -3,12 +3,16 @@ stage on its own (no other stream competing for the GPU):
 
     python -m periodicdmk_amd.stages --paths mixed6,fp64_9 --stages 1 --calls 3
     python -m periodicdmk_amd.stages --paths fp64_6 --stages 15 --calls 4 --bloch 2,-1,0.5
+    python -m periodicdmk_amd.stages --workload clustered --n 4000000 --paths mixed6,fp64_9,fp64_12 --stages 15
 
 Stage bits: 1 proxy / plane-wave chain, 2 near field, 4 self correction, 8 far field.
 With --bloch the trees are quasi-periodic and the calls are evaluate_complex (device
 pointers).  Prints the mean wall time per call after the first (synchronized).
-Workload: cubic L=1, N uniform positions (splitmix64 seed 11), neutral U[-1,1] charges
-(seed 100), n_per_leaf 300, nz 4 -- the same points as bench.py.
+Workloads (n_per_leaf 300, nz 4):
+  uniform   -- cubic L=1, N uniform positions (splitmix64 seed 11), neutral U[-1,1] charges
+               (seed 100): the same points as bench.py;
+  clustered -- cell columns (1,0,0), (0.3,1,0), (0.2,0.25,1), N points in 64 Gaussian
+               clusters (standard deviation 0.02, seed 21), +/-1 charges (seed 22).
 """
 import argparse
 import sys
@@ -17,7 +21,7 @@ import time
 import periodicdmk_amd as P
 from periodicdmk_amd import workloads as W
 
-PATHS = {"mixed6": ("float", 6), "fp64_6": ("double", 6), "fp64_9": ("double", 9)}
+PATHS = {"mixed6": ("float", 6), "fp64_6": ("double", 6), "fp64_9": ("double", 9), "fp64_12": ("double", 12)}
 
 
 def main():
@@ -27,13 +31,19 @@ def main():
     ap.add_argument("--calls", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--bloch", default="", help="bx,by,bz: quasi-periodic trees")
+    ap.add_argument("--workload", default="uniform", choices=("uniform", "clustered"))
     args = ap.parse_args()
     if P.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    cell = W.cubic_cell()
-    pos = W.uniform_positions(args.n, 11, cell)
-    q = P.DeviceArray.from_numpy(W.neutral_uniform_charges(args.n, 100))
+    if args.workload == "uniform":
+        cell = W.cubic_cell()
+        pos = W.uniform_positions(args.n, 11, cell)
+        q = P.DeviceArray.from_numpy(W.neutral_uniform_charges(args.n, 100))
+    else:
+        cell = W.triclinic_cell()
+        pos = W.gaussian_mixture(args.n, 64, 0.02, 21, cell)
+        q = P.DeviceArray.from_numpy(W.pm1_charges(args.n, 22))
     for name in [p for p in args.paths.split(",") if p]:
         prec, nd = PATHS[name]
         bloch = [float(v) for v in args.bloch.split(",")] if args.bloch else None
@@ -51,8 +61,10 @@ def main():
             times.append(time.perf_counter() - t0)
         ms = 1e3 * sum(times[1:]) / max(1, len(times) - 1)
         kind = "evaluate_complex" if bloch else "evaluate_stages"
-        print(f"[stages] {name}: {args.calls} calls of {kind} stages {args.stages}, {ms:.3f} ms per call after the first",
-              flush=True)
+        info = s.build_info()
+        print(f"[stages] {name}: {args.calls} calls of {kind} stages {args.stages}, {ms:.3f} ms per call after the first"
+              f" (build {info['build_ms']:.0f} ms, {info['proxy_slots']} proxy slots, plane waves by level: "
+              f"{info['pw_by_level']})", flush=True)
         out.free()
         s.close()
     q.free()
