@@ -34,6 +34,22 @@ int pdmk_gpu_n_points(void* h);
 int pdmk_gpu_evaluate_double(void* h, const double* d_charges, double* d_pot, void* stream);
 int pdmk_gpu_evaluate_float(void* h, const double* d_charges, float* d_pot, void* stream);
 
+/// A batch of evaluate() calls on device pointers captured once into a HIP graph and then
+/// replayed with one launch, for many small trees (say 256 cells x 32 charge vectors) whose
+/// evaluate() is a few dozen short kernels each.  No reference counterpart: the reference
+/// runs trees one after another on the CPU.  Call i evaluates handles[i] on d_charges[i]
+/// (FP64, N of that handle) into d_pots[i] (the handle's precision); the calls of one handle
+/// keep their order, handles are dealt to n_branches concurrent branches.  The buffers are
+/// bound at creation: a launch reads the charges they hold then.  One handle's work must not
+/// overlap another use of that handle (its scratch is shared).  create returns NULL on a
+/// NULL or quasi-periodic handle, a NULL buffer, n_calls or n_branches <= 0 or a HIP error;
+/// launch enqueues on `stream` (NULL = the graph's own stream) and returns; destroy waits for
+/// the graph's own stream and frees it.
+void* pdmk_gpu_graph_create(int n_calls, void* const* handles, const double* const* d_charges, void* const* d_pots,
+                            int n_branches);
+int pdmk_gpu_graph_launch(void* g, void* stream);
+int pdmk_gpu_graph_destroy(void* g);
+
 /// evaluate_timed() on device pointers: runs synchronously and fills
 /// times_ms[0..2] = {nufft_ms, dmk_ms, direct_ms} with the same stage grouping as
 /// EvalTiming (cpp/include/pdmk/eval_timing.hpp:10-19,

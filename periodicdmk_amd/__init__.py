@@ -75,6 +75,9 @@ def lib():
         "pdmk_evaluate_complex_float": (C.c_int, [vp, C.c_int, dp, dp, C.POINTER(C.c_float)]),
         "pdmk_gpu_evaluate_complex": (C.c_int, [vp, vp, vp, C.c_int]),
         "pdmk_gpu_build_info": (C.c_int, [vp, dp]),
+        "pdmk_gpu_graph_create": (vp, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
+        "pdmk_gpu_graph_launch": (C.c_int, [vp, vp]),
+        "pdmk_gpu_graph_destroy": (C.c_int, [vp]),
         "pdmk_campaign_sha256": (C.c_int, [vp, C.c_size_t, vp]),
         "pdmk_campaign_write": (C.c_int, [C.c_char_p, C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_ulonglong, dp,
                                           C.c_uint, C.POINTER(C.c_uint), dp, dp, dp]),
@@ -163,6 +166,38 @@ def ewald_targets(cell, positions, charges, targets, s: float = 5.0):
     if lib().pdmk_gpu_ewald_targets(_d(cellc), pos.shape[0], _d(pos), _d(q), t.size, _i(t), float(s), _d(pot)) != 0:
         raise RuntimeError("pdmk_gpu_ewald_targets failed")
     return pot
+
+
+class BatchGraph:
+    """evaluate() calls on device buffers captured into one HIP graph (pdmk_gpu_graph_create).
+
+    ``calls`` is a list of (solver, charges DeviceArray, potentials DeviceArray); the buffers
+    are bound at creation and each ``launch()`` reads the charges they hold then."""
+
+    def __init__(self, calls, n_branches: int = 8):
+        n = len(calls)
+        H = (C.c_void_p * max(1, n))(*[c[0]._h for c in calls])
+        Q = (C.c_void_p * max(1, n))(*[c[1].ptr for c in calls])
+        O = (C.c_void_p * max(1, n))(*[c[2].ptr for c in calls])
+        self._g = lib().pdmk_gpu_graph_create(n, H, Q, O, int(n_branches))
+        if not self._g:
+            raise RuntimeError("pdmk_gpu_graph_create failed")
+
+    def launch(self, stream: int = 0) -> None:
+        """Enqueue one replay (on the graph's own stream unless ``stream`` is given)."""
+        if lib().pdmk_gpu_graph_launch(self._g, C.c_void_p(stream) if stream else None) != 0:
+            raise RuntimeError("pdmk_gpu_graph_launch failed")
+
+    def close(self) -> None:
+        if getattr(self, "_g", None):
+            lib().pdmk_gpu_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:

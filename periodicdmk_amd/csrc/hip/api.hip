@@ -16,8 +16,11 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <optional>
 #include <stdexcept>
+#include <unordered_map>
+#include <vector>
 
 namespace pdmk {
 
@@ -197,6 +200,70 @@ void destroy_impl(void *h) {
     delete H;
 }
 
+// A captured list of evaluate() calls on device pointers, replayed as one HIP graph.  The
+// calls of one tree keep their order on one branch stream; trees are dealt to the branches
+// in order of first appearance, and the branches run concurrently.
+struct BatchGraph {
+    hipStream_t origin = nullptr;
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> ev;  // ev[0]: fork; ev[1 + k]: join of branch k
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchGraph() {
+        if (origin) (void)hipStreamSynchronize(origin);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipStream_t b : branch) (void)hipStreamDestroy(b);
+        if (origin) (void)hipStreamDestroy(origin);
+    }
+};
+
+BatchGraph *make_batch_graph(int n_calls, void *const *handles, const double *const *d_charges, void *const *d_pots,
+                             int n_branches) {
+    using pdmk::gpu::check;
+    if (n_calls <= 0 || n_branches <= 0 || !handles || !d_charges || !d_pots)
+        throw std::invalid_argument("need n_calls > 0, n_branches > 0 and the three arrays");
+    for (int i = 0; i < n_calls; ++i) {
+        auto *H = static_cast<Handle *>(handles[i]);
+        if (!H || !d_charges[i] || !d_pots[i]) throw std::invalid_argument("null handle or buffer in call list");
+        const bool qp = H->is_float ? tree_of<float>(H)->plan().qp : tree_of<double>(H)->plan().qp;
+        if (qp) throw std::invalid_argument("quasi-periodic handle in call list");
+    }
+    auto G = std::make_unique<BatchGraph>();
+    check(hipStreamCreate(&G->origin), "hipStreamCreate");
+    G->branch.assign(n_branches, nullptr);
+    for (auto &b : G->branch) check(hipStreamCreate(&b), "hipStreamCreate");
+    G->ev.assign(n_branches + 1, nullptr);
+    for (auto &e : G->ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    std::unordered_map<void *, int> branch_of;
+    check(hipStreamBeginCapture(G->origin, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        check(hipEventRecord(G->ev[0], G->origin), "hipEventRecord");
+        for (hipStream_t b : G->branch) check(hipStreamWaitEvent(b, G->ev[0], 0), "hipStreamWaitEvent");
+        for (int i = 0; i < n_calls; ++i) {
+            auto *H = static_cast<Handle *>(handles[i]);
+            const int k = branch_of.emplace(handles[i], (int)branch_of.size() % n_branches).first->second;
+            if (H->is_float)
+                tree_of<float>(H)->evaluate_device(d_charges[i], static_cast<float *>(d_pots[i]), G->branch[k]);
+            else
+                tree_of<double>(H)->evaluate_device(d_charges[i], static_cast<double *>(d_pots[i]), G->branch[k]);
+        }
+        for (int k = 0; k < n_branches; ++k) {
+            check(hipEventRecord(G->ev[1 + k], G->branch[k]), "hipEventRecord");
+            check(hipStreamWaitEvent(G->origin, G->ev[1 + k], 0), "hipStreamWaitEvent");
+        }
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(G->origin, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check(hipStreamEndCapture(G->origin, &G->graph), "hipStreamEndCapture");
+    check(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    return G.release();
+}
+
 template <typename F>
 int guarded(const char *what, F &&f) {
     try {
@@ -364,6 +431,32 @@ int pdmk_gpu_evaluate_float(void *h, const double *d_charges, float *d_pot, void
         tree_of<float>(h)->evaluate_device(d_charges, d_pot, stream);
         return 0;
     });
+}
+
+void *pdmk_gpu_graph_create(int n_calls, void *const *handles, const double *const *d_charges, void *const *d_pots,
+                            int n_branches) {
+    try {
+        return make_batch_graph(n_calls, handles, d_charges, d_pots, n_branches);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_gpu_graph_create: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+int pdmk_gpu_graph_launch(void *g, void *stream) {
+    if (!g) return -1;
+    return guarded("pdmk_gpu_graph_launch", [&] {
+        auto *G = static_cast<BatchGraph *>(g);
+        pdmk::gpu::check(hipGraphLaunch(G->exec, stream ? static_cast<hipStream_t>(stream) : G->origin),
+                         "hipGraphLaunch");
+        return 0;
+    });
+}
+
+int pdmk_gpu_graph_destroy(void *g) {
+    if (!g) return -1;
+    delete static_cast<BatchGraph *>(g);
+    return 0;
 }
 
 int pdmk_gpu_evaluate_timed(void *h, const double *d_charges, void *d_pot, double *times_ms) {

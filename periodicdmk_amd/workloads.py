@@ -8,7 +8,8 @@ Workloads (cubic cells have L = 1):
   * ``neutral_uniform_charges(n, seed)``  -- i.i.d. U[-1,1], mean subtracted;
   * ``gaussian_mixture(...)``             -- isotropic Gaussian clusters, wrapped
     periodically in FP64 fractional coordinates;
-  * ``pm1_charges(n, seed)``              -- +/-1 with equal counts, shuffled.
+  * ``pm1_charges(n, seed)``              -- +/-1 with equal counts, shuffled;
+  * ``batch_cells(...)``                  -- the batched workload's cells and point counts.
 """
 from __future__ import annotations
 
@@ -78,3 +79,19 @@ def gaussian_mixture(n: int, n_clusters: int, sigma: float, seed: int, cell=None
     frac -= np.floor(frac)
     frac[frac >= 1.0] = 0.0
     return frac @ cell.T
+
+
+def batch_cells(n_cells: int = 256, seed: int = 31, n_min: int = 2_000, n_max: int = 200_000):
+    """Cells and point counts of the batched workload (many small trees): cell i is cubic with
+    L = 1 when u0 < 1/2, else triclinic with columns (1,0,0), (t1,1,0), (t2,t3,1), t ~ U[0, 0.3];
+    N_i is log-uniform in [n_min, n_max].  Points of cell i: ``uniform_positions(N_i, 1000 + i,
+    cell)``; charge vector v of cell i: ``neutral_uniform_charges(N_i, 100000 + 32 i + v)``."""
+    u = uniform01(seed, 5 * n_cells).reshape(n_cells, 5)
+    out = []
+    for i in range(n_cells):
+        cell = cubic_cell()
+        if u[i, 0] >= 0.5:
+            cell[0, 1], cell[0, 2], cell[1, 2] = 0.3 * u[i, 1], 0.3 * u[i, 2], 0.3 * u[i, 3]
+        n = int(round(np.exp(np.log(n_min) + u[i, 4] * (np.log(n_max) - np.log(n_min)))))
+        out.append((cell, n))
+    return out

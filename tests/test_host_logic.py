@@ -124,6 +124,16 @@ def test_argument_errors_without_gpu(pdmk):
     assert L.pdmk_optimize_cubic_cover(dp(cell), 0, C.byref(m), C.byref(n), C.byref(p), dp(vec), dp(vec),
                                        dp(vec), dp(vec), dp(vec)) == 1
     L.pdmk_destroy_double(None)  # no-op
+    # batched graphs: an empty call list or a NULL graph is rejected before any GPU work
+    assert not L.pdmk_gpu_graph_create(0, None, None, None, 1)
+    assert L.pdmk_gpu_graph_launch(None, None) == -1
+    assert L.pdmk_gpu_graph_destroy(None) == -1
+    # Ewald at targets: N <= 0 and s <= 0 are rejected before any GPU work
+    idx = np.zeros(1, dtype=np.int32)
+    ip = idx.ctypes.data_as(C.POINTER(C.c_int))
+    assert L.pdmk_gpu_ewald_targets(dp(cell), 0, dp(pos), dp(q), 1, ip, 5.0, dp(out)) == -1
+    assert L.pdmk_gpu_ewald_targets(dp(cell), 1, dp(pos), dp(q), 1, ip, 0.0, dp(out)) == -1
+    assert L.pdmk_gpu_ewald_targets(dp(cell), 1, dp(pos), dp(q), 0, None, 5.0, None) == 0
 
 
 def test_workload_generator_is_deterministic():
@@ -137,6 +147,13 @@ def test_workload_generator_is_deterministic():
     assert abs(q.sum()) < 1e-12
     qpm = W.pm1_charges(1000, 3)
     assert qpm.sum() == 0 and set(np.unique(qpm)) == {-1.0, 1.0}
+    cells = W.batch_cells(256)
+    ns = np.array([n for _, n in cells])
+    assert ns.min() >= 2000 and ns.max() <= 200_000
+    assert 60 < sum(np.allclose(c, np.eye(3)) for c, _ in cells) < 196  # about half cubic
+    for c, _ in cells:
+        assert np.allclose(np.tril(c, -1), 0) and np.all(np.diag(c) == 1) and c.max() <= 1.0
+    assert W.batch_cells(256)[7][1] == cells[7][1]
 
 
 def _frame_ok(r):
