@@ -39,12 +39,14 @@ int pdmk_gpu_evaluate_float(void* h, const double* d_charges, float* d_pot, void
 /// evaluate() is a few dozen short kernels each.  No reference counterpart: the reference
 /// runs trees one after another on the CPU.  Call i evaluates handles[i] on d_charges[i]
 /// (FP64, N of that handle) into d_pots[i] (the handle's precision); the calls of one handle
-/// keep their order, handles are dealt to n_branches concurrent branches.  The buffers are
+/// keep their order, handles are dealt to n_branches branches, each a graph of its own
+/// launched on its own stream (the branches run concurrently on the hardware queues: set
+/// GPU_MAX_HW_QUEUES to about n_branches for the process to get them).  The buffers are
 /// bound at creation: a launch reads the charges they hold then.  One handle's work must not
 /// overlap another use of that handle (its scratch is shared).  create returns NULL on a
 /// NULL or quasi-periodic handle, a NULL buffer, n_calls or n_branches <= 0 or a HIP error;
-/// launch enqueues on `stream` (NULL = the graph's own stream) and returns; destroy waits for
-/// the graph's own stream and frees it.
+/// launch forks the branches from `stream` (NULL = the graph's own stream), joins them back
+/// into it and returns; destroy waits for the graph's streams and frees it.
 void* pdmk_gpu_graph_create(int n_calls, void* const* handles, const double* const* d_charges, void* const* d_pots,
                             int n_branches);
 int pdmk_gpu_graph_launch(void* g, void* stream);

@@ -200,22 +200,40 @@ void destroy_impl(void *h) {
     delete H;
 }
 
-// A captured list of evaluate() calls on device pointers, replayed as one HIP graph.  The
-// calls of one tree keep their order on one branch stream; trees are dealt to the branches
-// in order of first appearance, and the branches run concurrently.
+// A captured list of evaluate() calls on device pointers.  Trees are dealt to n_branches
+// branches in order of first appearance; each branch's calls (one tree's calls in their
+// order) are captured into a graph of its own, launched on the branch's stream, so the
+// branches run concurrently on the device's hardware queues.  A launch forks the branches
+// from the caller's stream and joins them back.
 struct BatchGraph {
     hipStream_t origin = nullptr;
     std::vector<hipStream_t> branch;
     std::vector<hipEvent_t> ev;  // ev[0]: fork; ev[1 + k]: join of branch k
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraph_t> graph;
+    std::vector<hipGraphExec_t> exec;
     ~BatchGraph() {
         if (origin) (void)hipStreamSynchronize(origin);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        for (hipStream_t b : branch) (void)hipStreamDestroy(b);
+        for (hipStream_t b : branch)
+            if (b) (void)hipStreamSynchronize(b);
+        for (hipGraphExec_t x : exec)
+            if (x) (void)hipGraphExecDestroy(x);
+        for (hipGraph_t g : graph)
+            if (g) (void)hipGraphDestroy(g);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t b : branch)
+            if (b) (void)hipStreamDestroy(b);
         if (origin) (void)hipStreamDestroy(origin);
+    }
+    void launch(hipStream_t s) {
+        using pdmk::gpu::check;
+        check(hipEventRecord(ev[0], s), "hipEventRecord");
+        for (size_t k = 0; k < exec.size(); ++k) {
+            check(hipStreamWaitEvent(branch[k], ev[0], 0), "hipStreamWaitEvent");
+            check(hipGraphLaunch(exec[k], branch[k]), "hipGraphLaunch");
+            check(hipEventRecord(ev[1 + k], branch[k]), "hipEventRecord");
+            check(hipStreamWaitEvent(s, ev[1 + k], 0), "hipStreamWaitEvent");
+        }
     }
 };
 
@@ -230,37 +248,41 @@ BatchGraph *make_batch_graph(int n_calls, void *const *handles, const double *co
         const bool qp = H->is_float ? tree_of<float>(H)->plan().qp : tree_of<double>(H)->plan().qp;
         if (qp) throw std::invalid_argument("quasi-periodic handle in call list");
     }
+    std::unordered_map<void *, int> branch_of;
+    std::vector<std::vector<int>> calls_of(n_branches);
+    for (int i = 0; i < n_calls; ++i) {
+        const int k = branch_of.emplace(handles[i], (int)branch_of.size() % n_branches).first->second;
+        calls_of[k].push_back(i);
+    }
+    while (!calls_of.empty() && calls_of.back().empty()) calls_of.pop_back();  // fewer trees than branches
+    const int nb = (int)calls_of.size();
     auto G = std::make_unique<BatchGraph>();
     check(hipStreamCreate(&G->origin), "hipStreamCreate");
-    G->branch.assign(n_branches, nullptr);
+    G->branch.assign(nb, nullptr);
     for (auto &b : G->branch) check(hipStreamCreate(&b), "hipStreamCreate");
-    G->ev.assign(n_branches + 1, nullptr);
+    G->ev.assign(nb + 1, nullptr);
     for (auto &e : G->ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    std::unordered_map<void *, int> branch_of;
-    check(hipStreamBeginCapture(G->origin, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    try {
-        check(hipEventRecord(G->ev[0], G->origin), "hipEventRecord");
-        for (hipStream_t b : G->branch) check(hipStreamWaitEvent(b, G->ev[0], 0), "hipStreamWaitEvent");
-        for (int i = 0; i < n_calls; ++i) {
-            auto *H = static_cast<Handle *>(handles[i]);
-            const int k = branch_of.emplace(handles[i], (int)branch_of.size() % n_branches).first->second;
-            if (H->is_float)
-                tree_of<float>(H)->evaluate_device(d_charges[i], static_cast<float *>(d_pots[i]), G->branch[k]);
-            else
-                tree_of<double>(H)->evaluate_device(d_charges[i], static_cast<double *>(d_pots[i]), G->branch[k]);
+    G->graph.assign(nb, nullptr);
+    G->exec.assign(nb, nullptr);
+    for (int k = 0; k < nb; ++k) {
+        check(hipStreamBeginCapture(G->branch[k], hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            for (int i : calls_of[k]) {
+                auto *H = static_cast<Handle *>(handles[i]);
+                if (H->is_float)
+                    tree_of<float>(H)->evaluate_device(d_charges[i], static_cast<float *>(d_pots[i]), G->branch[k]);
+                else
+                    tree_of<double>(H)->evaluate_device(d_charges[i], static_cast<double *>(d_pots[i]), G->branch[k]);
+            }
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(G->branch[k], &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
         }
-        for (int k = 0; k < n_branches; ++k) {
-            check(hipEventRecord(G->ev[1 + k], G->branch[k]), "hipEventRecord");
-            check(hipStreamWaitEvent(G->origin, G->ev[1 + k], 0), "hipStreamWaitEvent");
-        }
-    } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(G->origin, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
+        check(hipStreamEndCapture(G->branch[k], &G->graph[k]), "hipStreamEndCapture");
+        check(hipGraphInstantiate(&G->exec[k], G->graph[k], nullptr, nullptr, 0), "hipGraphInstantiate");
     }
-    check(hipStreamEndCapture(G->origin, &G->graph), "hipStreamEndCapture");
-    check(hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     return G.release();
 }
 
@@ -447,8 +469,7 @@ int pdmk_gpu_graph_launch(void *g, void *stream) {
     if (!g) return -1;
     return guarded("pdmk_gpu_graph_launch", [&] {
         auto *G = static_cast<BatchGraph *>(g);
-        pdmk::gpu::check(hipGraphLaunch(G->exec, stream ? static_cast<hipStream_t>(stream) : G->origin),
-                         "hipGraphLaunch");
+        G->launch(stream ? static_cast<hipStream_t>(stream) : G->origin);
         return 0;
     });
 }
