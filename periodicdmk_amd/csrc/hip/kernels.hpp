@@ -19,6 +19,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dpp_fma.hpp"
 
 namespace pdmk::gpu {
@@ -499,6 +501,11 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 
 // ------------------------------------------------------------------ plane waves
 // Plane-wave mode layout: m = k1 * (H n) + k3 * n + k2 ([x][z][y], y fastest), k3 < H.
+// The polynomial <-> plane-wave matrices have the parity of the Chebyshev polynomials:
+// W(k, j) = sum_i Vinv(j, i) e^{-i (k - n/2) x_i h} over nodes x_i symmetric about 0 with
+// T_j(-x) = (-1)^j T_j(x), so W(k, j) is real for even j and imaginary for odd j (the other
+// part is rounding, below 1e-15 of the largest entry).  The transforms use that: a product
+// with W(k, j) costs one real multiply-add per real component instead of two.
 // Outgoing expansion pw(k1,k2,k3) = sum W(k1,a) W(k2,b) W(k3,c) C[c][b][a], W = poly2pw
 // (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3), 1-D grid, (box, slice) at
 // xcd_block index box * slices + slice (a box's slices share one L2).  Box x reads proxy
@@ -527,11 +534,9 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
             for (int k = 0; k < nk; ++k) {
                 R sr = 0, si = 0;
 #pragma unroll
-                for (int c = 0; c < P; ++c) {
-                    const Cx<R> w = W[k30 + k + NPW * c];
-                    sr = fma(w.re, x[c], sr);
-                    si = fma(w.im, x[c], si);
-                }
+                for (int c = 0; c < P; c += 2) sr = fma(W[k30 + k + NPW * c].re, x[c], sr);
+#pragma unroll
+                for (int c = 1; c < P; c += 2) si = fma(W[k30 + k + NPW * c].im, x[c], si);
                 T1[k * PP + j] = {sr, si};
             }
         }
@@ -552,9 +557,15 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
                 R sr = 0, si = 0;
 #pragma unroll
                 for (int b = 0; b < P; ++b) {
-                    const Cx<R> m = W[k2 + NPW * b];
-                    sr = fma(m.re, v[b].re, fma(-m.im, v[b].im, sr));
-                    si = fma(m.re, v[b].im, fma(m.im, v[b].re, si));
+                    if (b % 2 == 0) {
+                        const R m = W[k2 + NPW * b].re;
+                        sr = fma(m, v[b].re, sr);
+                        si = fma(m, v[b].im, si);
+                    } else {
+                        const R m = W[k2 + NPW * b].im;
+                        sr = fma(-m, v[b].im, sr);
+                        si = fma(m, v[b].re, si);
+                    }
                 }
                 T2[(k * NPW + k2) * TS + a] = {sr, si};
             }
@@ -576,9 +587,15 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
                 R sr = 0, si = 0;
 #pragma unroll
                 for (int a = 0; a < P; ++a) {
-                    const Cx<R> m = W[k1 + NPW * a];
-                    sr = fma(m.re, v[a].re, fma(-m.im, v[a].im, sr));
-                    si = fma(m.re, v[a].im, fma(m.im, v[a].re, si));
+                    if (a % 2 == 0) {
+                        const R m = W[k1 + NPW * a].re;
+                        sr = fma(m, v[a].re, sr);
+                        si = fma(m, v[a].im, si);
+                    } else {
+                        const R m = W[k1 + NPW * a].im;
+                        sr = fma(-m, v[a].im, sr);
+                        si = fma(m, v[a].re, si);
+                    }
                 }
                 dst[(size_t)k1 * COL] = {sr, si};
             }
@@ -633,26 +650,43 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
     {
         const int a0 = slice * AC;
         const int na = min(AC, P - a0);
+        // Each step's outputs are polynomial indices (a0 + a, b0 + b, c0 + c): the matrix row of
+        // an output is real (even index) or imaginary (odd).  The parity of the first output
+        // (uniform across a workgroup or a wavefront) selects a template instance, in which the
+        // unrolled loops fold every output's parity.
+        using Even = std::integral_constant<int, 0>;
+        using Odd = std::integral_constant<int, 1>;
         // x: U1[a][k3][k2] = sum_k1 V(k1, a0 + a) pin[k1][k3][k2], column = (k3, k2)
-        for (int col = threadIdx.x; col < COL; col += blockDim.x) {
-            Cx<R> acc[AC];
+        auto xstep = [&](auto par) {
+            constexpr int A0P = decltype(par)::value;
+            for (int col = threadIdx.x; col < COL; col += blockDim.x) {
+                Cx<R> acc[AC];
 #pragma unroll
-            for (int a = 0; a < AC; ++a) acc[a] = {R(0), R(0)};
-            for (int k1 = 0; k1 < NPW; ++k1) {
-                const Cx<R> x = X[(size_t)k1 * COL + col];
+                for (int a = 0; a < AC; ++a) acc[a] = {R(0), R(0)};
+                for (int k1 = 0; k1 < NPW; ++k1) {
+                    const Cx<R> x = X[(size_t)k1 * COL + col];
 #pragma unroll
-                for (int a = 0; a < AC; ++a) {
-                    if (a < na) {
-                        const Cx<R> m = V[k1 + NPW * (a0 + a)];
-                        acc[a].re = fma(m.re, x.re, fma(-m.im, x.im, acc[a].re));
-                        acc[a].im = fma(m.re, x.im, fma(m.im, x.re, acc[a].im));
+                    for (int a = 0; a < AC; ++a) {
+                        if (a < na) {
+                            if ((A0P + a) % 2 == 0) {
+                                const R m = V[k1 + NPW * (a0 + a)].re;
+                                acc[a].re = fma(m, x.re, acc[a].re);
+                                acc[a].im = fma(m, x.im, acc[a].im);
+                            } else {
+                                const R m = V[k1 + NPW * (a0 + a)].im;
+                                acc[a].re = fma(-m, x.im, acc[a].re);
+                                acc[a].im = fma(m, x.re, acc[a].im);
+                            }
+                        }
                     }
                 }
-            }
 #pragma unroll
-            for (int a = 0; a < AC; ++a)
-                if (a < na) U1[a * COL + col] = acc[a];
-        }
+                for (int a = 0; a < AC; ++a)
+                    if (a < na) U1[a * COL + col] = acc[a];
+            }
+        };
+        if (a0 & 1) xstep(Odd{});
+        else xstep(Even{});
         __syncthreads();
         // y: U2[a][k3][b] = h(k3) sum_k2 V(k2, b) U1[a][k3][k2]; slot = (column w = a H + k3,
         // segment of b, uniform across a wavefront)
@@ -663,24 +697,35 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
             t3_slot<SGY>(t, w, sg);
             if (w >= na * H) continue;
             const int k3 = w % H, b0 = sg * PSY;
-            Cx<R> acc[PSY];
+            auto ystep = [&](auto par) {
+                constexpr int B0P = decltype(par)::value;
+                Cx<R> acc[PSY];
 #pragma unroll
-            for (int b = 0; b < PSY; ++b) acc[b] = {R(0), R(0)};
-            for (int k2 = 0; k2 < NPW; ++k2) {
-                const Cx<R> x = U1[w * NPW + k2];
+                for (int b = 0; b < PSY; ++b) acc[b] = {R(0), R(0)};
+                for (int k2 = 0; k2 < NPW; ++k2) {
+                    const Cx<R> x = U1[w * NPW + k2];
 #pragma unroll
-                for (int b = 0; b < PSY; ++b) {
-                    if (b0 + b < P) {
-                        const Cx<R> m = V[k2 + NPW * (b0 + b)];
-                        acc[b].re = fma(m.re, x.re, fma(-m.im, x.im, acc[b].re));
-                        acc[b].im = fma(m.re, x.im, fma(m.im, x.re, acc[b].im));
+                    for (int b = 0; b < PSY; ++b) {
+                        if (b0 + b < P) {
+                            if ((B0P + b) % 2 == 0) {
+                                const R m = V[k2 + NPW * (b0 + b)].re;
+                                acc[b].re = fma(m, x.re, acc[b].re);
+                                acc[b].im = fma(m, x.im, acc[b].im);
+                            } else {
+                                const R m = V[k2 + NPW * (b0 + b)].im;
+                                acc[b].re = fma(-m, x.im, acc[b].re);
+                                acc[b].im = fma(m, x.re, acc[b].im);
+                            }
+                        }
                     }
                 }
-            }
-            const R hw = (k3 >= half) ? R(0.5) : R(1);
+                const R hw = (k3 >= half) ? R(0.5) : R(1);
 #pragma unroll
-            for (int b = 0; b < PSY; ++b)
-                if (b0 + b < P) U2[w * TS2 + b0 + b] = {acc[b].re * hw, acc[b].im * hw};
+                for (int b = 0; b < PSY; ++b)
+                    if (b0 + b < P) U2[w * TS2 + b0 + b] = {acc[b].re * hw, acc[b].im * hw};
+            };
+            if (b0 & 1) ystep(Odd{});
+            else ystep(Even{});
         }
         __syncthreads();
         // z: down[c][b][a0 + a] (+)= 2 Re sum_k3 V(k3, c) U2[a][k3][b]; slot = (column w = b na + a,
@@ -692,24 +737,29 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
             t3_slot<SGZ>(t, w, sg);
             if (w >= na * P) continue;
             const int b = w / na, a = w - b * na, c0 = sg * PSZ;
-            R acc[PSZ];
+            auto zstep = [&](auto par) {
+                constexpr int C0P = decltype(par)::value;
+                R acc[PSZ];
 #pragma unroll
-            for (int c = 0; c < PSZ; ++c) acc[c] = R(0);
-            for (int k3 = 0; k3 < H; ++k3) {
-                const Cx<R> x = U2[(a * H + k3) * TS2 + b];
+                for (int c = 0; c < PSZ; ++c) acc[c] = R(0);
+                for (int k3 = 0; k3 < H; ++k3) {
+                    const Cx<R> x = U2[(a * H + k3) * TS2 + b];
 #pragma unroll
-                for (int c = 0; c < PSZ; ++c) {
-                    if (c0 + c < P) {
-                        const Cx<R> m = V[k3 + NPW * (c0 + c)];
-                        acc[c] = fma(m.re, x.re, fma(-m.im, x.im, acc[c]));
+                    for (int c = 0; c < PSZ; ++c) {
+                        if (c0 + c < P) {
+                            if ((C0P + c) % 2 == 0) acc[c] = fma(V[k3 + NPW * (c0 + c)].re, x.re, acc[c]);
+                            else acc[c] = fma(-V[k3 + NPW * (c0 + c)].im, x.im, acc[c]);
+                        }
                     }
                 }
-            }
-            R *dst = down + (size_t)b * P + a0 + a;
+                R *dst = down + (size_t)b * P + a0 + a;
 #pragma unroll
-            for (int c = 0; c < PSZ; ++c)
-                if (c0 + c < P)
-                    dst[(size_t)(c0 + c) * PP] = (zero ? R(0) : dst[(size_t)(c0 + c) * PP]) + R(2) * acc[c];
+                for (int c = 0; c < PSZ; ++c)
+                    if (c0 + c < P)
+                        dst[(size_t)(c0 + c) * PP] = (zero ? R(0) : dst[(size_t)(c0 + c) * PP]) + R(2) * acc[c];
+            };
+            if (c0 & 1) zstep(Odd{});
+            else zstep(Even{});
         }
         __syncthreads();
     }

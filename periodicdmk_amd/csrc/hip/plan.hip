@@ -73,11 +73,11 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         }
         build_ms[1] = ms_since(t_start);
     } catch (...) {
-        for (auto &e : ev) hipEventDestroy(e);
-        for (hipEvent_t e : {e_in, e_gather, e_far, e_near, e_up}) hipEventDestroy(e);
-        hipStreamDestroy(s_far);
-        hipStreamDestroy(s_near);
-        hipStreamDestroy(stream);
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {e_in, e_gather, e_far, e_near, e_up}) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(s_far);
+        (void)hipStreamDestroy(s_near);
+        (void)hipStreamDestroy(stream);
         throw;
     }
     const host::DmkTables T = host::make_tables(n_digits_user, g.rc, t.nlevels, g.umat);
