@@ -855,25 +855,29 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval_targets(const R *__restrict
 //   R[t, (c, b)] = sum_a T_a(x_t) C[(c, b), a]          (GEMM, a padded to 32)
 //   pot_t += sum_(c,b) R[t, (c, b)] T_b(y_t) T_c(z_t)
 // with v_mfma_f64_16x16x4f64 (operand layout as in k_charge2proxy_mfma: A[l % 16][l / 16],
-// B[l / 16][l % 16], D[l / 16 + 4 r][l % 16]).  Wave w takes the (c, b) tiles w, w + NW, ...;
-// for each it loads the tile's 8 B operands once and runs the 4 target tiles of the stage
+// B[l / 16][l % 16], D[l / 16 + 4 r][l % 16]).  Wave w takes the (c, b) tiles w, w + NW, ...,
+// whose B operands stay in registers for the whole leaf; each stage runs the TT target tiles
 // as independent accumulators.  Lane partials are reduced over each 16-lane row, then over
-// the waves in a fixed order.
+// the waves in a fixed order.  (Reloading the B operands per stage of 64 targets read each
+// leaf's proxy 4 times: 1846 us at N=1e6, 9 digits.)
 template <int P> struct EvMfmaCfg {
     static constexpr int RT = (P * P + 15) / 16;
     static constexpr int NW = 8;
-    static constexpr int TT = 4;          // target tiles per stage
-    static constexpr int NT = 16 * TT;    // targets per stage
+    static constexpr int MT = (RT + NW - 1) / NW;  // (c, b) tiles per wave
+    static constexpr int TT = 2;                   // target tiles per stage
+    static constexpr int NT = 16 * TT;             // targets per stage
     static constexpr int PA = 32;
 };
-
+// One workgroup per leaf.  Wave w owns the (c, b) tiles w, w + NW, ...: their B operands
+// (the leaf's proxy rows) are loaded into registers once per leaf, and the targets pass by
+// in stages of NT, so the proxy is read from memory once per leaf.
 template <int P>
 __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
     const double *__restrict__ down_base, const int *__restrict__ leaf_down, const double *__restrict__ leaf_geom,
     const int *__restrict__ leaf_trg, const P4<double> *__restrict__ trg, double *__restrict__ pot_sorted) {
     chain_priority();
     using C = EvMfmaCfg<P>;
-    constexpr int RT = C::RT, NW = C::NW, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
+    constexpr int RT = C::RT, NW = C::NW, MT = C::MT, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
     static_assert(P <= PA, "order above the padded a count");
     __shared__ double Tx[NT * PA], Ty[NT * P], Tz[NT * P], red[NW * NT];
     const int lf = blockIdx.x, lane = threadIdx.x & 63;
@@ -882,6 +886,17 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
     const double *Cm = down_base + (size_t)leaf_down[lf] * P * PP;
     const double cx = leaf_geom[4 * lf], cy = leaf_geom[4 * lf + 1], cz = leaf_geom[4 * lf + 2], sc = leaf_geom[4 * lf + 3];
     const int beg = leaf_trg[2 * lf], cnt = leaf_trg[2 * lf + 1];
+    // B operands of tile rt = w + NW m: lane (kk, jj) supplies C[rt 16 + jj][4 ks + kk]
+    double B[MT][PA / 4];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int row = (w + NW * m) * 16 + jj;
+#pragma unroll
+        for (int ks = 0; ks < PA / 4; ++ks) {
+            const int a = 4 * ks + kk;
+            B[m][ks] = (row < PP && a < P) ? Cm[(size_t)row * P + a] : 0.0;
+        }
+    }
     for (int t0 = 0; t0 < cnt; t0 += NT) {
         __syncthreads();
         for (int task = threadIdx.x; task < 3 * NT; task += blockDim.x) {
@@ -916,33 +931,21 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
         for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
             for (int e = 0; e < 4; ++e) part[tt][e] = 0.0;
-        // B operands of (c, b) tile rt: lane (kk, jj) supplies C[rt 16 + jj][4 ks + kk]
-        auto load_b = [&](double (&B)[PA / 4], int rt) {
-            const int row = rt * 16 + jj;
 #pragma unroll
-            for (int ks = 0; ks < PA / 4; ++ks) {
-                const int a = 4 * ks + kk;
-                B[ks] = (row < PP && a < P) ? Cm[(size_t)row * P + a] : 0.0;
-            }
-        };
-        double Bn[PA / 4];
-        load_b(Bn, w);
-        for (int rt = w; rt < RT; rt += NW) {
+        for (int m = 0; m < MT; ++m) {
+            const int rt = w + NW * m;
+            if (rt >= RT) break;  // wave-uniform
             const int row = rt * 16 + jj;  // (c, b) of the B column this lane supplies
-            const bool rok = row < PP;
-            double B[PA / 4];
-#pragma unroll
-            for (int ks = 0; ks < PA / 4; ++ks) B[ks] = Bn[ks];
-            if (rt + NW < RT) load_b(Bn, rt + NW);  // next tile's operands, in flight meanwhile
             c2p_d4 D[TT];
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt) D[tt] = c2p_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int ks = 0; ks < PA / 4; ++ks)
 #pragma unroll
-                for (int tt = 0; tt < TT; ++tt) D[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[tt][ks], B[ks], D[tt], 0, 0, 0);
+                for (int tt = 0; tt < TT; ++tt)
+                    D[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[tt][ks], B[m][ks], D[tt], 0, 0, 0);
             // D[tt][e] = R[target tt 16 + kk + 4 e, (c, b) = rt 16 + jj]
-            const int cb = rok ? row : 0, c = cb / P, b = cb % P;
+            const int cb = row < PP ? row : 0, c = cb / P, b = cb % P;
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
