@@ -53,7 +53,11 @@ PARAMS = [("cubic", "double", d) for d in (3, 6, 9, 12)] + [
     ("clustered", "double", 6), ("clustered", "double", 9), ("clustered", "float", 6),
     ("single_level", "double", 6), ("single_level", "float", 6),
     ("long_cell", "double", 6),
-]
+] + [
+    # the other digit counts: each has its own residual polynomial (and, below 6 digits, the
+    # even/odd split evaluation) on one of the four proxy orders
+    ("cubic", "double", d) for d in (1, 2, 4, 5, 7, 8, 10, 11)
+] + [("cubic", "float", d) for d in (1, 2, 4, 5)]
 
 
 @pytest.mark.parametrize("name,precision,digits", PARAMS)
