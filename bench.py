@@ -34,6 +34,22 @@ PEAK_HBM_GBS = 8000.0
 EWALD_TARGETS = 1000
 
 
+def path_roofline(res):
+    """Composite roofline of one evaluate(): t_roof = F_main / P_main + F_far / P_fp64 + B / BW
+    over the plan's exact algorithmic counts (near-field flops of in-support pairs, transform
+    flops, far-field flops, O(N) and near-field bytes), with the published peaks (vector or
+    matrix FP32 157.3, FP64 78.6 TFLOP/s; HBM 8 TB/s); frac = t_roof / measured time."""
+    w = res["work"]
+    main_peak = PEAK_FP32_TFLOPS if res["precision"] == "float" else PEAK_FP64_TFLOPS
+    f_main = w["nf_flops"] + w["transform_flops"]
+    nbytes = w["on_bytes"] + w["nf_bytes"]
+    t_ms = 1e3 * (f_main / (main_peak * 1e12) + w["ff_flops"] / (PEAK_FP64_TFLOPS * 1e12) +
+                  nbytes / (PEAK_HBM_GBS * 1e9))
+    return {"t_roof_ms": round(t_ms, 4), "ms_measured": round(res["ms_per_step"], 4),
+            "frac": round(t_ms / res["ms_per_step"], 4), "flops_main": f_main, "main_peak_tflops": main_peak,
+            "flops_far_fp64": w["ff_flops"], "bytes": nbytes}
+
+
 def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, charges_dev, n_vec):
     t0 = time.time()
     s = P.PDMKSolver(cell, pos, n_digits=n_digits, n_per_leaf=npl, nz=nz, precision=precision)
@@ -224,6 +240,10 @@ def main():
         "tree_build": tree_build,
         "paths": {k: {kk: vv for kk, vv in v.items() if kk not in ("stats",)} for k, v in results.items()},
     }
+    for nm in results:
+        results[nm]["path_roofline"] = path_roofline(results[nm])
+    line["path_roofline"] = results[next(iter(results)) if "mixed6" not in results else "mixed6"]["path_roofline"]
+    line["paths"] = {k: {kk: vv for kk, vv in v.items() if kk not in ("stats",)} for k, v in results.items()}
     if "mixed6" in results and "fp64_6" in results:
         line["mixed_vs_fp64_6"] = round(results["fp64_6"]["ms_per_step"] / results["mixed6"]["ms_per_step"], 3)
     # a throughput of wrong potentials is no measurement: flag any path whose potentials are
