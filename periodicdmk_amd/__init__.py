@@ -77,6 +77,7 @@ def lib():
         "pdmk_gpu_build_info": (C.c_int, [vp, dp]),
         "pdmk_gpu_graph_create": (vp, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
         "pdmk_gpu_graph_launch": (C.c_int, [vp, vp]),
+        "pdmk_gpu_set_single_stream": (C.c_int, [vp, C.c_int]),
         "pdmk_gpu_graph_destroy": (C.c_int, [vp]),
         "pdmk_campaign_sha256": (C.c_int, [vp, C.c_size_t, vp]),
         "pdmk_campaign_write": (C.c_int, [C.c_char_p, C.c_uint, C.c_uint, C.c_int, C.c_int, C.c_ulonglong, dp,
@@ -325,6 +326,12 @@ class PDMKSolver:
         f = lib().pdmk_gpu_evaluate_float if self.precision == "float" else lib().pdmk_gpu_evaluate_double
         if f(self._h, C.c_void_p(d_charges), C.c_void_p(d_pot), C.c_void_p(stream)) != 0:
             raise RuntimeError("pdmk_gpu_evaluate failed (see stderr)")
+
+    def set_single_stream(self, on: bool = True) -> None:
+        """evaluate() on the caller's stream only (pdmk_gpu_set_single_stream): for many trees
+        evaluated side by side."""
+        if lib().pdmk_gpu_set_single_stream(self._h, 1 if on else 0) != 0:
+            raise RuntimeError("pdmk_gpu_set_single_stream failed")
 
     def evaluate_timed(self, d_charges: int, d_pot: int):
         t = np.zeros(3)

@@ -8,7 +8,8 @@ Cells and point counts from workloads.batch_cells (cubic or triclinic, N log-uni
 [2e3, 2e5]); 6 digits in FP32, n_per_leaf 300, nz 4; charges resident in device memory.
 Many trees run side by side only on as many hardware queues as the process has
 (GPU_MAX_HW_QUEUES, 4 by default): measured on 256 cells x 8 vectors, eager calls reach
-95 Mpts/s with 4 queues and 170 with 16, graphs 97 and 109.
+97 Mpts/s with 4 queues and 172 with 16, graphs 98 and 110; with --single-stream (each
+tree's stages one after another on its own stream) eager calls reach 141 with 4 queues.
 Each graph holds every tree's calls for --per-graph of the vectors; a pass launches
 vectors / per-graph graphs back to back.  The graph's potentials are checked bit for bit
 against the eager pass.  Prints one JSON line: points x vectors per second for both.
@@ -32,6 +33,8 @@ def main():
     ap.add_argument("--branches", type=int, default=8)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--n-max", type=int, default=200_000)
+    ap.add_argument("--single-stream", action="store_true",
+                    help="each tree's evaluate() on its own stream only (pdmk_gpu_set_single_stream)")
     args = ap.parse_args()
     if P.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -44,6 +47,8 @@ def main():
     for i, (cell, n) in enumerate(cells):
         s = P.PDMKSolver(cell, W.uniform_positions(n, 1000 + i, cell), n_digits=6, n_per_leaf=300, nz=4,
                          precision="float")
+        if args.single_stream:
+            s.set_single_stream(True)
         qs = [P.DeviceArray.from_numpy(W.neutral_uniform_charges(n, 100000 + 32 * i + v)) for v in range(args.vectors)]
         outs = [P.DeviceArray(n, np.float32) for _ in range(args.vectors)]
         trees.append((s, qs, outs))
@@ -101,6 +106,7 @@ def main():
         "graph_mpts_per_s": round(total / min(t_graph) / 1e6, 2),
         "graphs": len(graphs),
         "branches": args.branches,
+        "single_stream": bool(args.single_stream),
         "graph_equals_eager": bool(same),
         "tree_build_s": round(build_s, 1),
         "graph_build_s": round(graph_build_s, 1),

@@ -465,6 +465,13 @@ void *pdmk_gpu_graph_create(int n_calls, void *const *handles, const double *con
     }
 }
 
+int pdmk_gpu_set_single_stream(void *h, int on) {
+    if (!h) return -1;
+    if (static_cast<Handle *>(h)->is_float) tree_of<float>(h)->plan().single_stream = on != 0;
+    else tree_of<double>(h)->plan().single_stream = on != 0;
+    return 0;
+}
+
 int pdmk_gpu_graph_launch(void *g, void *stream) {
     if (!g) return -1;
     return guarded("pdmk_gpu_graph_launch", [&] {

@@ -843,7 +843,7 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     // evaluate_timed's split (far field, proxy/plane waves, near field) needs the stages one
     // after the other; otherwise the far field and the near field run on their own streams
     // beside the transform chain and join before the final combination
-    const bool serial = times_ms != nullptr;
+    const bool serial = times_ms != nullptr || single_stream;
     hipStream_t sf = serial ? s : s_far, sn = serial ? s : s_near;
     if (!serial) {
         check(hipEventRecord(e_in, s), "hipEventRecord");

@@ -120,6 +120,9 @@ struct Plan {
     // the level's shifts); c2p_scratch then shares pw_out's memory (the upward pass is done
     // before the first plane wave is formed, on the same stream).
     bool pw_by_level = false;
+    // evaluate() on the caller's stream only (the stages one after another): for callers that
+    // run many trees side by side, where the trees, not the stages, fill the GPU
+    bool single_stream = false;
     Real *c2p_buffer() { return pw_by_level ? pw_out.template as<Real>() : c2p_scratch.template as<Real>(); }
     DevBuf mats_rm;                         // parent <-> child matrices: [transposed][minus / plus][p][p]
     // plane waves

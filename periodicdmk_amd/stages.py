@@ -6,8 +6,10 @@ stage on its own (no other stream competing for the GPU):
     python -m periodicdmk_amd.stages --workload clustered --n 4000000 --paths mixed6,fp64_9,fp64_12 --stages 15
 
 Stage bits: 1 proxy / plane-wave chain, 2 near field, 4 self correction, 8 far field.
-With --bloch the trees are quasi-periodic and the calls are evaluate_complex (device
-pointers).  Prints the mean wall time per call after the first (synchronized).
+--stages 15 calls evaluate() itself (device pointers; far field, near field and chain side
+by side); other selections run the selected stages one after another.  With --bloch the
+trees are quasi-periodic and the calls are evaluate_complex (device pointers).  Prints the
+mean wall time per call after the first (synchronized).
 Workloads (n_per_leaf 300, nz 4):
   uniform   -- cubic L=1, N uniform positions (splitmix64 seed 11), neutral U[-1,1] charges
                (seed 100): the same points as bench.py;
@@ -55,12 +57,14 @@ def main():
             if bloch:
                 if P.lib().pdmk_gpu_evaluate_complex(s._h, P.C.c_void_p(q.ptr), P.C.c_void_p(out.ptr), args.stages) != 0:
                     raise RuntimeError("pdmk_gpu_evaluate_complex failed")
+            elif args.stages == 15:
+                s.evaluate_device(q.ptr, out.ptr)  # evaluate() itself, its stages side by side
             else:
-                s.evaluate_stages(q.ptr, out.ptr, args.stages)
+                s.evaluate_stages(q.ptr, out.ptr, args.stages)  # the selected stages one after another
             P.synchronize()
             times.append(time.perf_counter() - t0)
         ms = 1e3 * sum(times[1:]) / max(1, len(times) - 1)
-        kind = "evaluate_complex" if bloch else "evaluate_stages"
+        kind = "evaluate_complex" if bloch else ("evaluate" if args.stages == 15 else "evaluate_stages")
         info = s.build_info()
         print(f"[stages] {name}: {args.calls} calls of {kind} stages {args.stages}, {ms:.3f} ms per call after the first"
               f" (build {info['build_ms']:.0f} ms, {info['proxy_slots']} proxy slots, plane waves by level: "

@@ -81,6 +81,21 @@ def test_graph_single_branch_and_repeated_handle(pdmk, cuda, trees):
         g.close()
 
 
+def test_single_stream_equals_concurrent(pdmk, cuda, trees):
+    """pdmk_gpu_set_single_stream runs the same kernels on the same values, one stream."""
+    ref = eager(pdmk, trees)
+    for s, _, _ in trees:
+        s.set_single_stream(True)
+    try:
+        got = eager(pdmk, trees)
+    finally:
+        for s, _, _ in trees:
+            s.set_single_stream(False)
+    for t in range(len(trees)):
+        for v in range(3):
+            np.testing.assert_array_equal(got[t][v], ref[t][v])
+
+
 def test_graph_errors(pdmk, cuda, trees):
     s, qs, outs = trees[0]
     with pytest.raises(RuntimeError):

@@ -128,6 +128,7 @@ def test_argument_errors_without_gpu(pdmk):
     assert not L.pdmk_gpu_graph_create(0, None, None, None, 1)
     assert L.pdmk_gpu_graph_launch(None, None) == -1
     assert L.pdmk_gpu_graph_destroy(None) == -1
+    assert L.pdmk_gpu_set_single_stream(None, 1) == -1
     # Ewald at targets: N <= 0 and s <= 0 are rejected before any GPU work
     idx = np.zeros(1, dtype=np.int32)
     ip = idx.ctypes.data_as(C.POINTER(C.c_int))
