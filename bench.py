@@ -173,12 +173,20 @@ def main():
             t0 = time.perf_counter()
             ref[nd] = o.evaluate(qs[qi])
             te = time.perf_counter() - t0
-            o.close()
             if nd == 6:
                 cpu = {"value": n / te / 1e6, "unit": "Mpts/s", "cores": threads, "kind": "port",
                        "sample": f"full workload (N={n}, n_digits=6, FP64), one evaluate() after a "
                                  f"{tb:.1f} s tree build; CPU oracle restating the reference evaluate() "
                                  f"with an exact direct-DFT far field", "seconds": te}
+                # the same evaluate() on the same tree on one core
+                print("[bench] CPU oracle on one thread", file=sys.stderr, flush=True)
+                prev = O.set_threads(1)
+                t0 = time.perf_counter()
+                o.evaluate(qs[qi])
+                t1 = time.perf_counter() - t0
+                O.set_threads(prev)
+                cpu["single_core"] = {"value": n / t1 / 1e6, "unit": "Mpts/s", "cores": 1, "seconds": t1}
+            o.close()
         for nm in wanted:
             r = ref[paths[nm][1]]
             d = pots[nm] - r

@@ -57,8 +57,15 @@ def lib():
         L.oracle_cheb_tables.argtypes = [C.c_int, dp, dp, dp]
         L.oracle_ewald_direct.argtypes = [dp, C.c_int, dp, dp, C.c_double, dp, dp]
         L.oracle_ewald_cells.argtypes = [dp, C.c_int, dp, dp, C.c_double, C.c_int, ip, dp, C.c_double]
+        L.oracle_set_threads.restype = C.c_int
+        L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's loops; returns the previous count."""
+    return int(lib().oracle_set_threads(int(n)))
 
 
 def _d(a):

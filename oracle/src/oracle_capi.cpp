@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <omp.h>
 
 using namespace oracle;
 
@@ -188,6 +189,14 @@ int oracle_ewald_cells(const double *cell, int N, const double *pos, const doubl
                        const int *tidx, double *pot, double alpha) {
     try { ewald_cells(cell, N, pos, q, s, M, tidx, pot, alpha); return 0; }
     catch (const std::exception &e) { std::fprintf(stderr, "oracle_ewald_cells: %s\n", e.what()); return -1; }
+}
+
+// OpenMP threads of the oracle's parallel loops (the CPU baseline is timed at all threads and
+// at one).  Returns the previous count.
+int oracle_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
 }
 
 }  // extern "C"
