@@ -27,6 +27,10 @@ def load_helper():
 
 
 def test_helper_exports_entry(pdmk):
+    """The C++ caller compiles against include/ and links libpdmk.so (built here if missing)."""
+    if not os.path.exists(HELPER):
+        import __graft_entry__
+        __graft_entry__.build_api_check()
     assert hasattr(load_helper(), "api_check")
 
 
