@@ -137,6 +137,63 @@ def test_repeat_evaluate_is_deterministic(pdmk, cuda):
         assert np.sum(np.abs(p1 - p2)) > 1e-6
 
 
+def test_reused_tree_equals_fresh_tree(pdmk, cuda):
+    """A tree that evaluated other charges gives the potentials of a fresh tree
+    (cpp/tests/test_pbc_tree_update_charges.cpp:11-76: update_charges + eval = new tree)."""
+    cell, pos, q, npl = case("triclinic")
+    q2 = W.neutral_uniform_charges(pos.shape[0], 77)
+    for prec in ("double", "float"):
+        a = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision=prec)
+        a.evaluate(q)
+        reused = a.evaluate(q2)
+        a.close()
+        b = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision=prec)
+        np.testing.assert_array_equal(reused, b.evaluate(q2))
+        b.close()
+
+
+def test_evaluate_timed_parts_cover_wall_time(pdmk, cuda):
+    """evaluate_timed's three parts (far field, proxy / plane waves, near field) add up to
+    the call's wall time (cpp/tests/test_evaluate_timed.cpp:47-48: sum in [0.95, 1] wall);
+    N = 1e6 so that the host call overhead is small against the GPU time."""
+    import time
+    n = 1_000_000
+    cell = W.cubic_cell()
+    s = pdmk.PDMKSolver(cell, W.uniform_positions(n, 11, cell), n_digits=6, n_per_leaf=300, precision="float")
+    qd = pdmk.DeviceArray.from_numpy(W.neutral_uniform_charges(n, 100))
+    out = pdmk.DeviceArray(n, s.dtype)
+    try:
+        s.evaluate_timed(qd.ptr, out.ptr)
+        ratios = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            t = s.evaluate_timed(qd.ptr, out.ptr)  # synchronous
+            wall = 1e3 * (time.perf_counter() - t0)
+            ratios.append(sum(t.values()) / wall)
+        r = max(ratios)
+        assert 0.95 <= r <= 1.0, ratios
+    finally:
+        qd.free()
+        out.free()
+        s.close()
+
+
+def test_deep_tree_parity(pdmk, oracle_mod, cuda):
+    """A 3e-7 cluster refines to 23 levels (built with the points on the host): the full
+    evaluate against the oracle (cpp/tests/test_nonuniform.cpp covers deep trees)."""
+    cub = W.cubic_cell()
+    pos = np.vstack([0.3 + 3e-7 * np.random.default_rng(3).random((600, 3)), W.uniform_positions(400, 8, cub)])
+    q = W.neutral_uniform_charges(pos.shape[0], 9)
+    o = oracle_mod.OracleTree(cub, pos, 6, 20, 4, precision="double")
+    ref = o.evaluate(q)
+    o.close()
+    for prec, tol in (("double", 1e-12), ("float", 1e-5)):
+        s = pdmk.PDMKSolver(cub, pos, n_digits=6, n_per_leaf=20, nz=4, precision=prec)
+        assert s.stats()["n_levels"] >= 20
+        assert rel(s.evaluate(q).astype(np.float64), ref) < tol
+        s.close()
+
+
 def test_device_entry_points(pdmk, cuda):
     import ctypes as C
     cell, pos, q, npl = case("cubic")
