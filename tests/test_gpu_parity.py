@@ -41,6 +41,9 @@ def case(name):
     if name == "single_level":  # N = 1e4, n_per_leaf 300: every leaf is a level-0 box
         c = W.cubic_cell()
         return c, W.uniform_positions(10_000, 1, c), W.neutral_uniform_charges(10_000, 2), 300
+    if name == "rectangular":  # orthorhombic, sides 1 x 1.5 x 2 (cpp/tests/test_nonuniform.cpp rectangular cells)
+        c = np.diag([1.0, 1.5, 2.0])
+        return c, W.uniform_positions(4000, 12, c), W.neutral_uniform_charges(4000, 13), 40
     if name == "long_cell":  # |a2| > |a1|: the lattice frame reorders the axes
         c = np.array([[1.0, 0.0, 0.1], [0.0, 1.7, 0.0], [0.0, 0.2, 1.2]])
         return c, W.uniform_positions(3000, 9, c), W.neutral_uniform_charges(3000, 10), 40
@@ -52,7 +55,8 @@ PARAMS = [("cubic", "double", d) for d in (3, 6, 9, 12)] + [
     ("triclinic", "double", 6), ("triclinic", "double", 9), ("triclinic", "float", 6),
     ("clustered", "double", 6), ("clustered", "double", 9), ("clustered", "float", 6),
     ("single_level", "double", 6), ("single_level", "float", 6),
-    ("long_cell", "double", 6),
+    ("long_cell", "double", 6), ("rectangular", "double", 6), ("rectangular", "float", 6),
+    ("rectangular", "double", 9),
 ] + [
     # the other digit counts: each has its own residual polynomial (and, below 6 digits, the
     # even/odd split evaluation) on one of the four proxy orders
