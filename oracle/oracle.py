@@ -5,6 +5,11 @@ reference ``PeriodicDMKTreeT<Real,3>`` construction and ``evaluate()``
 (cpp/src/periodic_dmk.cpp:41-186) plus the FP64 Ewald sums
 (cpp/src/ewald_nufft.cpp:777-910).  Only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` may use this module, and only as the checker.
+
+Parity unpinned (checked end to end against the FP64 Ewald sum only): SCTL's
+approx_rsqrt inside the reference residual kernel (replaced by an exact 1/sqrt), DUCC's
+NUFFT far field (replaced by the exact sum it approximates to eps/4), and the reference's
+exact tree and intermediate arrays, for which no fixture exists.
 """
 from __future__ import annotations
 

@@ -22,6 +22,14 @@
 //   * the residual kernel uses an exact 1/sqrt instead of SCTL's approx_rsqrt;
 //   * the far field is the exact truncated Fourier sum evaluated as a direct DFT
 //     in FP64 (DUCC's NUFFT approximates the same sum to eps/4).
+//
+// Parity unpinned (no fixture or closed form fixes these exactly; they are checked end to
+// end against the FP64 Ewald sum only):
+//   * SCTL's approx_rsqrt<digits> inside the reference residual kernel
+//     (cpp/include/dmk/vector_kernels.hpp:65-73), replaced here by an exact 1/sqrt;
+//   * DUCC's NUFFT output for the far field (DUCC0 @ 1da2c31dbcdc3a542ae3e9f723385acc8bcb9ec2,
+//     not vendored), replaced here by the exact sum it approximates;
+//   * the reference's exact tree and intermediate arrays (proxy charges, plane waves).
 #pragma once
 
 #include "otables.hpp"
