@@ -141,6 +141,18 @@ def test_repeat_evaluate_is_deterministic(pdmk, cuda):
         assert np.sum(np.abs(p1 - p2)) > 1e-6
 
 
+@pytest.mark.parametrize("digits,tol", [(6, 1e-5), (9, 1e-8)])
+def test_energy_vs_ewald(pdmk, cuda, digits, tol):
+    """Energy 1/2 sum q phi of clustered charges against the Ewald energy
+    (cpp/tests/test_nonuniform.cpp:369-377)."""
+    cell, pos, q, npl = case("clustered")
+    s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=npl, precision="double")
+    e = 0.5 * np.dot(q, s.evaluate(q))
+    s.close()
+    _, e_ref = pdmk.ewald(cell, pos, q, 6.0)
+    assert abs(e - e_ref) < tol * abs(e_ref), (e, e_ref)
+
+
 def test_reused_tree_equals_fresh_tree(pdmk, cuda):
     """A tree that evaluated other charges gives the potentials of a fresh tree
     (cpp/tests/test_pbc_tree_update_charges.cpp:11-76: update_charges + eval = new tree)."""
