@@ -349,7 +349,10 @@ template <class R, int P, int NPW> struct P2pwCfg {
 };
 template <class R, int P, int NPW> struct Pw2pCfg {
     static constexpr int H = (NPW + 1) / 2;
-    static constexpr int AC = tf_slices(P, (H * NPW + H * OddStride<P>::v) * (int)sizeof(Cx<R>));
+    // (FP64 at p = 28: 2 slices of a per workgroup, 43 KB, so three workgroups share a CU;
+    // 3 per workgroup at 65 KB fit two: N=1e6 9-digit evaluate() 9.12 -> 9.07 ms)
+    static constexpr int AC = tf_slices(P, (H * NPW + H * OddStride<P>::v) * (int)sizeof(Cx<R>), 0,
+                                        (P == 28 && sizeof(R) == 8) ? 48 * 1024 : kTfLdsBytes);
     static constexpr int slices = (P + AC - 1) / AC;
     // the y and z steps have AC H and AC P columns of P outputs for 256 threads; the outputs of
     // a column are split over SGY / SGZ wavefronts when that fills more of them
