@@ -338,7 +338,10 @@ constexpr int tf_slices(int total, int per_slice, int reserved = 0, int budget =
 // workgroup of its own (grid y), so a level with few boxes still fills the GPU.
 template <class R, int P, int NPW> struct P2pwCfg {
     static constexpr int H = (NPW + 1) / 2;
-    static constexpr int K3 = tf_slices(H, (P * P + NPW * OddStride<P>::v) * (int)sizeof(Cx<R>));
+    // (FP64 at p = 28: one k3 plane per workgroup, 31 KB, five workgroups per CU; two planes at
+    // 61 KB fit two: N=1e6 9-digit evaluate() 9.06 -> 8.95 ms)
+    static constexpr int K3 = tf_slices(H, (P * P + NPW * OddStride<P>::v) * (int)sizeof(Cx<R>), 0,
+                                        (P == 28 && sizeof(R) == 8) ? 32 * 1024 : kTfLdsBytes);
     static constexpr int slices = (H + K3 - 1) / K3;
     // the y and x steps have K3 P and K3 n columns of n outputs for 256 threads; the outputs
     // of a column are split over SGY / SGX wavefronts when that fills more of them
