@@ -339,11 +339,11 @@ constexpr int tf_slices(int total, int per_slice, int reserved = 0, int budget =
 template <class R, int P, int NPW> struct P2pwCfg {
     static constexpr int H = (NPW + 1) / 2;
     // (FP64 at p = 28: one k3 plane per workgroup, 31 KB, five workgroups per CU; two planes at
-    // 61 KB fit two: N=1e6 9-digit evaluate() 9.06 -> 8.95 ms.  FP32 at p = 18: 4 planes at
-    // 27 KB instead of 9 at 60 KB, with the 32 KB budget of the plane-wave -> proxy side too:
-    // 6-digit evaluate() 3.06 -> 2.95 ms)
+    // 61 KB fit two: N=1e6 9-digit evaluate() 9.06 -> 8.95 ms.  At p = 18 a 32 KB budget, on
+    // the plane-wave -> proxy side too: FP32 4 planes instead of 9, 6-digit evaluate()
+    // 3.06 -> 2.95 ms; FP64 2 instead of 4, 4.72 -> 4.60 ms)
     static constexpr int K3 = tf_slices(H, (P * P + NPW * OddStride<P>::v) * (int)sizeof(Cx<R>), 0,
-                                        ((P == 28 && sizeof(R) == 8) || (P == 18 && sizeof(R) == 4)) ? 32 * 1024 : kTfLdsBytes);
+                                        ((P == 28 && sizeof(R) == 8) || P == 18) ? 32 * 1024 : kTfLdsBytes);
     static constexpr int slices = (H + K3 - 1) / K3;
     // the y and x steps have K3 P and K3 n columns of n outputs for 256 threads; the outputs
     // of a column are split over SGY / SGX wavefronts when that fills more of them
@@ -357,7 +357,7 @@ template <class R, int P, int NPW> struct Pw2pCfg {
     // (FP64 at p = 28: 2 slices of a per workgroup, 43 KB, so three workgroups share a CU;
     // 3 per workgroup at 65 KB fit two: N=1e6 9-digit evaluate() 9.12 -> 9.07 ms)
     static constexpr int AC = tf_slices(P, (H * NPW + H * OddStride<P>::v) * (int)sizeof(Cx<R>), 0,
-                                        (P == 28 && sizeof(R) == 8) ? 48 * 1024 : ((P == 18 && sizeof(R) == 4) ? 32 * 1024 : kTfLdsBytes));
+                                        (P == 28 && sizeof(R) == 8) ? 48 * 1024 : (P == 18 ? 32 * 1024 : kTfLdsBytes));
     static constexpr int slices = (P + AC - 1) / AC;
     // the y and z steps have AC H and AC P columns of P outputs for 256 threads; the outputs of
     // a column are split over SGY / SGZ wavefronts when that fills more of them
