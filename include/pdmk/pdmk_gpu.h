@@ -46,7 +46,10 @@ int pdmk_gpu_evaluate_float(void* h, const double* d_charges, float* d_pot, void
 /// overlap another use of that handle (its scratch is shared).  create returns NULL on a
 /// NULL or quasi-periodic handle, a NULL buffer, n_calls or n_branches <= 0 or a HIP error;
 /// launch forks the branches from `stream` (NULL = the graph's own stream), joins them back
-/// into it and returns; destroy waits for the graph's streams and frees it.
+/// into it and returns; destroy waits for the graph's streams and frees it.  A graph uses its
+/// handles' device memory: pdmk_destroy_* on a handle that a live graph uses only marks it,
+/// and the handle is freed when the last such graph is destroyed (no other call may use a
+/// handle after its destroy call).
 /// on != 0: the handle's evaluate() runs its stages one after another on the caller's stream
 /// instead of forking the far field and the near field onto the handle's own streams.  For
 /// callers that evaluate many trees at once: their trees, not the stages, then fill the GPU

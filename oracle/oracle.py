@@ -34,6 +34,18 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def use_native() -> str:
+    """Build (if needed) and load the -march=native variant of the oracle for this host
+    instead of the portable one; call before the first use of the oracle.  Used by the timed
+    CPU baseline, so that it runs at the host's full vector width."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("use_native() must come before the oracle library is loaded")
+    subprocess.run(["make", "-C", _HERE, "native"], check=True, stdout=subprocess.DEVNULL)
+    _LIB_PATH = os.path.join(_HERE, "_build", "native", "liboracle.so")
+    return _LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -35,6 +35,7 @@ SOURCES = [
     ("hip/ewald.hip", True),
     ("hip/tree_gpu.hip", True),
     ("hip/api.hip", True),
+    ("hip/costmodel.hip", True),
 ]
 
 

@@ -115,10 +115,23 @@ AutoPeriodicDMKTree make_periodic_dmk_tree(const Mat3 &cell, const std::vector<V
 namespace {
 
 // Handles carry a precision tag so that device-pointer entry points can check them.
+// A captured graph holds a handle's device buffers, streams and events: `graphs` counts the
+// live graphs that use it, and a destroy call while one lives is deferred (`dead`) until the
+// last of them is destroyed, so a graph never replays into freed memory.
 struct Handle {
     int is_float;
     void *tree;
+    int graphs = 0;
+    bool dead = false;
 };
+
+void free_handle(Handle *H) {
+    if (H->is_float)
+        delete static_cast<pdmk::PeriodicDMKTreeT<float, 3> *>(H->tree);
+    else
+        delete static_cast<pdmk::PeriodicDMKTreeT<double, 3> *>(H->tree);
+    delete H;
+}
 
 template <typename Real>
 pdmk::PeriodicDMKTreeT<Real, 3> *tree_of(void *h) {
@@ -196,8 +209,11 @@ template <typename Real>
 void destroy_impl(void *h) {
     if (!h) return;
     auto *H = static_cast<Handle *>(h);
-    delete static_cast<pdmk::PeriodicDMKTreeT<Real, 3> *>(H->tree);
-    delete H;
+    if (H->graphs > 0) {
+        H->dead = true;  // freed with the last graph that uses it
+        return;
+    }
+    free_handle(H);
 }
 
 // A captured list of evaluate() calls on device pointers.  Trees are dealt to n_branches
@@ -211,6 +227,7 @@ struct BatchGraph {
     std::vector<hipEvent_t> ev;  // ev[0]: fork; ev[1 + k]: join of branch k
     std::vector<hipGraph_t> graph;
     std::vector<hipGraphExec_t> exec;
+    std::vector<Handle *> users;  // distinct handles the graph captured (each counted once)
     ~BatchGraph() {
         if (origin) (void)hipStreamSynchronize(origin);
         for (hipStream_t b : branch)
@@ -224,6 +241,8 @@ struct BatchGraph {
         for (hipStream_t b : branch)
             if (b) (void)hipStreamDestroy(b);
         if (origin) (void)hipStreamDestroy(origin);
+        for (Handle *H : users)
+            if (--H->graphs == 0 && H->dead) free_handle(H);
     }
     void launch(hipStream_t s) {
         using pdmk::gpu::check;
@@ -257,6 +276,12 @@ BatchGraph *make_batch_graph(int n_calls, void *const *handles, const double *co
     while (!calls_of.empty() && calls_of.back().empty()) calls_of.pop_back();  // fewer trees than branches
     const int nb = (int)calls_of.size();
     auto G = std::make_unique<BatchGraph>();
+    for (const auto &kv : branch_of) {
+        auto *H = static_cast<Handle *>(kv.first);
+        if (H->dead) throw std::invalid_argument("destroyed handle in call list");
+        ++H->graphs;
+        G->users.push_back(H);
+    }
     check(hipStreamCreate(&G->origin), "hipStreamCreate");
     G->branch.assign(nb, nullptr);
     for (auto &b : G->branch) check(hipStreamCreate(&b), "hipStreamCreate");

@@ -51,7 +51,19 @@ int nc_for_digits(int d) {
 template <typename Real>
 Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user, int n_per_leaf, int nz,
                  const double *bloch_in) {
-    if (n_digits_user < 1 || n_digits_user > 12) throw std::invalid_argument("PeriodicDMK: n_digits must be in 1..12");
+    // eps = 10^-max(1, n_digits) (cpp/src/periodic_dmk.cpp:52): below 1 the tree is built
+    // for 1 digit; above 12 there are no tables
+    if (n_digits_user > 12) throw std::invalid_argument("PeriodicDMK: n_digits must be at most 12");
+    n_digits_user = std::max(1, n_digits_user);
+    // streams and events are released on any throw below (a destructor does not run for a
+    // partly built object; the DevBuf members free themselves)
+    struct HandleGuard {
+        Plan *pl;
+        bool armed = true;
+        ~HandleGuard() {
+            if (armed) pl->release_handles();
+        }
+    } guard{this};
     check(hipStreamCreate(&stream), "hipStreamCreate");
     for (auto &e : ev) check(hipEventCreate(&e), "hipEventCreate");
     check(hipStreamCreate(&s_far), "hipStreamCreate");
@@ -63,23 +75,14 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     const auto t_start = clk::now();
     host::Geometry g;
     host::Tree t;
-    try {
-        // per-point tree construction on the GPU; a tree deeper than its path keys hold is
-        // built with the points on the host (same result)
-        tree_on_gpu = build_tree_gpu(cell, pos, N, nz, n_per_leaf, stream, g, t);
-        if (!tree_on_gpu) {
-            g = host::make_geometry(cell, pos, N, nz);
-            t = host::build_tree(g, n_per_leaf);
-        }
-        build_ms[1] = ms_since(t_start);
-    } catch (...) {
-        for (auto &e : ev) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {e_in, e_gather, e_far, e_near, e_up}) (void)hipEventDestroy(e);
-        (void)hipStreamDestroy(s_far);
-        (void)hipStreamDestroy(s_near);
-        (void)hipStreamDestroy(stream);
-        throw;
+    // per-point tree construction on the GPU; a tree deeper than its path keys hold is
+    // built with the points on the host (same result)
+    tree_on_gpu = build_tree_gpu(cell, pos, N, nz, n_per_leaf, stream, g, t);
+    if (!tree_on_gpu) {
+        g = host::make_geometry(cell, pos, N, nz);
+        t = host::build_tree(g, n_per_leaf);
     }
+    build_ms[1] = ms_since(t_start);
     const host::DmkTables T = host::make_tables(n_digits_user, g.rc, t.nlevels, g.umat);
     n = N;
     rc = g.rc;
@@ -705,19 +708,32 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     trg_perm_host = t.trg_order;
     check(hipStreamSynchronize(stream), "plan build");
     build_ms[0] = ms_since(t_start);
+    guard.armed = false;
+}
+
+template <typename Real>
+void Plan<Real>::release_handles() {
+    for (hipStream_t st : {stream, s_far, s_near})
+        if (st) (void)hipStreamSynchronize(st);
+    for (auto e : kev) (void)hipEventDestroy(e);
+    kev.clear();
+    for (auto &e : ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    for (hipEvent_t *e : {&e_in, &e_gather, &e_far, &e_near, &e_up}) {
+        if (*e) (void)hipEventDestroy(*e);
+        *e = nullptr;
+    }
+    for (hipStream_t *st : {&stream, &s_far, &s_near}) {
+        if (*st) (void)hipStreamDestroy(*st);
+        *st = nullptr;
+    }
 }
 
 template <typename Real>
 Plan<Real>::~Plan() {
-    for (hipStream_t st : {stream, s_far, s_near})
-        if (st) (void)hipStreamSynchronize(st);
-    for (auto e : kev) (void)hipEventDestroy(e);
-    for (auto &e : ev)
-        if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : {e_in, e_gather, e_far, e_near, e_up})
-        if (e) (void)hipEventDestroy(e);
-    for (hipStream_t st : {stream, s_far, s_near})
-        if (st) (void)hipStreamDestroy(st);
+    release_handles();
 }
 
 namespace {

@@ -152,6 +152,7 @@ struct Plan {
     Plan(const double *cell_colmajor, const double *pos, int N, int n_digits, int n_per_leaf, int nz,
          const double *bloch = nullptr);
     ~Plan();
+    void release_handles();  // synchronize, then destroy the streams and events
     Plan(const Plan &) = delete;
     Plan &operator=(const Plan &) = delete;
 

@@ -295,6 +295,9 @@ Tree build_tree_with(const Geometry &g, int n_per_leaf, PointBackend &pts) {
     // refinement while a leaf holds more than ceil(1.5 * n_per_leaf) sources; the boxes of
     // one level are split together, then their points are distributed
     const int limit = (int)std::ceil(n_per_leaf * 1.5);
+    // integer box coordinates at level L reach mle * 2^L; children past 2^30 would overflow
+    // them (more than `limit` coincident points split without end)
+    const long long mle_max = std::max({t.mle[0], t.mle[1], t.mle[2]});
     int max_level = 0;
     for (int lvl = 0; lvl < 50; ++lvl) {
         if ((int)t.by_level.size() <= lvl) break;
@@ -302,6 +305,9 @@ Tree build_tree_with(const Geometry &g, int n_per_leaf, PointBackend &pts) {
         for (int b : t.by_level[lvl])
             if (t.first_child[b] < 0 && t.src_count[b] > limit) split.push_back(b);
         if (split.empty()) break;
+        if ((mle_max << (lvl + 1)) > (1LL << 30))
+            throw std::invalid_argument("PeriodicDMK: more than ceil(1.5 * n_per_leaf) coincident points; the tree "
+                                        "would exceed its depth limit");
         for (int b : split) B.make_children(b);
         pts.distribute(t, split);
         max_level = lvl + 1;

@@ -241,6 +241,17 @@ def test_edge_inputs(pdmk, oracle_mod, cuda):
         assert rel(s.evaluate(q).astype(np.float64), o.evaluate(q)) < tol
 
 
+def test_n_digits_below_one_builds_one_digit_tree(pdmk, cuda):
+    # eps = 10^-max(1, n_digits) (cpp/src/periodic_dmk.cpp:52): n_digits 0 or -3 is 1 digit
+    cell, pos, q, npl = case("cubic")
+    ref = pdmk.PDMKSolver(cell, pos, n_digits=1, n_per_leaf=npl, precision="double").evaluate(q)
+    for nd in (0, -3):
+        got = pdmk.PDMKSolver(cell, pos, n_digits=nd, n_per_leaf=npl, precision="double").evaluate(q)
+        np.testing.assert_array_equal(got, ref)
+    with pytest.raises(RuntimeError):
+        pdmk.PDMKSolver(cell, pos, n_digits=13, n_per_leaf=npl, precision="double")
+
+
 def test_wrong_charge_count_raises(pdmk, cuda):
     cell, pos, q, npl = case("cubic")
     s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, precision="double")

@@ -204,3 +204,14 @@ def test_optimize_cubic_cover_errors(pdmk):
             pdmk.optimize_cubic_cover(cell, nz)
     with pytest.raises(ValueError, match="code -1"):
         pdmk.optimize_cubic_cover(np.zeros((3, 3)), 4)
+
+
+def test_host_tree_refuses_endless_refinement(pdmk):
+    # more than ceil(1.5 * n_per_leaf) coincident points would split without end; the
+    # builder stops before its integer box coordinates overflow and reports the input
+    pos = np.full((40, 3), 0.3)
+    with pytest.raises(RuntimeError, match="pdmk_host_tree failed"):
+        pdmk.host_tree(np.eye(3), pos, 10, 4)
+    # at the limit (15 coincident points, n_per_leaf 10) nothing splits endlessly
+    h = pdmk.host_tree(np.eye(3), pos[:15], 10, 4)
+    assert h["sizes"][2] > 0
