@@ -94,6 +94,17 @@ int pdmk_evaluate_complex_float(void* h, int N, const double* positions_3xN, con
 /// pdmk_gpu_evaluate_stages (15 = everything).
 int pdmk_gpu_evaluate_complex(void* h, const double* d_charges, void* d_pot, int stages);
 
+/// pdmk_ewald_nufft without device allocation.  No counterpart in the reference: for callers
+/// that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).  The
+/// same arguments, potentials and energy (bit for bit, or to 1e-12 when the structure-factor
+/// partial sums are cut into fewer point tiles to fit); every device array is a slice of a
+/// static per-GPU scratch array of pdmk_ewald_noalloc_scratch_bytes() (128 MiB), so it
+/// returns -1 (size in the message) when N and the mode count need more.  Calls that overlap
+/// on one GPU share the array and race.
+int pdmk_ewald_nufft_noalloc(const double* cell_3x3, int N, const double* positions_3xN, const double* charges,
+                             double s, double* potentials_out, double* energy_out);
+unsigned long long pdmk_ewald_noalloc_scratch_bytes(void);
+
 /// FP64 Ewald potentials at M of the N points only, computed on the GPU: pot_out[t] is the
 /// potential at point target_indices[t] (an index into positions_3xN / charges, which are laid
 /// out as in pdmk_ewald_nufft), equal to pdmk_ewald_nufft's at that point.  Replaces

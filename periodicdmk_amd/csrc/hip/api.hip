@@ -337,6 +337,20 @@ int pdmk_ewald_nufft(const double *cell, int N, const double *pos, const double 
     }
 }
 
+int pdmk_ewald_nufft_noalloc(const double *cell, int N, const double *pos, const double *charges, double s,
+                             double *pot_out, double *energy_out) {
+    try {
+        if (!cell || !pos || !charges || !pot_out) throw std::invalid_argument("null argument");
+        pdmk::gpu::ewald_sum_noalloc(cell, N, pos, charges, s, pot_out, energy_out);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pdmk_ewald_nufft_noalloc: %s\n", e.what());
+        return -1;
+    }
+}
+
+unsigned long long pdmk_ewald_noalloc_scratch_bytes(void) { return pdmk::gpu::ewald_scratch_bytes(); }
+
 int pdmk_gpu_ewald_targets(const double *cell, int N, const double *pos, const double *charges, int M,
                            const int *target_indices, double s, double *pot_out) {
     try {

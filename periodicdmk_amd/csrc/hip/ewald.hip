@@ -21,7 +21,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 namespace pdmk::gpu {
@@ -281,7 +284,52 @@ struct StreamGuard {
 
 }  // namespace
 
-void ewald_sum(const double *cell, int n, const double *pos, const double *q, double s, double *pot, double *energy) {
+namespace {
+
+// Device arrays of one Ewald call: owned buffers, or slices of an arena.
+struct Store {
+    Arena *arena = nullptr;
+    std::vector<DevBuf> owned;
+    template <class T> T *put(const std::vector<T> &v) {
+        if (arena) return arena->put(v);
+        owned.emplace_back();
+        owned.back().upload(v);
+        return owned.back().template as<T>();
+    }
+    template <class T> T *make(size_t n) {
+        if (arena) return arena->take_n<T>(n);
+        owned.emplace_back();
+        owned.back().alloc(sizeof(T) * n);
+        return owned.back().template as<T>();
+    }
+};
+
+struct EwaldDev {
+    double4 *pts = nullptr, *nshift = nullptr;
+    int *bstart = nullptr, *nptr = nullptr, *nbin = nullptr, *sorted = nullptr;
+    double *q = nullptr, *real = nullptr, *far = nullptr, *phi = nullptr;
+};
+
+EwaldDev stage_ewald(Store &S, FourierSum &fs, const Bins &B, const Modes &M, const std::vector<double> &theta,
+                     const double *q, int n, int max_blocks) {
+    EwaldDev d;
+    d.pts = S.put(B.pts);
+    d.bstart = S.put(B.bstart);
+    d.nptr = S.put(B.nptr);
+    d.nbin = S.put(B.nbin);
+    d.nshift = S.put(B.nshift);
+    d.sorted = S.put(B.sorted_of);
+    d.q = S.put(std::vector<double>(q, q + n));
+    d.real = S.make<double>(n);
+    d.far = S.make<double>(n);
+    d.phi = S.make<double>(n);
+    fs.setup(theta, n, 2 * M.mmax[0] + 1, 2 * M.mmax[1] + 1, 2 * M.mmax[2] + 1, M.mijk, M.mw, S.arena, max_blocks);
+    return d;
+}
+
+// The Ewald sum with its device arrays owned (arena == nullptr) or carved from `arena`.
+void ewald_sum_in(const double *cell, int n, const double *pos, const double *q, double s, double *pot,
+                  double *energy, Arena *arena) {
     if (n <= 0) throw std::invalid_argument("N must be positive");
     if (!(s > 0)) throw std::invalid_argument("s must be positive");
     const Lat L = make_lat(cell);
@@ -293,40 +341,87 @@ void ewald_sum(const double *cell, int n, const double *pos, const double *q, do
     const Bins B = make_bins(L, frac, q, n, rcut);
     const Modes M = make_modes(L, alpha, kmax);
 
-    StreamGuard guard;
-    const hipStream_t st = guard.s;
-    DevBuf d_pts, d_bstart, d_nptr, d_nbin, d_nshift, d_real, d_far, d_q, d_sorted, d_phi;
-    d_pts.upload(B.pts);
-    d_bstart.upload(B.bstart);
-    d_nptr.upload(B.nptr);
-    d_nbin.upload(B.nbin);
-    d_nshift.upload(B.nshift);
-    d_sorted.upload(B.sorted_of);
-    d_q.upload(std::vector<double>(q, q + n));
-    d_real.alloc(sizeof(double) * (size_t)n);
-    d_far.alloc(sizeof(double) * (size_t)n);
-    d_phi.alloc(sizeof(double) * (size_t)n);
+    int max_blocks = 2048;
+    if (arena) {
+        // size the request before touching the memory: everything but the structure factor
+        // partial sums, then as many partial-sum blocks (point tiles) as the rest holds
+        Arena dry;
+        Store ds{&dry, {}};
+        FourierSum f0;
+        stage_ewald(ds, f0, B, M, theta, q, n, 1);
+        const size_t per_block = sizeof(double2) * (size_t)std::max<size_t>(1, M.mw.size());
+        const size_t fixed = dry.used - per_block + 256;
+        if (fixed > arena->cap || (arena->cap - fixed) / per_block < 1)
+            throw std::length_error("Ewald sum of N=" + std::to_string(n) + " points and " + std::to_string(M.mw.size()) +
+                                    " modes needs " + std::to_string(fixed + per_block) +
+                                    " bytes of device scratch, more than the " + std::to_string(arena->cap) +
+                                    " available");
+        max_blocks = (int)std::min<size_t>(2048, (arena->cap - fixed) / per_block);
+        arena->used = 0;
+    }
+    Store S{arena, {}};
     FourierSum fs;
-    fs.setup(theta, n, 2 * M.mmax[0] + 1, 2 * M.mmax[1] + 1, 2 * M.mmax[2] + 1, M.mijk, M.mw);
-    fs.run(d_q.as<double>(), d_far.as<double>(), st);
+    const EwaldDev d = stage_ewald(S, fs, B, M, theta, q, n, max_blocks);
+
+    // the arena path uses the null stream (nothing to create)
+    std::unique_ptr<StreamGuard> guard;
+    if (!arena) guard = std::make_unique<StreamGuard>();
+    const hipStream_t st = guard ? guard->s : nullptr;
+    fs.run(d.q, d.far, st);
     const dim3 grid(B.nbins, (B.max_occ + EW_THREADS - 1) / EW_THREADS);
-    hipLaunchKernelGGL(k_ewald_real, grid, dim3(EW_THREADS), 0, st, d_pts.as<double4>(), d_bstart.as<int>(),
-                       d_nptr.as<int>(), d_nbin.as<int>(), d_nshift.as<double4>(), alpha, rcut * rcut,
-                       d_real.as<double>());
+    hipLaunchKernelGGL(k_ewald_real, grid, dim3(EW_THREADS), 0, st, d.pts, d.bstart, d.nptr, d.nbin, d.nshift, alpha,
+                       rcut * rcut, d.real);
     check(hipGetLastError(), "k_ewald_real");
     double Q = 0;
     for (int i = 0; i < n; ++i) Q += q[i];
     const double shift = M_PI * Q / L.vol * (1 / (alpha_ref * alpha_ref) - 1 / (alpha * alpha));
-    hipLaunchKernelGGL(k_ewald_finalize, dim3((n + 255) / 256), dim3(256), 0, st, d_real.as<double>(), d_sorted.as<int>(),
-                       d_far.as<double>(), d_q.as<double>(), 2 * alpha / std::sqrt(M_PI), shift, d_phi.as<double>(), n);
+    hipLaunchKernelGGL(k_ewald_finalize, dim3((n + 255) / 256), dim3(256), 0, st, d.real, d.sorted, d.far, d.q,
+                       2 * alpha / std::sqrt(M_PI), shift, d.phi, n);
     check(hipGetLastError(), "k_ewald_finalize");
-    check(hipMemcpyAsync(pot, d_phi.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+    check(hipMemcpyAsync(pot, d.phi, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (energy) {
         double e = 0;
         for (int i = 0; i < n; ++i) e += q[i] * pot[i];
         *energy = 0.5 * e - M_PI * Q * Q / (2 * L.vol * alpha_ref * alpha_ref);
     }
+}
+
+// Device scratch of the allocation-free Ewald entry point: one copy per GPU, held until the
+// process ends.  Host code uses the per-device address hipGetSymbolAddress returns, never
+// the array's name (a host-side stand-in).
+constexpr size_t kEwaldScratchBytes = size_t(128) << 20;
+__device__ __align__(256) char ewald_scratch[kEwaldScratchBytes];
+
+char *ewald_scratch_ptr() {
+    static std::mutex mu;
+    static std::vector<char *> ptr;  // per device
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(ewald_scratch)), "hipGetSymbolAddress(ewald_scratch)");
+        ptr[dev] = static_cast<char *>(p);
+    }
+    return ptr[dev];
+}
+
+}  // namespace
+
+void ewald_sum(const double *cell, int n, const double *pos, const double *q, double s, double *pot, double *energy) {
+    ewald_sum_in(cell, n, pos, q, s, pot, energy, nullptr);
+}
+
+size_t ewald_scratch_bytes() { return kEwaldScratchBytes; }
+
+void ewald_sum_noalloc(const double *cell, int n, const double *pos, const double *q, double s, double *pot,
+                       double *energy) {
+    Arena a;
+    a.base = ewald_scratch_ptr();
+    a.cap = kEwaldScratchBytes;
+    ewald_sum_in(cell, n, pos, q, s, pot, energy, &a);
 }
 
 double ewald_targets_alpha(const double *cell, int n, const double *pos, int m, const int *targets, double s) {

@@ -1,6 +1,8 @@
 // FP64 Ewald sum on the GPU (the computation behind pdmk_ewald_nufft).
 #pragma once
 
+#include <cstddef>
+
 namespace pdmk::gpu {
 
 // Potentials pot[N] (and the energy when energy != nullptr) of N charges q at positions
@@ -8,6 +10,13 @@ namespace pdmk::gpu {
 // s sets the truncation (r_cut = s/alpha, k_max = 2 s alpha).  Synchronous; throws on bad
 // arguments or HIP errors.
 void ewald_sum(const double *cell, int n, const double *pos, const double *q, double s, double *pot, double *energy);
+
+// The same sum with every device array in a static per-device scratch array of
+// ewald_scratch_bytes() (no device allocation): throws std::length_error when N and the mode
+// count need more.  Calls that overlap on one GPU race (they share the array).
+void ewald_sum_noalloc(const double *cell, int n, const double *pos, const double *q, double s, double *pot,
+                       double *energy);
+size_t ewald_scratch_bytes();
 
 // Potentials pot[M] at the points targets[0 .. M) (indices into the N points) only
 // (ewald_nufft_targets, cpp/src/ewald_nufft.cpp:663-759): the real-space sum is formed at the M

@@ -36,6 +36,16 @@ void DevBuf::alloc(size_t nbytes) {
     if (nbytes) check(hipMalloc(&p, nbytes), "hipMalloc");
 }
 
+void *Arena::take(size_t nbytes) {
+    const size_t off = (used + 255) & ~size_t(255);
+    if (!dry() && off + nbytes > cap)
+        throw std::length_error("device scratch exhausted: " + std::to_string(off + nbytes) + " bytes needed, " +
+                                std::to_string(cap) + " available");
+    used = off + nbytes;
+    return dry() ? nullptr : base + off;
+}
+
+
 namespace {
 
 constexpr size_t kPinBytes = size_t(512) << 20;  // incoming plane-wave scratch cap
@@ -1057,14 +1067,21 @@ void Plan<Real>::evaluate_complex_host(const double *q, Real *out) {
 }
 
 void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
-                       const std::vector<int> &M, const std::vector<double> &mode_w) {
+                       const std::vector<int> &M, const std::vector<double> &mode_w, Arena *arena, int max_blocks) {
     n = n_points;
     ms = ms_;
     mt = mt_;
     mu = mu_;
     n_modes = (int)mode_w.size();
-    theta.upload(theta_host);
-    w.upload(mode_w);
+    // an array in its owned buffer, or in a slice of the arena
+    auto put = [&](DevBuf &own, const auto &v) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        if (arena) return static_cast<const T *>(arena->put(v));
+        own.upload(v);
+        return own.template as<const T>();
+    };
+    theta_d = put(theta, theta_host);
+    w_d = put(w, mode_w);
     // lines of constant (j, k) and consecutive i, cut into chunks of FF_CHUNK modes
     std::vector<int> cijk, cm0, clen, ljk, li, lptr;
     for (int m = 0; m < n_modes;) {
@@ -1087,9 +1104,9 @@ void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int 
     }
     n_lines = (int)lptr.size();
     n_chunks = (int)cm0.size();
-    chunk_ijk.upload(cijk);
-    chunk_m0.upload(cm0);
-    chunk_len.upload(clen);
+    chunk_ijk_d = put(chunk_ijk, cijk);
+    chunk_m0_d = put(chunk_m0, cm0);
+    chunk_len_d = put(chunk_len, clen);
     // weighted structure factors per line, padded at the high-i end to a multiple of
     // FF_QUAD modes (zeros), so the evaluation reads them FF_QUAD at a time
     std::vector<int4> lv;
@@ -1102,17 +1119,25 @@ void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int 
         for (int r = 0; r < len; ++r) pix[lptr[L] + r] = npad + r;
         npad += nq * FF_QUAD;
     }
-    lines.upload(lv);
-    line_base.upload(lbase);
-    pidx.upload(pix);
-    blocks = std::max(1, std::min(2048, (n + FF_TP - 1) / FF_TP));
-    Sp.alloc(sizeof(double2) * (size_t)blocks * std::max(1, n_modes));
-    S.alloc(sizeof(double2) * (size_t)std::max(1, npad));
-    check(hipMemset(S.p, 0, S.bytes), "hipMemset");  // padding stays zero
+    lines_d = put(lines, lv);
+    line_base_d = put(line_base, lbase);
+    pidx_d = put(pidx, pix);
+    blocks = std::max(1, std::min(std::min(2048, max_blocks), (n + FF_TP - 1) / FF_TP));
+    const size_t sp_n = (size_t)blocks * std::max(1, n_modes), s_n = (size_t)std::max(1, npad);
+    if (arena) {
+        Sp_d = arena->take_n<double2>(sp_n);
+        S_d = arena->take_n<double2>(s_n);
+    } else {
+        Sp.alloc(sizeof(double2) * sp_n);
+        S.alloc(sizeof(double2) * s_n);
+        Sp_d = Sp.as<double2>();
+        S_d = S.as<double2>();
+    }
+    if (S_d) check(hipMemset(S_d, 0, sizeof(double2) * s_n), "hipMemset");  // padding stays zero
 }
 
 void FourierSum::run(const double *d_q, double *d_out, hipStream_t s) const {
-    run_at(d_q, theta.as<double>(), n, d_out, s);
+    run_at(d_q, theta_d, n, d_out, s);
 }
 
 void FourierSum::run_at(const double *d_q, const double *d_theta_eval, int n_eval, double *d_out, hipStream_t s) const {
@@ -1124,15 +1149,13 @@ void FourierSum::run_at(const double *d_q, const double *d_theta_eval, int n_eva
     }
     const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1);
     dim3 grid(blocks, (n_chunks + 255) / 256);
-    hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, theta.as<double>(), d_q, n, ms, mt, mu,
-                       chunk_ijk.as<int>(), chunk_m0.as<int>(), chunk_len.as<int>(), n_chunks, n_modes,
-                       Sp.as<double2>());
+    hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, theta_d, d_q, n, ms, mt, mu, chunk_ijk_d, chunk_m0_d,
+                       chunk_len_d, n_chunks, n_modes, Sp_d);
     check(hipGetLastError(), "k_ff_structure");
-    hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32), dim3(256), 0, s, Sp.as<double2>(), blocks, w.as<double>(),
-                       pidx.as<int>(), n_modes, S.as<double2>());
+    hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32), dim3(256), 0, s, Sp_d, blocks, w_d, pidx_d, n_modes, S_d);
     check(hipGetLastError(), "k_ff_reduce");
-    hipLaunchKernelGGL(k_ff_eval, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_theta_eval, n_eval, lines.as<int4>(),
-                       line_base.as<int>(), n_lines, S.as<double2>(), d_out);
+    hipLaunchKernelGGL(k_ff_eval, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_theta_eval, n_eval, lines_d,
+                       line_base_d, n_lines, S_d, d_out);
     check(hipGetLastError(), "k_ff_eval");
 }
 

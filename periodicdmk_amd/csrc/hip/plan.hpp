@@ -38,6 +38,24 @@ struct DevBuf {
     }
 };
 
+// Bump allocation over device memory the caller provides (256-byte aligned slices).  In a
+// dry run (base == nullptr) it only counts the bytes, returns null pointers and copies
+// nothing, so a caller can size a request before it touches the memory.  take() throws
+// std::length_error past the capacity.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    bool dry() const { return base == nullptr; }
+    void *take(size_t nbytes);
+    template <class T> T *take_n(size_t n) { return static_cast<T *>(take(sizeof(T) * n)); }
+    // synchronous host -> device copy into a fresh slice
+    template <class T> T *put(const std::vector<T> &v) {
+        T *d = take_n<T>(v.size());
+        if (!dry() && !v.empty()) check(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        return d;
+    }
+};
+
 struct Stats {
     long long n_src = 0, nsall = 0, nboxes = 0, n_levels = 0, p = 0, n_pw = 0, ms = 0, mt = 0, mu = 0;
     long long n_leaves = 0, n_list1 = 0, n_direct_leaves = 0, n_far_modes = 0, n_digits = 0, is_float = 0,
@@ -53,11 +71,20 @@ struct Work {
 //   out[u] = sum_{m in the half space} Re( w_m S(m) e^{+i m.theta_u} ),  S(m) = sum_u q_u e^{-i m.theta_u},
 // theta = 2 pi x (fractional coordinates, wrapped to [0, 1)).  Modes (i, j, k) come in lines
 // of constant (j, k) with consecutive i; index ranges are [-m/2, (m-1)/2] for m = ms, mt, mu.
+// Its arrays are owned buffers, or slices of an Arena when setup() is given one.  The
+// structure factors are summed in `blocks` partial sums of point tiles (at most max_blocks),
+// then added in block order.
 struct FourierSum {
     int n = 0, ms = 0, mt = 0, mu = 0, n_modes = 0, n_chunks = 0, n_lines = 0, blocks = 0;
     DevBuf theta, w, chunk_ijk, chunk_m0, chunk_len, lines, line_base, pidx, Sp, S;
+    const double *theta_d = nullptr, *w_d = nullptr;
+    const int *chunk_ijk_d = nullptr, *chunk_m0_d = nullptr, *chunk_len_d = nullptr, *line_base_d = nullptr,
+              *pidx_d = nullptr;
+    const int4 *lines_d = nullptr;
+    double2 *Sp_d = nullptr, *S_d = nullptr;
     void setup(const std::vector<double> &theta_host, int n_points, int ms_, int mt_, int mu_,
-               const std::vector<int> &mode_ijk, const std::vector<double> &mode_w);
+               const std::vector<int> &mode_ijk, const std::vector<double> &mode_w, Arena *arena = nullptr,
+               int max_blocks = 2048);
     void run(const double *d_q, double *d_out, hipStream_t s) const;  // enqueue only
     // S(m) from the setup points, then out[t] at n_eval other points (theta_eval[3t + d],
     // device, same convention); enqueue only
