@@ -120,13 +120,37 @@ int pdmk_gpu_ewald_targets(const double* cell_3x3, int N, const double* position
 /// is_float, n_nf_steps}.
 int pdmk_gpu_stats(void* h, long long* out16);
 
+/// Autotuner of the launch shapes (project-written; the reference has no GPU kernels to tune):
+/// times `repeats` evaluate() calls (after one warm-up) on the handle for each candidate of
+/// each knob in turn -- near-field wavefronts per workgroup {4, 8, 16}, near-field start
+/// (at once / after the upward pass), proxy -> plane-wave and plane-wave -> proxy LDS
+/// budgets {16, 32, 64} KiB (slices per workgroup), tensor-transform planes per workgroup
+/// (halved or not) -- keeping a change only when it is at least 2% faster, and applies the
+/// result.  The choice is cached per device and plan kind (order, precision, log2 buckets of
+/// N, proxy slots and near-field items) for the process, and appended to the file
+/// PDMK_TUNE_CACHE names when set; new handles of a cached kind start with it.  Potentials do
+/// not depend on the knobs beyond FP rounding.  out8 = {ms before (built-in rules), ms after,
+/// the 5 knobs as in pdmk_gpu_knobs, 1 if the handle started from a cached choice}.  -1 on a
+/// NULL handle or a HIP error.
+int pdmk_gpu_autotune(void* h, int repeats, double* out8);
+/// Knobs of a handle: set5 (NULL = unchanged) = {near-field wavefronts per workgroup (0 = rule,
+/// 2, 4, 8, 16), near-field late start (-1 = rule, 0, 1), proxy -> plane-wave LDS KiB (0 = rule,
+/// 16, 32, 64), plane-wave -> proxy LDS KiB (same), tensor-transform plane divisor (1, 2, 4)};
+/// get5 (NULL = not read) receives the current ones.  -1 on a NULL handle or a value outside
+/// those sets.
+int pdmk_gpu_knobs(void* h, const int* set5, int* get5);
+/// Forget this process's tuned choices (handles built afterwards start with the built-in
+/// shapes; the PDMK_TUNE_CACHE file is not read again).
+void pdmk_gpu_tune_cache_clear(void);
+
 /// Construction of the handle: out[0..3] = {total ms, tree ms (geometry, images, tree),
 /// far-field plan ms, 1 if the tree's per-point work (images, binning, distribution, sort)
 /// ran on the GPU else 0, boxes folded onto a periodic counterpart (their proxies and
 /// outgoing plane waves are not computed), proxy slots computed, 1 if the outgoing plane
-/// waves are held one level at a time (when all of them do not fit in device memory) else 0}.
+/// waves are held one level at a time (when all of them do not fit in device memory) else 0,
+/// 1 if the near field walks per-row step lists (k_near_rows: FP32 plans at 6 digits) else 0}.
 /// No reference counterpart (measurement hook).
-int pdmk_gpu_build_info(void* h, double* out7);
+int pdmk_gpu_build_info(void* h, double* out8);
 
 /// Algorithmic work per evaluate(): out[0..7] = {near-field pairs in support,
 /// near-field candidate pairs, near-field flops (in-support pairs x (2*NC+12)),

@@ -75,6 +75,9 @@ def lib():
         "pdmk_evaluate_complex_float": (C.c_int, [vp, C.c_int, dp, dp, C.POINTER(C.c_float)]),
         "pdmk_gpu_evaluate_complex": (C.c_int, [vp, vp, vp, C.c_int]),
         "pdmk_gpu_build_info": (C.c_int, [vp, dp]),
+        "pdmk_gpu_autotune": (C.c_int, [vp, C.c_int, dp]),
+        "pdmk_gpu_knobs": (C.c_int, [vp, ip, ip]),
+        "pdmk_gpu_tune_cache_clear": (None, []),
         "pdmk_gpu_graph_create": (vp, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
         "pdmk_gpu_graph_launch": (C.c_int, [vp, vp]),
         "pdmk_gpu_set_single_stream": (C.c_int, [vp, C.c_int]),
@@ -360,10 +363,35 @@ class PDMKSolver:
 
     def build_info(self):
         """Construction times (ms) and whether the tree's per-point work ran on the GPU."""
-        o = np.zeros(7)
+        o = np.zeros(8)
         lib().pdmk_gpu_build_info(self._h, _d(o))
         return {"build_ms": float(o[0]), "tree_ms": float(o[1]), "far_plan_ms": float(o[2]), "tree_on_gpu": bool(o[3]),
-                "folded_boxes": int(o[4]), "proxy_slots": int(o[5]), "pw_by_level": bool(o[6])}
+                "folded_boxes": int(o[4]), "proxy_slots": int(o[5]), "pw_by_level": bool(o[6]),
+                "near_rows": bool(o[7])}
+
+    KNOBS = ("near_wpb", "near_late", "p2pw_bk", "pw2p_bk", "t3_div")
+
+    def autotune(self, repeats: int = 3):
+        """Time the launch-shape candidates on this tree and apply the fastest
+        (pdmk_gpu_autotune); returns ms before and after and the chosen knobs."""
+        o = np.zeros(8)
+        if lib().pdmk_gpu_autotune(self._h, int(repeats), _d(o)) != 0:
+            raise RuntimeError("pdmk_gpu_autotune failed (see stderr)")
+        return {"ms_default": float(o[0]), "ms_tuned": float(o[1]),
+                "knobs": dict(zip(self.KNOBS, (int(v) for v in o[2:7]))), "from_cache": bool(o[7])}
+
+    def knobs(self, **set_values):
+        """Current launch-shape knobs; keyword arguments set them first (pdmk_gpu_knobs)."""
+        cur = np.zeros(5, dtype=np.int32)
+        if lib().pdmk_gpu_knobs(self._h, None, _i(cur)) != 0:
+            raise RuntimeError("pdmk_gpu_knobs failed")
+        if set_values:
+            new = dict(zip(self.KNOBS, (int(v) for v in cur)))
+            new.update(set_values)
+            arr = np.array([new[k] for k in self.KNOBS], dtype=np.int32)
+            if lib().pdmk_gpu_knobs(self._h, _i(arr), _i(cur)) != 0:
+                raise ValueError(f"pdmk_gpu_knobs rejected {set_values}")
+        return dict(zip(self.KNOBS, (int(v) for v in cur)))
 
     def work(self):
         o = np.zeros(8)

@@ -8,7 +8,10 @@ kernels).  Training and inference run on the GPU in the library's own kernels
 (``pdmk_mlp_train``, ``pdmk_mlp_forward``; csrc/hip/costmodel.hip).  ``choose`` returns the
 configuration predicted fastest among those whose predicted error is at most twice the
 default configuration's (n_per_leaf 300, nz 4, FP32 kernels up to 6 digits as
-AutoPeriodicDMKTree); without a trained model it returns the defaults.
+AutoPeriodicDMKTree), provided its predicted time is below 0.85 x the default's (else the
+defaults); without a trained model it returns the defaults.
+
+The solver-timed solves it was trained on are kept in ``cost_model_solves/`` (JSON lines).
 
 The model ships as ``cost_model.json`` beside this file: the input and output
 standardisation (means and standard deviations) and the flat parameter array.
@@ -40,6 +43,8 @@ N_IN, N_OUT = 7, 2
 DEFAULT = (300, 4)
 NPL_CHOICES = (100, 150, 200, 300, 400, 600, 800)
 NZ_CHOICES = tuple(range(2, 9))
+GAIN_MARGIN = 0.85  # leave the defaults only for a predicted time below 0.85 x theirs
+DATA_DIR = os.path.join(HERE, "cost_model_solves")
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -141,7 +146,7 @@ def targets(ms, err) -> np.ndarray:
     return np.array([np.log10(ms), np.log10(max(err, 1e-16))])
 
 
-def fit(rows, steps: int = 4000, seed: int = 0, lr: float = 2e-3):
+def fit(rows, steps: int = 800, seed: int = 0, lr: float = 1e-3):
     """Train on solve records (dicts with features, npl, nz, is_float, ms, err): returns the
     model blob (standardisation + parameters) and the loss history."""
     X = np.array([config_inputs(r["features"], r["npl"], r["nz"], r["is_float"]) for r in rows])
@@ -194,6 +199,8 @@ def choose(cell, positions, n_digits, blob=None, path=MODEL_PATH, precision=None
         ms, err = predict(blob, f4, cands)
         ok = [i for i in range(len(cands)) if err[i] <= 2.0 * err[0]]
         b = min(ok, key=lambda i: ms[i])
+        if ms[b] > GAIN_MARGIN * ms[0]:
+            b = 0
         return {"n_per_leaf": cands[b][0], "nz": cands[b][1], "precision": precision, "model": True,
                 "pred_ms": float(ms[b]), "pred_err": float(err[b]), "pred_default_ms": float(ms[0]),
                 "pred_default_err": float(err[0]), "features": f4.tolist()}
@@ -346,7 +353,8 @@ def main(argv=None):
     g.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
-    t.add_argument("--steps", type=int, default=4000)
+    t.add_argument("--steps", type=int, default=800)
+    t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--out", default=MODEL_PATH)
     h = sub.add_parser("heldout")
     h.add_argument("--problems", type=int, default=200)
@@ -357,8 +365,9 @@ def main(argv=None):
         generate(a.problems, a.seed, a.out, a.configs, a.budget_s)
     elif a.cmd == "train":
         rows = read_records(a.data)
-        blob, hist = fit(rows, a.steps)
-        save_model(blob, a.out, {"solves": len(rows), "steps": a.steps, "final_loss": float(hist[-1])})
+        blob, hist = fit(rows, a.steps, lr=a.lr)
+        save_model(blob, a.out, {"solves": len(rows), "steps": a.steps, "lr": a.lr, "final_loss": float(hist[-1]),
+                                 "data": [os.path.basename(d) for d in a.data]})
         print(f"[costmodel] trained on {len(rows)} solves: loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
     else:
         heldout(a.problems, a.seed, a.out)

@@ -560,6 +560,53 @@ int pdmk_gpu_stats(void *h, long long *o) {
     return 0;
 }
 
+int pdmk_gpu_autotune(void *h, int repeats, double *out8) {
+    if (!h) return -1;
+    return guarded("pdmk_gpu_autotune", [&] {
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        auto run = [&](auto &pl) {
+            pl.autotune(repeats, &v[0], &v[1]);
+            const pdmk::gpu::Knobs &k = pl.knobs;
+            v[2] = k.near_wpb;
+            v[3] = k.near_late;
+            v[4] = k.p2pw_bk;
+            v[5] = k.pw2p_bk;
+            v[6] = k.t3_div;
+            v[7] = pl.knobs_cached ? 1 : 0;
+        };
+        if (static_cast<Handle *>(h)->is_float) run(tree_of<float>(h)->plan());
+        else run(tree_of<double>(h)->plan());
+        if (out8) std::memcpy(out8, v, sizeof(v));
+        return 0;
+    });
+}
+
+void pdmk_gpu_tune_cache_clear(void) { pdmk::gpu::tune_cache_clear(); }
+
+int pdmk_gpu_knobs(void *h, const int *set5, int *get5) {
+    if (!h) return -1;
+    return guarded("pdmk_gpu_knobs", [&] {
+        auto apply = [&](auto &pl) {
+            pdmk::gpu::Knobs &k = pl.knobs;
+            if (set5) {
+                const int wpb = set5[0], late = set5[1], b1 = set5[2], b2 = set5[3], div = set5[4];
+                auto ok_bk = [](int b) { return b == 0 || b == 16 || b == 32 || b == 64; };
+                if (!(wpb == 0 || wpb == 2 || wpb == 4 || wpb == 8 || wpb == 16) || late < -1 || late > 1 ||
+                    !ok_bk(b1) || !ok_bk(b2) || !(div == 1 || div == 2 || div == 4))
+                    throw std::invalid_argument("knob value outside its candidates");
+                k = pdmk::gpu::Knobs{wpb, late, b1, b2, div};
+            }
+            if (get5) {
+                const int v[5] = {k.near_wpb, k.near_late, k.p2pw_bk, k.pw2p_bk, k.t3_div};
+                std::memcpy(get5, v, sizeof(v));
+            }
+        };
+        if (static_cast<Handle *>(h)->is_float) apply(tree_of<float>(h)->plan());
+        else apply(tree_of<double>(h)->plan());
+        return 0;
+    });
+}
+
 int pdmk_gpu_build_info(void *h, double *o) {
     if (!h || !o) return -1;
     const bool f = static_cast<Handle *>(h)->is_float;
@@ -568,7 +615,9 @@ int pdmk_gpu_build_info(void *h, double *o) {
     const int folded = f ? tree_of<float>(h)->plan().n_folded : tree_of<double>(h)->plan().n_folded;
     const int slots = f ? tree_of<float>(h)->plan().n_up : tree_of<double>(h)->plan().n_up;
     const bool by_level = f ? tree_of<float>(h)->plan().pw_by_level : tree_of<double>(h)->plan().pw_by_level;
-    const double v[7] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0, (double)folded, (double)slots, by_level ? 1.0 : 0.0};
+    const bool rows = f ? tree_of<float>(h)->plan().nf_rows : tree_of<double>(h)->plan().nf_rows;
+    const double v[8] = {ms[0], ms[1], ms[2], gpu ? 1.0 : 0.0, (double)folded, (double)slots, by_level ? 1.0 : 0.0,
+                         rows ? 1.0 : 0.0};
     std::memcpy(o, v, sizeof(v));
     return 0;
 }

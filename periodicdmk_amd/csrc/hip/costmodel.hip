@@ -227,6 +227,7 @@ constexpr int kBlobHeader = 2 * kIn + 2 * kOut;
 constexpr int kNplChoices[] = {100, 150, 200, 300, 400, 600, 800};
 constexpr int kNzMin = 2, kNzMax = 8;
 constexpr int kDefaultNpl = 300, kDefaultNz = 4;
+constexpr double kGainMargin = 0.85;  // leave the defaults only for a predicted time below 0.85 x theirs
 
 struct Candidate { int npl, nz, is_float; };
 
@@ -326,8 +327,9 @@ int pdmk_tune_features(const double *cell, int N, const double *pos, int n_digit
 }
 
 // The configuration the model predicts fastest among those whose predicted error is at most
-// twice the default configuration's: out3 = (n_per_leaf, nz, is_float); pred4 (optional) =
-// predicted (ms, error) of the choice and of the default.  Returns the number of candidates.
+// twice the default configuration's, kept only when its predicted time is below 0.85 x the
+// default's: out3 = (n_per_leaf, nz, is_float); pred4 (optional) = predicted (ms, error) of the
+// choice and of the default.  Returns the number of candidates.
 int pdmk_tune_choose(const float *blob, int blob_len, const double *f4, int *out3, double *pred4) {
     using namespace pdmk::gpu;
     int n_cand = -1;
@@ -351,6 +353,7 @@ int pdmk_tune_choose(const float *blob, int blob_len, const double *f4, int *out
         int best = 0;
         for (int b = 1; b < B; ++b)
             if (log_err(b) <= err_cap && log_ms(b) < log_ms(best)) best = b;
+        if (log_ms(best) > log_ms(0) + std::log10(kGainMargin)) best = 0;
         out3[0] = C[best].npl;
         out3[1] = C[best].nz;
         out3[2] = C[best].is_float;

@@ -336,14 +336,16 @@ constexpr int tf_slices(int total, int per_slice, int reserved = 0, int budget =
 
 // Slices of the proxy <-> plane-wave transforms that fit the LDS budget; each slice is a
 // workgroup of its own (grid y), so a level with few boxes still fills the GPU.
-template <class R, int P, int NPW> struct P2pwCfg {
+// BK: the LDS budget in KiB per workgroup (0 = the default below); the autotuner picks
+// among 0, 16, 32 and 64 per plan.
+template <class R, int P, int NPW, int BK = 0> struct P2pwCfg {
     static constexpr int H = (NPW + 1) / 2;
     // (FP64 at p = 28: one k3 plane per workgroup, 31 KB, five workgroups per CU; two planes at
     // 61 KB fit two: N=1e6 9-digit evaluate() 9.06 -> 8.95 ms.  At p = 18 a 32 KB budget, on
     // the plane-wave -> proxy side too: FP32 4 planes instead of 9, 6-digit evaluate()
     // 3.06 -> 2.95 ms; FP64 2 instead of 4, 4.72 -> 4.60 ms)
     static constexpr int K3 = tf_slices(H, (P * P + NPW * OddStride<P>::v) * (int)sizeof(Cx<R>), 0,
-                                        ((P == 28 && sizeof(R) == 8) || P == 18) ? 32 * 1024 : kTfLdsBytes);
+                                        BK ? BK * 1024 : (((P == 28 && sizeof(R) == 8) || P == 18) ? 32 * 1024 : kTfLdsBytes));
     static constexpr int slices = (H + K3 - 1) / K3;
     // the y and x steps have K3 P and K3 n columns of n outputs for 256 threads; the outputs
     // of a column are split over SGY / SGX wavefronts when that fills more of them
@@ -352,12 +354,12 @@ template <class R, int P, int NPW> struct P2pwCfg {
     static constexpr int SGY = cap(NT / ((K3 * P + 63) / 64 * 64)), SGX = cap(NT / ((K3 * NPW + 63) / 64 * 64));
     static constexpr int PSY = (NPW + SGY - 1) / SGY, PSX = (NPW + SGX - 1) / SGX;
 };
-template <class R, int P, int NPW> struct Pw2pCfg {
+template <class R, int P, int NPW, int BK = 0> struct Pw2pCfg {
     static constexpr int H = (NPW + 1) / 2;
     // (FP64 at p = 28: 2 slices of a per workgroup, 43 KB, so three workgroups share a CU;
     // 3 per workgroup at 65 KB fit two: N=1e6 9-digit evaluate() 9.12 -> 9.07 ms)
     static constexpr int AC = tf_slices(P, (H * NPW + H * OddStride<P>::v) * (int)sizeof(Cx<R>), 0,
-                                        (P == 28 && sizeof(R) == 8) ? 48 * 1024 : (P == 18 ? 32 * 1024 : kTfLdsBytes));
+                                        BK ? BK * 1024 : ((P == 28 && sizeof(R) == 8) ? 48 * 1024 : (P == 18 ? 32 * 1024 : kTfLdsBytes)));
     static constexpr int slices = (P + AC - 1) / AC;
     // the y and z steps have AC H and AC P columns of P outputs for 256 threads; the outputs of
     // a column are split over SGY / SGZ wavefronts when that fills more of them
@@ -518,15 +520,15 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3), 1-D grid, (box, slice) at
 // xcd_block index box * slices + slice (a box's slices share one L2).  Box x reads proxy
 // slot slot0 + x and writes plane-wave slot x.
-template <class R, int P, int NPW>
+template <class R, int P, int NPW, int BK = 0>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
                                                    const Cx<R> *__restrict__ W, int slot0, int n_boxes) {
     chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, TS = OddStride<P>::v;
-    constexpr int K3 = P2pwCfg<R, P, NPW>::K3;
+    constexpr int K3 = P2pwCfg<R, P, NPW, BK>::K3;
     __shared__ Cx<R> T1[K3 * PP];        // [k][b][a]
     __shared__ Cx<R> T2[K3 * NPW * TS];  // [k][k2][a] rows of TS
-    constexpr int NS = P2pwCfg<R, P, NPW>::slices;
+    constexpr int NS = P2pwCfg<R, P, NPW, BK>::slices;
     const int lg = xcd_block(blockIdx.x, gridDim.x), bx = lg / NS, slice = lg - bx * NS;
     if (bx >= n_boxes) return;
     const R *C = proxy + (size_t)(slot0 + bx) * P * PP;
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
         __syncthreads();
         // y: T2[k][k2][a] = sum_b W(k2, b) T1[k][b][a]; slot = (column (k, a), segment of k2,
         // uniform across a wavefront)
-        constexpr int SGY = P2pwCfg<R, P, NPW>::SGY, PSY = P2pwCfg<R, P, NPW>::PSY;
+        constexpr int SGY = P2pwCfg<R, P, NPW, BK>::SGY, PSY = P2pwCfg<R, P, NPW, BK>::PSY;
         const int ny = (nk * P + 63) / 64 * 64 * SGY;
         for (int t = threadIdx.x; t < ny; t += blockDim.x) {
             int w, sg;
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
         __syncthreads();
         // x: pw[k1][k30 + k][k2] = sum_a W(k1, a) T2[k][k2][a]; slot = (column w = k n + k2,
         // segment of k1, uniform across a wavefront)
-        constexpr int SGX = P2pwCfg<R, P, NPW>::SGX, PSX = P2pwCfg<R, P, NPW>::PSX;
+        constexpr int SGX = P2pwCfg<R, P, NPW, BK>::SGX, PSX = P2pwCfg<R, P, NPW, BK>::PSX;
         const int nx = (nk * NPW + 63) / 64 * 64 * SGX;
         for (int t = threadIdx.x; t < nx; t += blockDim.x) {
             int w, sg;
@@ -639,17 +641,17 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
 // (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  One
 // workgroup per (box, slice of a).
-template <class R, int P, int NPW>
+template <class R, int P, int NPW, int BK = 0>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
                                                    int item0, int n_items, R *__restrict__ down_base) {
     chain_priority();
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, half = NPW / 2, TS2 = OddStride<P>::v;
-    constexpr int AC = Pw2pCfg<R, P, NPW>::AC;
+    constexpr int AC = Pw2pCfg<R, P, NPW, BK>::AC;
     __shared__ Cx<R> U1[AC * COL];       // [a][k3][k2]
     __shared__ Cx<R> U2[AC * H * TS2];   // [a][k3][b] rows of TS2
     // 1-D grid: (box, slice of a) at xcd_block index box * slices + slice
-    constexpr int NS = Pw2pCfg<R, P, NPW>::slices;
+    constexpr int NS = Pw2pCfg<R, P, NPW, BK>::slices;
     const int lg = xcd_block(blockIdx.x, gridDim.x), it = lg / NS, slice = lg - it * NS;
     if (it >= n_items) return;
     const Cx<R> *X = pin + (size_t)it * NPW * COL;
@@ -698,7 +700,7 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
         __syncthreads();
         // y: U2[a][k3][b] = h(k3) sum_k2 V(k2, b) U1[a][k3][k2]; slot = (column w = a H + k3,
         // segment of b, uniform across a wavefront)
-        constexpr int SGY = Pw2pCfg<R, P, NPW>::SGY, PSY = Pw2pCfg<R, P, NPW>::PSY;
+        constexpr int SGY = Pw2pCfg<R, P, NPW, BK>::SGY, PSY = Pw2pCfg<R, P, NPW, BK>::PSY;
         const int ny = (na * H + 63) / 64 * 64 * SGY;
         for (int t = threadIdx.x; t < ny; t += blockDim.x) {
             int w, sg;
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
         __syncthreads();
         // z: down[c][b][a0 + a] (+)= 2 Re sum_k3 V(k3, c) U2[a][k3][b]; slot = (column w = b na + a,
         // segment of c, uniform across a wavefront)
-        constexpr int SGZ = Pw2pCfg<R, P, NPW>::SGZ, PSZ = Pw2pCfg<R, P, NPW>::PSZ;
+        constexpr int SGZ = Pw2pCfg<R, P, NPW, BK>::SGZ, PSZ = Pw2pCfg<R, P, NPW, BK>::PSZ;
         const int nz = (na * P + 63) / 64 * 64 * SGZ;
         for (int t = threadIdx.x; t < nz; t += blockDim.x) {
             int w, sg;
@@ -1180,6 +1182,306 @@ __global__ __launch_bounds__(1024) void k_near_field(const P4<R> *__restrict__ s
         acc += (double)ea;
     }
     if (active) pot_sorted[it.x + lane] += (R)acc;
+}
+
+// ---------------------------------------------------------- near field in quad rows (FP32)
+// The wavefront's 64 targets form 16 rows of 4 (lane l = 4 g + j: row g, target it.x + l).
+// Every row walks a step list of its own: the sources (over all list1 entries of the target
+// leaf, in entry order, and in walk order inside an entry) that reach at least one of its 4
+// targets.  The 4 targets of a row are much closer together than the 64 of the wavefront, so
+// fewer of a step's lanes fall outside the support: at N=1e6 (uniform) about 54% of the lanes
+// of a step are in support against 34% when the whole wavefront shares one list.
+//
+// Step lists (built once per tree, k_near_rows_build): per item, K steps per row (K = the
+// longest row's count rounded up to 4; shorter rows are padded with a far sentinel point),
+// stored as source indices in blocks of 4 steps: block b holds at element 4 g + j the source
+// of step 4 b + j of row g, so lane l loads element l of each block (one coalesced 256-byte
+// read) and the 4 points of a row's block reach the row's lanes by DPP quad broadcasts.
+// Per (item, row) the non-empty entries are listed as slots (entry ordinal, end step), ending
+// with a terminator slot (nothing in support).
+//
+// For each target the in-support sources appear in its row's list in the same order as in
+// the wavefront's list of k_near_field, each entry's sum is formed the same way and added to
+// the FP64 total in entry order (entries without steps add nothing, there as 0.0 here), so
+// the potentials equal k_near_field's bit for bit.
+constexpr int NF_ROWS = 16;          // rows of 4 targets per wavefront
+constexpr int NF_ROW_PAD = 2 * 64;   // sentinel indices past the last block (loads run one block ahead)
+
+template <int S>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), S * 0x55, 0xF, 0xF, true));
+}
+
+// PASS 0: steps of row g in the k-th entry of item i -> cnt[(ie_base[i] + k) * 16 + g], and the
+// in-support pairs added to *pairs.  PASS 1: the source indices into the row blocks from
+// steps + wave_base[i] (the item's K, as built on the host, in items[i].w).
+template <class R, int PASS>
+__global__ __launch_bounds__(256) void k_near_rows_build(
+    const P4<R> *__restrict__ src, const P4<R> *__restrict__ trg, const int4 *__restrict__ items,
+    const P4<R> *__restrict__ wbox, const int *__restrict__ l1_ptr, const NfEntry *__restrict__ l1,
+    const P4<R> *__restrict__ geom, const P4<R> *__restrict__ cbox, const int *__restrict__ ie_base,
+    int *__restrict__ cnt, const long long *__restrict__ wave_base, int *__restrict__ steps,
+    unsigned long long *__restrict__ pairs, int n_items) {
+    const int item = blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= n_items) return;  // whole wavefront
+    const int lane = threadIdx.x & 63;
+    const int4 it = items[item];
+    const bool active = lane < it.y;
+    const P4<R> me = active ? trg[it.x + lane] : P4<R>{R(1e18), R(1e18), R(1e18), R(0)};
+    const P4<R> blo = wbox[2 * item], bhi = wbox[2 * item + 1];
+    const int eb = l1_ptr[it.z], base = ie_base[item];
+    int *out = PASS == 1 ? steps + wave_base[item] + lane : nullptr;
+    unsigned long long np = 0;
+    int k_row = 0;  // steps of this lane's row so far (the same in the row's 4 lanes)
+    for (int e = eb; e < l1_ptr[it.z + 1]; ++e) {
+        const NfEntry E = l1[e];
+        const P4<R> gm = geom[e];
+        const R bs = gm.w, d2max = bs * bs;
+        const R tx = me.x + gm.x, ty = me.y + gm.y, tz = me.z + gm.z;
+        const R lx = blo.x + gm.x, ly = blo.y + gm.y, lz = blo.z + gm.z;
+        const R hx = bhi.x + gm.x, hy = bhi.y + gm.y, hz = bhi.z + gm.z;
+        const int nch = (E.scnt + NF_CHUNK - 1) / NF_CHUNK;
+        int c_row = 0;
+        for (int c0 = 0; c0 < nch; c0 += 64) {
+            bool keep = false;
+            if (c0 + lane < nch) {
+                const P4<R> cl = cbox[2 * (E.cbase + c0 + lane)], ch = cbox[2 * (E.cbase + c0 + lane) + 1];
+                const R dx = fmax(fmax(cl.x - hx, lx - ch.x), R(0));
+                const R dy = fmax(fmax(cl.y - hy, ly - ch.y), R(0));
+                const R dz = fmax(fmax(cl.z - hz, lz - ch.z), R(0));
+                keep = fma(dx, dx, fma(dy, dy, dz * dz)) < d2max;
+            }
+            unsigned long long mask = __ballot(keep);
+            while (mask) {
+                const int k = __ffsll((unsigned long long)mask) - 1;
+                mask &= mask - 1;
+                const int s0 = E.sbeg + (c0 + k) * NF_CHUNK;
+                const int ns = min(NF_CHUNK, E.sbeg + E.scnt - s0);
+                for (int j = 0; j < ns; ++j) {
+                    const P4<R> q = src[s0 + j];
+                    const R dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
+                    const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    const unsigned long long in = __ballot(r2 < d2max && r2 > R(1e-30));
+                    np += __popcll(in);
+                    if ((in >> (lane & ~3)) & 0xFull) {
+                        if (PASS == 1 && (k_row & 3) == (lane & 3)) out[(size_t)(k_row >> 2) * 64] = s0 + j;
+                        ++k_row;
+                        ++c_row;
+                    }
+                }
+            }
+        }
+        if (PASS == 0 && (lane & 3) == 0) cnt[(size_t)(base + e - eb) * NF_ROWS + (lane >> 2)] = c_row;
+    }
+    if (PASS == 0 && lane == 0) atomicAdd(pairs, np);
+}
+
+// Residual evaluation over the row step lists (FP32, NC coefficients in the x form).  Row g
+// of item i lists its non-empty entries in slots rslot[(rs_base[i] + s) * 16 + g] =
+// (end step << 8) | ordinal of the entry in the target leaf's list1, followed by a terminator
+// (ordinal = number of entries: bs = 0, nothing in support).  Each wavefront copies its leaf's
+// entry geometry (offset, bs, 2/bs, -bs/2, bs^2) and its rows' slots into LDS (ne_max + 1
+// entries per wavefront), so an entry switch reads LDS only.
+template <int NC>
+__global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict__ src, const P4<float> *__restrict__ trg,
+                                                   const int4 *__restrict__ items, const int *__restrict__ l1_ptr,
+                                                   const P4<float> *__restrict__ geom, const int *__restrict__ rs_base,
+                                                   const int *__restrict__ rslot, const long long *__restrict__ wave_base,
+                                                   const int *__restrict__ steps, const float *__restrict__ coeffs,
+                                                   float *__restrict__ pot_sorted, int item0, int item1, int ne_max) {
+    extern __shared__ float4 nf_lds[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = item0 + xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + w;
+    if (item >= item1) return;  // whole wavefront (no workgroup barrier below)
+    const int lane = threadIdx.x & 63, g = lane >> 2;
+    const int4 it = items[item];  // (first target, count, target-leaf list index, K)
+    // this wavefront's LDS: ne_max + 1 entries x (offset+bs, consts), then (ne_max + 1) x 16 slots
+    float4 *geo = nf_lds + (size_t)w * (ne_max + 1) * 6;
+    int *slots = reinterpret_cast<int *>(geo + 2 * (ne_max + 1));
+    const int eb = l1_ptr[it.z], ne = l1_ptr[it.z + 1] - eb;
+    for (int e = lane; e <= ne; e += 64) {
+        const P4<float> gm = e < ne ? geom[eb + e] : P4<float>{0.0f, 0.0f, 0.0f, 0.0f};
+        const float bs = gm.w;
+        geo[2 * e] = make_float4(gm.x, gm.y, gm.z, bs * bs);
+        geo[2 * e + 1] = make_float4(e < ne ? 2.0f / bs : 0.0f, -bs / 2.0f, 0.0f, 0.0f);
+    }
+    const int *rs = rslot + (size_t)rs_base[item] * NF_ROWS;
+    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) slots[k] = rs[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool active = lane < it.y;
+    const P4<float> me = active ? trg[it.x + lane] : P4<float>{1e18f, 1e18f, 1e18f, 0.0f};
+    float c[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    float tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
+    int end = 0, ns = 0;
+    double acc = 0;
+    float ea = 0;
+    auto enter = [&]() {  // the row's next non-empty entry (or the terminator)
+        acc += (double)ea;
+        ea = 0;
+        const int v = slots[ns * NF_ROWS + g];
+        ++ns;
+        const int o = v & 255;
+        end = v >> 8;
+        const float4 a = geo[2 * o], b = geo[2 * o + 1];
+        tx = me.x + a.x;
+        ty = me.y + a.y;
+        tz = me.z + a.z;
+        d2max = a.w;
+        rsc = b.x;
+        cen = b.y;
+    };
+    enter();
+    const int K = it.w;
+    const int *sp = steps + wave_base[item] + lane;
+    int id = sp[0];
+    P4<float> pc = src[id];
+    id = sp[64];
+    for (int k = 0; k < K; k += 4) {
+        const P4<float> pn = src[id];            // next block's point (the padding keeps it in bounds)
+        id = sp[(size_t)(k + 8) * NF_ROWS];      // and the index of the block after it
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (k + s >= end) enter();           // rows reach their entries' ends at different steps
+            float px, py, pz;
+            switch (s) {
+            case 0: px = quad_bcast<0>(pc.x); py = quad_bcast<0>(pc.y); pz = quad_bcast<0>(pc.z); break;
+            case 1: px = quad_bcast<1>(pc.x); py = quad_bcast<1>(pc.y); pz = quad_bcast<1>(pc.z); break;
+            case 2: px = quad_bcast<2>(pc.x); py = quad_bcast<2>(pc.y); pz = quad_bcast<2>(pc.z); break;
+            default: px = quad_bcast<3>(pc.x); py = quad_bcast<3>(pc.y); pz = quad_bcast<3>(pc.z); break;
+            }
+            // (source - target: the negated difference, exactly, so r^2 is the same)
+            const float dx = px - tx, dy = py - ty, dz = pz - tz;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float u = 0;
+            if (r2 < d2max && r2 > 1e-30f) {
+                const float rinv = rsqrt_full<float>(r2);
+                const float x = fmaf(r2, rinv, cen) * rsc;
+                float p = c[NC - 1];
+#pragma unroll
+                for (int i = NC - 2; i >= 0; --i) p = fmaf(p, x, c[i]);
+                u = p * rinv;
+            }
+            float pq;
+            switch (s) {
+            case 0: pq = quad_bcast<0>(pc.w); break;
+            case 1: pq = quad_bcast<1>(pc.w); break;
+            case 2: pq = quad_bcast<2>(pc.w); break;
+            default: pq = quad_bcast<3>(pc.w); break;
+            }
+            ea = fmaf(pq, u, ea);
+        }
+        pc = pn;
+    }
+    acc += (double)ea;
+    if (active) pot_sorted[it.x + lane] += (float)acc;
+}
+
+// The FP64 row kernel (NC coefficients, x form).  The same step lists and slots; a row's 4
+// points of a block reach its lanes through LDS (64-bit DPP has no quad broadcast): each lane
+// writes the point it loaded, then every lane reads its row's point of the step.  Per
+// wavefront LDS: 64 points (2 KiB), then (ne_max + 1) entries x 48 bytes of geometry and
+// (ne_max + 1) x 16 slots.  Bit-identical to k_near_field<double, NC, false> for the same
+// reason as the FP32 kernel.
+__host__ __device__ inline size_t near_rows64_lds_per_wave(int ne_max) {
+    return 64 * 2 * sizeof(double2) + (size_t)(ne_max + 1) * 3 * sizeof(double2) + (size_t)(ne_max + 1) * NF_ROWS * 4;
+}
+
+template <int NC>
+__global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restrict__ src,
+                                                      const P4<double> *__restrict__ trg,
+                                                      const int4 *__restrict__ items, const int *__restrict__ l1_ptr,
+                                                      const P4<double> *__restrict__ geom,
+                                                      const int *__restrict__ rs_base, const int *__restrict__ rslot,
+                                                      const long long *__restrict__ wave_base,
+                                                      const int *__restrict__ steps, const double *__restrict__ coeffs,
+                                                      double *__restrict__ pot_sorted, int item0, int item1, int ne_max) {
+    extern __shared__ double2 nf_lds64[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = item0 + xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + w;
+    if (item >= item1) return;  // whole wavefront (no workgroup barrier below)
+    const int lane = threadIdx.x & 63, g = lane >> 2;
+    const int4 it = items[item];  // (first target, count, target-leaf list index, K)
+    double2 *pts = reinterpret_cast<double2 *>(reinterpret_cast<char *>(nf_lds64) + (size_t)w * near_rows64_lds_per_wave(ne_max));
+    double2 *geo = pts + 128;
+    int *slots = reinterpret_cast<int *>(geo + 3 * (ne_max + 1));
+    const int eb = l1_ptr[it.z], ne = l1_ptr[it.z + 1] - eb;
+    for (int e = lane; e <= ne; e += 64) {
+        const P4<double> gm = e < ne ? geom[eb + e] : P4<double>{0.0, 0.0, 0.0, 0.0};
+        const double bs = gm.w;
+        geo[3 * e] = make_double2(gm.x, gm.y);
+        geo[3 * e + 1] = make_double2(gm.z, bs * bs);
+        geo[3 * e + 2] = make_double2(e < ne ? 2.0 / bs : 0.0, -bs / 2.0);
+    }
+    const int *rs = rslot + (size_t)rs_base[item] * NF_ROWS;
+    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) slots[k] = rs[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool active = lane < it.y;
+    const P4<double> me = active ? trg[it.x + lane] : P4<double>{1e18, 1e18, 1e18, 0.0};
+    double c[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    const double ctop = c[NC - 1] + (double)vgpr_zero();  // leading coefficient in a VGPR (see k_near_field)
+    double tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
+    int end = 0, ns = 0;
+    double acc = 0, ea = 0;
+    auto enter = [&]() {  // the row's next non-empty entry (or the terminator)
+        acc += ea;
+        ea = 0;
+        const int v = slots[ns * NF_ROWS + g];
+        ++ns;
+        const int o = v & 255;
+        end = v >> 8;
+        const double2 a = geo[3 * o], b = geo[3 * o + 1], cc = geo[3 * o + 2];
+        tx = me.x + a.x;
+        ty = me.y + a.y;
+        tz = me.z + b.x;
+        d2max = b.y;
+        rsc = cc.x;
+        cen = cc.y;
+    };
+    enter();
+    const int K = it.w;
+    const int *sp = steps + wave_base[item] + lane;
+    int id = sp[0];
+    P4<double> pc = src[id];
+    id = sp[64];
+    for (int k = 0; k < K; k += 4) {
+        const P4<double> pn = src[id];           // next block's point (the padding keeps it in bounds)
+        id = sp[(size_t)(k + 8) * NF_ROWS];      // and the index of the block after it
+        pts[2 * lane] = make_double2(pc.x, pc.y);
+        pts[2 * lane + 1] = make_double2(pc.z, pc.w);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (k + s >= end) enter();           // rows reach their entries' ends at different steps
+            const double2 a = pts[2 * (4 * g + s)], b = pts[2 * (4 * g + s) + 1];
+            // (source - target: the negated difference, exactly, so r^2 is the same)
+            const double dx = a.x - tx, dy = a.y - ty, dz = b.x - tz;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (r2 < d2max && r2 > 1e-30) {
+                const double rinv = rsqrt_full<double>(r2);
+                const double x = fma(r2, rinv, cen) * rsc;
+                double p = ctop;
+#pragma unroll
+                for (int i = NC - 2; i >= 0; --i) p = fma(p, x, c[i]);
+                ea = fma(p * rinv, b.y, ea);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // every lane has read the block before it is overwritten
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        pc = pn;
+    }
+    acc += ea;
+    if (active) pot_sorted[it.x + lane] += acc;
 }
 
 template <class R>

@@ -17,6 +17,9 @@
 #include <type_traits>
 #include <unordered_map>
 #include <cstdio>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -45,10 +48,35 @@ void *Arena::take(size_t nbytes) {
     return dry() ? nullptr : base + off;
 }
 
-
 namespace {
 
 constexpr size_t kPinBytes = size_t(512) << 20;  // incoming plane-wave scratch cap
+
+// Tuned knobs per plan kind, shared by the plans of the process; optionally mirrored in the
+// text file named by PDMK_TUNE_CACHE (one "key near_wpb near_late p2pw_bk pw2p_bk t3_div"
+// line per kind, later lines win).
+std::mutex tune_mu;
+std::map<std::string, Knobs> tune_cache;
+bool tune_file_read = false;
+
+const char *tune_file() { return std::getenv("PDMK_TUNE_CACHE"); }
+
+void tune_cache_load() {
+    if (tune_file_read) return;
+    tune_file_read = true;
+    const char *f = tune_file();
+    if (!f) return;
+    std::ifstream in(f);
+    std::string key;
+    Knobs k;
+    while (in >> key >> k.near_wpb >> k.near_late >> k.p2pw_bk >> k.pw2p_bk >> k.t3_div) tune_cache[key] = k;
+}
+
+int log2_bucket(long long v) {
+    int b = 0;
+    while (v > 1) { v >>= 1; ++b; }
+    return b;
+}
 
 int nc_for_digits(int d) {
     static const int nc[] = {5, 5, 5, 6, 9, 11, 12, 15, 17, 18, 22, 23, 25};
@@ -176,6 +204,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             for (int i = 0; i < t.trg_count[b]; ++i) tw0[t.trg_begin[b] + i] = w;
         }
     srcv.resize((size_t)nsall + NF_CHUNK, P4<Real>{Real(0), Real(0), Real(0), Real(0)});  // chunk padding
+    srcv.push_back(P4<Real>{Real(1e18), Real(1e18), Real(1e18), Real(0)});  // sentinel: far from every target
     src.upload(srcv);
     srcv.resize(nsall);
     src_user.upload(su);
@@ -561,7 +590,93 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             n_ie += ptr[itv[i].z + 1] - ptr[itv[i].z];
         }
         nf_ie_base.upload(ieb);
-        if (n_nf_items > 0) {
+        // FP32 at 6 digits: quad-row step lists (k_near_rows); PDMK_NEAR_ROWS=0 keeps the
+        // wavefront-wide lists of k_near_field
+        const char *rows_env = std::getenv("PDMK_NEAR_ROWS");
+        nf_rows = !split && (nc == 12 || (std::is_same<Real, double>::value && (nc == 18 || nc == 25))) &&
+                  !(rows_env && rows_env[0] == '0');
+        std::vector<long long> wbase;
+        std::vector<int> rsb, rend;
+        long long total = 0;
+        unsigned long long np = 0;
+        const dim3 grid((n_nf_items + 3) / 4);
+        if (n_nf_items > 0 && nf_rows) {
+            DevBuf cnt, pairs;
+            cnt.alloc(sizeof(int) * (size_t)std::max(1, n_ie) * NF_ROWS);
+            pairs.alloc(sizeof(unsigned long long));
+            check(hipMemsetAsync(pairs.p, 0, sizeof(unsigned long long), stream), "hipMemsetAsync");
+            hipLaunchKernelGGL((k_near_rows_build<Real, 0>), grid, dim3(256), 0, stream, src.as<P4<Real>>(),
+                               trg.as<P4<Real>>(), nf_items.as<int4>(), nf_wbox.as<P4<Real>>(),
+                               nf_ptr.as<int>(), nf_l1.as<NfEntry>(), nf_geom.as<P4<Real>>(),
+                               nf_cbox.as<P4<Real>>(), nf_ie_base.as<int>(), cnt.as<int>(),
+                               (const long long *)nullptr, (int *)nullptr, pairs.as<unsigned long long>(), n_nf_items);
+            check(hipGetLastError(), "k_near_rows_build count");
+            std::vector<int> c((size_t)std::max(1, n_ie) * NF_ROWS);
+            check(hipMemcpyAsync(c.data(), cnt.p, sizeof(int) * c.size(), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+            check(hipMemcpyAsync(&np, pairs.p, sizeof(np), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            // per item: K = longest row rounded up to 4, block base; per (item, row): slots of the
+            // non-empty entries (geometry, cumulative end) and a terminator
+            wbase.assign(n_nf_items, 0);
+            rsb.assign(n_nf_items, 0);
+            int n_slots = 0, k_item_max = 0;
+            nf_ne_max = 0;
+            for (int i = 0; i < n_nf_items; ++i) {
+                const int ne = ptr[itv[i].z + 1] - ptr[itv[i].z];
+                nf_ne_max = std::max(nf_ne_max, ne);
+                rsb[i] = n_slots;
+                n_slots += ne + 1;
+                // slot = (end step << 8) | entry ordinal; unused slots hold the terminator
+                rend.resize((size_t)n_slots * NF_ROWS, (0x7fffff << 8) | std::min(ne, 255));
+                int kmax = 0;
+                for (int g = 0; g < NF_ROWS; ++g) {
+                    int k = 0, sl = 0;
+                    for (int e = 0; e < ne; ++e) {
+                        const int ce = c[(size_t)(ieb[i] + e) * NF_ROWS + g];
+                        if (ce == 0) continue;
+                        k += ce;
+                        rend[(size_t)(rsb[i] + sl) * NF_ROWS + g] = (std::min(k, 0x7fffff) << 8) | e;
+                        ++sl;
+                    }
+                    kmax = std::max(kmax, k);
+                }
+                const int K = (kmax + 3) & ~3;
+                k_item_max = std::max(k_item_max, K);
+                itv[i].w = K;
+                wbase[i] = total;
+                total += (long long)K * NF_ROWS;
+            }
+            n_nf_steps = total / NF_ROWS;  // steps per row summed over the items
+            // beyond int indexing, 8-bit entry ordinals or 23-bit step counts: wavefront lists
+            if (total + NF_ROW_PAD > 0x7fffffffLL || nf_ne_max > 255 || k_item_max > 0x7fff00) nf_rows = false;
+            // the row lists take at most a quarter of the free device memory (the plane waves
+            // and the rest of the plan come after them)
+            size_t mem_free = 0, mem_total = 0;
+            check(hipMemGetInfo(&mem_free, &mem_total), "hipMemGetInfo");
+            if ((double)(total + NF_ROW_PAD) * sizeof(int) > 0.25 * (double)mem_free) nf_rows = false;
+        }
+        if (n_nf_items > 0 && nf_rows) {
+            nf_items.upload(itv);
+            nf_rs_base.upload(rsb);
+            nf_rslot.upload(rend);
+            nf_wave_base.upload(wbase);
+            // the padding points at the sentinel source (far from every target)
+            const int sentinel = nsall + NF_CHUNK;
+            nf_row_steps.alloc(sizeof(int) * (size_t)(total + NF_ROW_PAD));
+            const long long n_fill = total + NF_ROW_PAD;
+            hipLaunchKernelGGL(k_fill<int>, dim3((unsigned)((n_fill + 255) / 256)), dim3(256), 0, stream,
+                               nf_row_steps.as<int>(), sentinel, (int)n_fill);
+            check(hipGetLastError(), "k_fill row steps");
+            hipLaunchKernelGGL((k_near_rows_build<Real, 1>), grid, dim3(256), 0, stream, src.as<P4<Real>>(),
+                               trg.as<P4<Real>>(), nf_items.as<int4>(), nf_wbox.as<P4<Real>>(),
+                               nf_ptr.as<int>(), nf_l1.as<NfEntry>(), nf_geom.as<P4<Real>>(),
+                               nf_cbox.as<P4<Real>>(), nf_ie_base.as<int>(), (int *)nullptr,
+                               nf_wave_base.as<long long>(), nf_row_steps.as<int>(), (unsigned long long *)nullptr,
+                               n_nf_items);
+            check(hipGetLastError(), "k_near_rows_build write");
+            work.nf_pairs_support = (double)np;
+        }
+        if (n_nf_items > 0 && !nf_rows) {
             DevBuf cnt, pairs;
             cnt.alloc(sizeof(int) * (size_t)std::max(1, n_ie));
             pairs.alloc(sizeof(unsigned long long));
@@ -717,6 +832,16 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     src_perm_host = t.src_order;
     trg_perm_host = t.trg_order;
     check(hipStreamSynchronize(stream), "plan build");
+    {
+        // launch shapes of an earlier tuned plan of the same kind
+        std::lock_guard<std::mutex> lock(tune_mu);
+        tune_cache_load();
+        auto it = tune_cache.find(tune_key());
+        if (it != tune_cache.end()) {
+            knobs = it->second;
+            knobs_cached = true;
+        }
+    }
     build_ms[0] = ms_since(t_start);
     guard.armed = false;
 }
@@ -752,10 +877,10 @@ namespace {
 // boxes still spreads over the whole GPU (about 1024 workgroups or more)
 template <class Real, int P, bool TRANSPOSE>
 void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launch &L, const Real *mats_rm,
-                    hipStream_t s) {
+                    hipStream_t s, int div) {
     if (L.n_items == 0) return;
     constexpr int KC = T3Cfg<Real, P>::KC;
-    int kc = KC;
+    int kc = std::max(1, KC / std::max(1, div));
     while (kc > 1 && (long)L.n_items * ((P + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
     const dim3 grid(L.n_items * ((P + kc - 1) / kc));  // (item, slice) pairs
     hipLaunchKernelGGL((k_tensor3<Real, P, TRANSPOSE>), grid, dim3(T3Cfg<Real, P>::threads), 0, s, in, out,
@@ -789,17 +914,41 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     }
     for (auto &C : pl.c2p_levels) {
         launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_buffer(), C.children,
-                                      pl.mats_rm.template as<Real>(), s);
+                                      pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div);
         hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256), dim3(256), 0, s,
                            pl.proxy_up.template as<Real>(), pl.c2p_buffer(),
                            C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents);
         check(hipGetLastError(), "k_sum_children");
     }
-    auto proxy2pw = [&](int slot0, int count) {
-        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW>), dim3(count * P2pwCfg<Real, P, NPW>::slices), dim3(256), 0, s,
-                           pl.proxy_up.template as<Real>(), pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(),
+    auto proxy2pw_bk = [&](auto bk, int slot0, int count) {
+        constexpr int BK = decltype(bk)::value;
+        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW, BK>), dim3(count * P2pwCfg<Real, P, NPW, BK>::slices), dim3(256), 0,
+                           s, pl.proxy_up.template as<Real>(), pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(),
                            slot0, count);
         check(hipGetLastError(), "k_proxy2pw");
+    };
+    auto proxy2pw = [&](int slot0, int count) {
+        switch (pl.knobs.p2pw_bk) {
+        case 16: proxy2pw_bk(std::integral_constant<int, 16>{}, slot0, count); break;
+        case 32: proxy2pw_bk(std::integral_constant<int, 32>{}, slot0, count); break;
+        case 64: proxy2pw_bk(std::integral_constant<int, 64>{}, slot0, count); break;
+        default: proxy2pw_bk(std::integral_constant<int, 0>{}, slot0, count); break;
+        }
+    };
+    auto pw2proxy_bk = [&](auto bk, const typename Plan<Real>::EvalLevel &E, int i0, int nb) {
+        constexpr int BK = decltype(bk)::value;
+        hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW, BK>), dim3(nb * Pw2pCfg<Real, P, NPW, BK>::slices), dim3(256), 0, s,
+                           pl.pin.template as<C>(), pl.pw2poly.template as<C>(), E.down.template as<int>(),
+                           E.zero.template as<int>(), i0, nb, pl.proxy_down.template as<Real>());
+        check(hipGetLastError(), "k_pw2proxy");
+    };
+    auto pw2proxy = [&](const typename Plan<Real>::EvalLevel &E, int i0, int nb) {
+        switch (pl.knobs.pw2p_bk) {
+        case 16: pw2proxy_bk(std::integral_constant<int, 16>{}, E, i0, nb); break;
+        case 32: pw2proxy_bk(std::integral_constant<int, 32>{}, E, i0, nb); break;
+        case 64: pw2proxy_bk(std::integral_constant<int, 64>{}, E, i0, nb); break;
+        default: pw2proxy_bk(std::integral_constant<int, 0>{}, E, i0, nb); break;
+        }
     };
     if (pl.n_up > 0 && !pl.pw_by_level) proxy2pw(0, pl.n_up);
     if (pl.e_upward) check(hipEventRecord(pl.e_upward, s), "hipEventRecord");
@@ -815,15 +964,11 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
                                E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0,
                                pl.pin.template as<C>());
             check(hipGetLastError(), "k_pw_shift");
-            hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW>), dim3(nb * Pw2pCfg<Real, P, NPW>::slices), dim3(256), 0, s,
-                               pl.pin.template as<C>(),
-                               pl.pw2poly.template as<C>(), E.down.template as<int>(), E.zero.template as<int>(), i0, nb,
-                               pl.proxy_down.template as<Real>());
-            check(hipGetLastError(), "k_pw2proxy");
+            pw2proxy(E, i0, nb);
         }
         auto &L = pl.p2c_levels[l];
         launch_tensor3<Real, P, false>(pl.proxy_down.template as<Real>(), pl.proxy_down.template as<Real>(), L,
-                                       pl.mats_rm.template as<Real>(), s);
+                                       pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div);
     }
     if (pl.n_leaf_eval > 0) {
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
@@ -854,6 +999,47 @@ void launch_near(Plan<Real> &pl, hipStream_t s, int wpb, int i0, int i1) {
                        pl.nf_ie_base.template as<int>(), pl.nf_ie_ptr.template as<long long>(), pl.nf_steps.template as<int>(),
                        pl.coeffs.template as<Real>(), pl.pot_near.template as<Real>(), i0, i1);
     check(hipGetLastError(), "k_near_field");
+}
+
+// quad-row near field (FP32 plans built with row step lists)
+template <class Real>
+void launch_near_rows(Plan<Real> &pl, hipStream_t s, int wpb, int i0, int i1) {
+    if constexpr (std::is_same<Real, float>::value) {
+        if (i1 <= i0) return;
+        // (the LDS tables of a workgroup's wavefronts stay within 64 KiB)
+        const size_t per_wave = (size_t)(pl.nf_ne_max + 1) * 6 * sizeof(float4);
+        while (wpb > 1 && (size_t)wpb * per_wave > 65536) wpb /= 2;
+        const size_t lds = (size_t)wpb * per_wave;
+        hipLaunchKernelGGL((k_near_rows<12>), dim3((i1 - i0 + wpb - 1) / wpb), dim3(64 * wpb), lds, s,
+                           pl.src.template as<P4<float>>(), pl.trg.template as<P4<float>>(),
+                           pl.nf_items.template as<int4>(), pl.nf_ptr.template as<int>(),
+                           pl.nf_geom.template as<P4<float>>(), pl.nf_rs_base.template as<int>(),
+                           pl.nf_rslot.template as<int>(), pl.nf_wave_base.template as<long long>(),
+                           pl.nf_row_steps.template as<int>(), pl.coeffs.template as<float>(),
+                           pl.pot_near.template as<float>(), i0, i1, pl.nf_ne_max);
+        check(hipGetLastError(), "k_near_rows");
+    } else {
+        if (i1 <= i0) return;
+        const size_t per_wave = near_rows64_lds_per_wave(pl.nf_ne_max);
+        while (wpb > 1 && (size_t)wpb * per_wave > 65536) wpb /= 2;
+        const size_t lds = (size_t)wpb * per_wave;
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3((i1 - i0 + wpb - 1) / wpb), dim3(64 * wpb), lds, s,
+                               pl.src.template as<P4<double>>(), pl.trg.template as<P4<double>>(),
+                               pl.nf_items.template as<int4>(), pl.nf_ptr.template as<int>(),
+                               pl.nf_geom.template as<P4<double>>(), pl.nf_rs_base.template as<int>(),
+                               pl.nf_rslot.template as<int>(), pl.nf_wave_base.template as<long long>(),
+                               pl.nf_row_steps.template as<int>(), pl.coeffs.template as<double>(),
+                               pl.pot_near.template as<double>(), i0, i1, pl.nf_ne_max);
+        };
+        switch (pl.nc) {
+        case 12: go(k_near_rows64<12>); break;
+        case 18: go(k_near_rows64<18>); break;
+        case 25: go(k_near_rows64<25>); break;
+        default: throw std::logic_error("row near field: unsupported coefficient count");
+        }
+        check(hipGetLastError(), "k_near_rows64");
+    }
 }
 
 }  // namespace
@@ -898,8 +1084,8 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
         check(hipEventRecord(e_gather, s), "hipEventRecord");
         check(hipStreamWaitEvent(sn, e_gather, 0), "hipStreamWaitEvent");
     }
-    const bool near_late = !serial && p >= 28;
-    const int wpb = near_late ? 16 : 4;
+    const bool near_late = !serial && (knobs.near_late >= 0 ? knobs.near_late == 1 : p >= 28);
+    const int wpb = knobs.near_wpb > 0 ? knobs.near_wpb : (near_late ? 16 : 4);
     auto near_field = [&](int i0, int i1) {
         mark(3, 0, sn);
         if (stages & 2) {
@@ -908,13 +1094,22 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
             case 3: launch_near<Real, 6, true>(*this, sn, wpb, i0, i1); break;
             case 4: launch_near<Real, 9, true>(*this, sn, wpb, i0, i1); break;
             case 5: launch_near<Real, 11, true>(*this, sn, wpb, i0, i1); break;
-            case 6: launch_near<Real, 12, false>(*this, sn, wpb, i0, i1); break;
+            case 6:
+                if (nf_rows) launch_near_rows<Real>(*this, sn, wpb, i0, i1);
+                else launch_near<Real, 12, false>(*this, sn, wpb, i0, i1);
+                break;
             case 7: launch_near<Real, 15, false>(*this, sn, wpb, i0, i1); break;
             case 8: launch_near<Real, 17, false>(*this, sn, wpb, i0, i1); break;
-            case 9: launch_near<Real, 18, false>(*this, sn, wpb, i0, i1); break;
+            case 9:
+                if (nf_rows) launch_near_rows<Real>(*this, sn, wpb, i0, i1);
+                else launch_near<Real, 18, false>(*this, sn, wpb, i0, i1);
+                break;
             case 10: launch_near<Real, 22, false>(*this, sn, wpb, i0, i1); break;
             case 11: launch_near<Real, 23, false>(*this, sn, wpb, i0, i1); break;
-            case 12: launch_near<Real, 25, false>(*this, sn, wpb, i0, i1); break;
+            case 12:
+                if (nf_rows) launch_near_rows<Real>(*this, sn, wpb, i0, i1);
+                else launch_near<Real, 25, false>(*this, sn, wpb, i0, i1);
+                break;
             default: throw std::logic_error("unsupported digit count");
             }
         }
@@ -965,6 +1160,98 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
         times_ms[0] = a;  // nufft_ms
         times_ms[1] = b;  // dmk_ms
         times_ms[2] = c;  // direct_ms
+    }
+}
+
+namespace {
+
+// Charges of the tuning runs: a fixed neutral-ish pattern (the timings do not depend on them).
+__global__ void k_tune_charges(double *q, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) q[i] = (double)((unsigned)(i * 2654435761u) >> 8) * (1.0 / 16777216.0) - 0.5;
+}
+
+}  // namespace
+
+void tune_cache_clear() {
+    std::lock_guard<std::mutex> lock(tune_mu);
+    tune_cache.clear();
+    tune_file_read = true;  // the file is not read again
+}
+
+template <typename Real>
+std::string Plan<Real>::tune_key() const {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    check(hipGetDevice(&dev), "hipGetDevice");
+    check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    return std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + ":p" + std::to_string(p) +
+           ":r" + std::to_string(sizeof(Real)) + ":n" + std::to_string(log2_bucket(n)) + ":b" +
+           std::to_string(log2_bucket((long long)n_up + 1)) + ":w" + std::to_string(log2_bucket((long long)n_nf_items + 1));
+}
+
+template <typename Real>
+void Plan<Real>::autotune(int repeats, double *ms_default, double *ms_tuned) {
+    repeats = std::max(1, repeats);
+    hipLaunchKernelGGL(k_tune_charges, dim3((n + 255) / 256), dim3(256), 0, stream, q_dev.as<double>(), n);
+    check(hipGetLastError(), "k_tune_charges");
+    hipEvent_t t0, t1;
+    check(hipEventCreate(&t0), "hipEventCreate");
+    check(hipEventCreate(&t1), "hipEventCreate");
+    auto time_ms = [&]() {
+        evaluate(q_dev.as<double>(), pot_dev.as<Real>(), stream);  // warm-up
+        check(hipEventRecord(t0, stream), "hipEventRecord");
+        for (int r = 0; r < repeats; ++r) evaluate(q_dev.as<double>(), pot_dev.as<Real>(), stream);
+        check(hipEventRecord(t1, stream), "hipEventRecord");
+        check(hipEventSynchronize(t1), "hipEventSynchronize");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+        return (double)ms / repeats;
+    };
+    const Knobs saved = knobs;
+    knobs = Knobs{};
+    const double base = time_ms();
+    double best = base;
+    // the rows near field keeps its LDS tables under 64 KiB per workgroup
+    const size_t rows_lds_wave = nf_rows ? (size_t)(nf_ne_max + 1) * 6 * 16 : 0;
+    auto try_values = [&](int Knobs::*field, std::initializer_list<int> values) {
+        int best_v = knobs.*field;
+        for (int v : values) {
+            if (v == best_v) continue;
+            if (field == &Knobs::near_wpb && rows_lds_wave * (size_t)v > 65536) continue;
+            knobs.*field = v;
+            const double t = time_ms();
+            if (t < 0.98 * best) {
+                best = t;
+                best_v = v;
+            }
+        }
+        knobs.*field = best_v;
+    };
+    try {
+        try_values(&Knobs::near_wpb, {4, 8, 16});
+        try_values(&Knobs::near_late, {0, 1});
+        try_values(&Knobs::p2pw_bk, {16, 32, 64});
+        try_values(&Knobs::pw2p_bk, {16, 32, 64});
+        try_values(&Knobs::t3_div, {2});
+    } catch (...) {
+        knobs = saved;
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        throw;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (ms_default) *ms_default = base;
+    if (ms_tuned) *ms_tuned = best;
+    std::lock_guard<std::mutex> lock(tune_mu);
+    tune_cache_load();
+    const std::string key = tune_key();
+    tune_cache[key] = knobs;
+    if (const char *f = tune_file()) {
+        std::ofstream out(f, std::ios::app);
+        out << key << " " << knobs.near_wpb << " " << knobs.near_late << " " << knobs.p2pw_bk << " " << knobs.pw2p_bk
+            << " " << knobs.t3_div << "\n";
     }
 }
 

@@ -115,6 +115,18 @@ struct QpFarField {
     template <class R> void run(const double *d_q, R *d_out, hipStream_t s) const;  // enqueue only
 };
 
+// Launch shapes of evaluate() the autotuner chooses among (0 / -1 = the built-in rule).
+struct Knobs {
+    int near_wpb = 0;    // near-field wavefronts per workgroup: 0 (4, or 16 for a late start), 4, 8, 16
+    int near_late = -1;  // near field after the upward pass: -1 (when p >= 28), 0, 1
+    int p2pw_bk = 0;     // proxy -> plane-wave LDS budget, KiB: 0 (per order and precision), 16, 32, 64
+    int pw2p_bk = 0;     // plane-wave -> proxy LDS budget, KiB: 0 (per order and precision), 16, 32, 64
+    int t3_div = 1;      // tensor transform: output planes per workgroup = the budget's count / t3_div (1, 2)
+};
+
+// Forget the tuned knobs of this process (plans built afterwards use the built-in shapes).
+void tune_cache_clear();
+
 template <typename Real>
 struct Plan {
     // sizes
@@ -167,6 +179,11 @@ struct Plan {
     int n_nf_leaves = 0, n_nf_items = 0;
     long long n_nf_steps = 0;
     DevBuf nf_trg, nf_ptr, nf_l1, nf_geom, nf_items, nf_wbox, nf_cbox, coeffs, nf_ie_base, nf_ie_ptr, nf_steps;
+    // quad-row step lists (FP32 at 6 digits): per item the row blocks from nf_wave_base[i]
+    // (K steps per row in nf_items[i].w), per (item, row) the entry slots from nf_rs_base[i]
+    bool nf_rows = false;
+    int nf_ne_max = 0;  // most list1 entries of a target leaf
+    DevBuf nf_rs_base, nf_rslot, nf_wave_base, nf_row_steps;
     // far field: exact Fourier sum, or the NUFFT for large N x modes
     int far_nufft = 0;
     FourierSum ff;
@@ -190,6 +207,16 @@ struct Plan {
 
     // host buffers in, host buffers out
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
+
+    // Launch shapes: set at construction from the per-device cache when a plan of the same
+    // kind (order, precision, size buckets) was tuned before in this process (or in the file
+    // PDMK_TUNE_CACHE names); autotune() times evaluate() on this plan over each knob's
+    // candidates in turn (coordinate descent, keeping a change only when it is at least 2%
+    // faster), stores the result in the cache and returns {ms default, ms tuned}.
+    Knobs knobs;
+    bool knobs_cached = false;
+    std::string tune_key() const;
+    void autotune(int repeats, double *ms_default, double *ms_tuned);
 
     // Quasi-periodic evaluate (bloch given at construction): d_out holds 2N Real, (Re, Im)
     // per point in user order.  Two passes of the proxy / plane-wave chain and the near field
