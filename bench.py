@@ -82,6 +82,8 @@ def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, char
     build_s = time.time() - t0
     if tune:
         tuning["autotune"] = s.autotune(repeats=3)
+        # the other paths keep the built-in launch shapes
+        P.lib().pdmk_gpu_tune_cache_clear()
     out = P.DeviceArray(n, s.dtype)
     for k in range(warmup):
         s.evaluate_device(charges_dev.offset((k % n_vec) * n), out.ptr, 0)
@@ -121,7 +123,7 @@ def _run_path(P, cell, pos, n, precision, n_digits, steps, warmup, npl, nz, char
         "stats": s.stats(), "work": s.work(), "nf_steps": s.stats()["n_nf_steps"],
         "last_charge_index": last_q,
         "host_buffer_ms": host_ms, "host_buffer_mpts_per_s": (n / (host_ms / 1000.0) / 1e6) if host_ms else None,
-        "n_per_leaf": npl, "nz": nz, "tuning": tuning,
+        "n_per_leaf": npl, "nz": nz, "tuning": tuning, "knobs": s.knobs(),
     }
     s.close()
     return res, pot
@@ -275,11 +277,14 @@ def main():
     nf_ms = head["kernel_ms"]["near_field"]
     peak = PEAK_FP32_TFLOPS if head["precision"] == "float" else PEAK_FP64_TFLOPS
     achieved = w["nf_flops"] / (nf_ms / 1000.0) / 1e12 if nf_ms > 0 else 0.0
-    traffic = None
+    # HBM bytes per near-field launch from the FETCH_SIZE / WRITE_SIZE counter passes recorded
+    # in profiles/ (counters need their own profiler runs)
+    traffic, traffic_src = None, None
     tpath = os.path.join(REPO, "profiles", "near_field_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tr = json.load(f)
+        traffic, traffic_src = tr.get("bytes_per_launch"), f"{tr.get('source')} ({tr.get('kernel')})"
     line = {
         "metric": METRIC,
         "value": round(head["mpts_per_s"], 3),
@@ -300,8 +305,9 @@ def main():
                    "n_points": n, "n_digits": head["n_digits"], "n_per_leaf": head["n_per_leaf"], "nz": head["nz"]},
         "roofline": {"bound": "valu", "engine": "VALU (near-field residual kernel, vector FP32)" if
                      head["precision"] == "float" else "VALU (near-field residual kernel, vector FP64)",
-                     "kernel": "k_near_field", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "kernel": (("k_near_rows" if head["precision"] == "float" else "k_near_rows64")
+                                if head["build"].get("near_rows") else "k_near_field"), "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "flops_per_launch": w["nf_flops"], "pairs_in_support": w["nf_pairs_support"],
                      "pairs_candidate": w["nf_pairs_candidate"], "kernel_ms": round(nf_ms, 4),
                      "kernel_ms_alone": round(head["near_field_alone_ms"], 4),

@@ -76,6 +76,7 @@ def lib():
         "pdmk_gpu_evaluate_complex": (C.c_int, [vp, vp, vp, C.c_int]),
         "pdmk_gpu_build_info": (C.c_int, [vp, dp]),
         "pdmk_gpu_autotune": (C.c_int, [vp, C.c_int, dp]),
+        "pdmk_gpu_evaluate_many": (C.c_int, [vp, C.c_int, vp, vp, vp]),
         "pdmk_gpu_knobs": (C.c_int, [vp, ip, ip]),
         "pdmk_gpu_tune_cache_clear": (None, []),
         "pdmk_gpu_graph_create": (vp, [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_int]),
@@ -329,6 +330,28 @@ class PDMKSolver:
         f = lib().pdmk_gpu_evaluate_float if self.precision == "float" else lib().pdmk_gpu_evaluate_double
         if f(self._h, C.c_void_p(d_charges), C.c_void_p(d_pot), C.c_void_p(stream)) != 0:
             raise RuntimeError("pdmk_gpu_evaluate failed (see stderr)")
+
+    def evaluate_many_device(self, n_vec: int, d_charges: int, d_pot: int, stream: int = 0):
+        """Enqueue evaluate() of n_vec vectors ([n_vec][N] FP64 charges in, Real potentials out,
+        device pointers; pdmk_gpu_evaluate_many)."""
+        if lib().pdmk_gpu_evaluate_many(self._h, int(n_vec), C.c_void_p(d_charges), C.c_void_p(d_pot),
+                                        C.c_void_p(stream)) != 0:
+            raise RuntimeError("pdmk_gpu_evaluate_many failed (see stderr)")
+
+    def evaluate_many(self, charges):
+        """Potentials (n_vec, N) for charges (n_vec, N), host arrays, through pdmk_gpu_evaluate_many."""
+        q = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        if q.ndim != 2 or q.shape[1] != self.n:
+            raise ValueError("charges must have shape (n_vec, N)")
+        qd = DeviceArray.from_numpy(q.reshape(-1))
+        out = DeviceArray(q.size, self.dtype)
+        try:
+            self.evaluate_many_device(q.shape[0], qd.ptr, out.ptr)
+            synchronize()
+            return out.numpy().reshape(q.shape)
+        finally:
+            qd.free()
+            out.free()
 
     def set_single_stream(self, on: bool = True) -> None:
         """evaluate() on the caller's stream only (pdmk_gpu_set_single_stream): for many trees

@@ -11,8 +11,10 @@ Many trees run side by side only on as many hardware queues as the process has
 97 Mpts/s with 4 queues and 172 with 16, graphs 98 and 110; with --single-stream (each
 tree's stages one after another on its own stream) eager calls reach 141 with 4 queues.
 Each graph holds every tree's calls for --per-graph of the vectors; a pass launches
-vectors / per-graph graphs back to back.  The graph's potentials are checked bit for bit
-against the eager pass.  Prints one JSON line: points x vectors per second for both.
+vectors / per-graph graphs back to back.  A third pass evaluates all of a tree's vectors in one
+call (pdmk_gpu_evaluate_many: the near field of 8 vectors per pass).  The graph's and the
+many-vector potentials are checked bit for bit against the eager pass.  Prints one JSON line:
+points x vectors per second for each.
 """
 import argparse
 import json
@@ -72,6 +74,33 @@ def main():
         t_eager.append(time.perf_counter() - t0)
     print(f"[batch] eager pass {1e3 * min(t_eager):.2f} ms", file=sys.stderr, flush=True)
 
+    # every vector of a tree in one call (pdmk_gpu_evaluate_many): contiguous [V][N] buffers
+    many = []
+    for i, (s, qs, outs) in enumerate(trees):
+        n = s.n
+        qall = P.DeviceArray.from_numpy(
+            np.concatenate([W.neutral_uniform_charges(n, 100000 + 32 * i + v) for v in range(args.vectors)]))
+        many.append((s, qall, P.DeviceArray(n * args.vectors, np.float32)))
+
+    def many_pass():
+        for s, qall, oall in many:
+            s.evaluate_many_device(args.vectors, qall.ptr, oall.ptr)
+        P.synchronize()
+
+    many_pass()
+    many_same = all(np.array_equal(oall.numpy().reshape(args.vectors, -1)[v], ref[t][v])
+                    for t, (_, _, oall) in enumerate(many) for v in range(args.vectors))
+    t_many = []
+    for _ in range(args.reps):
+        t0 = time.perf_counter()
+        many_pass()
+        t_many.append(time.perf_counter() - t0)
+    print(f"[batch] evaluate_many pass {1e3 * min(t_many):.2f} ms (equal to eager: {many_same})", file=sys.stderr,
+          flush=True)
+    for _, qall, oall in many:
+        qall.free()
+        oall.free()
+
     t0 = time.perf_counter()
     graphs = []
     for v0 in range(0, args.vectors, args.per_graph):
@@ -104,6 +133,9 @@ def main():
         "graph_ms": round(1e3 * min(t_graph), 3),
         "eager_mpts_per_s": round(total / min(t_eager) / 1e6, 2),
         "graph_mpts_per_s": round(total / min(t_graph) / 1e6, 2),
+        "many_ms": round(1e3 * min(t_many), 3),
+        "many_mpts_per_s": round(total / min(t_many) / 1e6, 2),
+        "many_equals_eager": bool(many_same),
         "graphs": len(graphs),
         "branches": args.branches,
         "single_stream": bool(args.single_stream),
@@ -118,7 +150,7 @@ def main():
         for a in qs + outs:
             a.free()
         s.close()
-    if not same:
+    if not (same and many_same):
         sys.exit(1)
 
 

@@ -6,10 +6,11 @@ problem's features (log10 N, log10 cell condition number, clustering index, n_di
 ``pdmk_tune_features``) and one candidate configuration (log2 n_per_leaf, nz, FP32 or FP64
 kernels).  Training and inference run on the GPU in the library's own kernels
 (``pdmk_mlp_train``, ``pdmk_mlp_forward``; csrc/hip/costmodel.hip).  ``choose`` returns the
-configuration predicted fastest among those whose predicted error is at most twice the
-default configuration's (n_per_leaf 300, nz 4, FP32 kernels up to 6 digits as
+configuration predicted fastest among those with nz >= 4 whose predicted error is at most
+twice the default configuration's (n_per_leaf 300, nz 4, FP32 kernels up to 6 digits as
 AutoPeriodicDMKTree), provided its predicted time is below 0.85 x the default's (else the
-defaults); without a trained model it returns the defaults.
+defaults); without a trained model it returns the defaults.  (The training solves sample
+nz from 2; below 4 the measured error grew by up to 40x, so the choice keeps to nz >= 4.)
 
 The solver-timed solves it was trained on are kept in ``cost_model_solves/`` (JSON lines).
 
@@ -42,7 +43,8 @@ MODEL_PATH = os.path.join(HERE, "cost_model.json")
 N_IN, N_OUT = 7, 2
 DEFAULT = (300, 4)
 NPL_CHOICES = (100, 150, 200, 300, 400, 600, 800)
-NZ_CHOICES = tuple(range(2, 9))
+NZ_CHOICES = tuple(range(2, 9))        # the training solves sample these
+NZ_CHOICES_PICK = tuple(range(4, 9))   # and the choice keeps to nz >= 4 (equal accuracy)
 GAIN_MARGIN = 0.85  # leave the defaults only for a predicted time below 0.85 x theirs
 DATA_DIR = os.path.join(HERE, "cost_model_solves")
 
@@ -194,7 +196,7 @@ def choose(cell, positions, n_digits, blob=None, path=MODEL_PATH, precision=None
         return {"n_per_leaf": DEFAULT[0], "nz": DEFAULT[1], "precision": prec_default, "model": False}
     if precision is not None:
         fl = 1 if precision == "float" else 0
-        cands = [(DEFAULT[0], DEFAULT[1], fl)] + [(a, b, fl) for a in NPL_CHOICES for b in NZ_CHOICES
+        cands = [(DEFAULT[0], DEFAULT[1], fl)] + [(a, b, fl) for a in NPL_CHOICES for b in NZ_CHOICES_PICK
                                                   if (a, b) != DEFAULT]
         ms, err = predict(blob, f4, cands)
         ok = [i for i in range(len(cands)) if err[i] <= 2.0 * err[0]]
@@ -305,12 +307,16 @@ def generate(n_problems: int, seed: int, out_path: str, configs_per_problem: int
                   f"k={prob['clusters']} ({time.time() - t_start:.0f} s)", flush=True)
 
 
-def heldout(n_problems: int, seed: int, out_path: str, blob=None):
+def heldout(n_problems: int, seed: int, out_path: str, blob=None, budget_s: float = 1e9):
     """Model choice against the defaults on new problems: time and error of both."""
     blob = load_model() if blob is None else blob
     rng = np.random.default_rng(seed)
+    t_start = time.time()
     with open(out_path, "a") as f:
         for i in range(n_problems):
+            if time.time() - t_start > budget_s:
+                print(f"[costmodel] time budget reached after {i} problems", flush=True)
+                break
             prob = random_problem(rng)
             ch = choose(prob["cell"], prob["pos"], prob["n_digits"], blob)
             tg, ew = _reference(prob)
@@ -360,6 +366,7 @@ def main(argv=None):
     h.add_argument("--problems", type=int, default=200)
     h.add_argument("--seed", type=int, default=1000003)
     h.add_argument("--out", required=True)
+    h.add_argument("--budget-s", type=float, default=1e9)
     a = ap.parse_args(argv)
     if a.cmd == "generate":
         generate(a.problems, a.seed, a.out, a.configs, a.budget_s)
@@ -370,7 +377,7 @@ def main(argv=None):
                                  "data": [os.path.basename(d) for d in a.data]})
         print(f"[costmodel] trained on {len(rows)} solves: loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
     else:
-        heldout(a.problems, a.seed, a.out)
+        heldout(a.problems, a.seed, a.out, budget_s=a.budget_s)
 
 
 if __name__ == "__main__":

@@ -560,6 +560,21 @@ int pdmk_gpu_stats(void *h, long long *o) {
     return 0;
 }
 
+int pdmk_gpu_evaluate_many(void *h, int n_vec, const void *d_charges, void *d_pot, void *stream) {
+    if (!h || (n_vec > 0 && (!d_charges || !d_pot))) return -1;
+    return guarded("pdmk_gpu_evaluate_many", [&] {
+        auto *H = static_cast<Handle *>(h);
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        if (H->is_float)
+            tree_of<float>(h)->plan().evaluate_many(n_vec, static_cast<const double *>(d_charges),
+                                                    static_cast<float *>(d_pot), st);
+        else
+            tree_of<double>(h)->plan().evaluate_many(n_vec, static_cast<const double *>(d_charges),
+                                                     static_cast<double *>(d_pot), st);
+        return 0;
+    });
+}
+
 int pdmk_gpu_autotune(void *h, int repeats, double *out8) {
     if (!h) return -1;
     return guarded("pdmk_gpu_autotune", [&] {

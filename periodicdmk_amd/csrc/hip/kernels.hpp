@@ -1380,6 +1380,162 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
     if (active) pot_sorted[it.x + lane] += (float)acc;
 }
 
+// ------------------------------------------------ near field of many charge vectors (FP32)
+// qv[i * VC + c] = charge c of sorted source i, for the VC vectors of a pass: vector v0 + c
+// of Q (vector-major, N per vector; zero past the last vector).
+template <int VC>
+__global__ void k_gather_many(const double *__restrict__ Q, int n_vec, int v0, int n, const int *__restrict__ src_user,
+                              int nsall, float *__restrict__ qv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsall) return;
+    const int u = src_user[i];
+#pragma unroll
+    for (int c = 0; c < VC; ++c) qv[(size_t)i * VC + c] = v0 + c < n_vec ? (float)Q[(size_t)(v0 + c) * n + u] : 0.0f;
+}
+
+// ea[c] += q[c](lane 4 g + S) * u for the 8 charges of a row's step: DPP-broadcast FMAs in one
+// asm statement that starts with s_nop 1, the two wait states a DPP read needs after a VALU
+// write of its source (the compiler does not check hazards inside inline asm).
+#define PDMK_FMAC_QUAD(SEL)                                                                                    \
+    asm("s_nop 1\n"                                                                                            \
+        "v_fmac_f32_dpp %0, %8, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"            \
+        "v_fmac_f32_dpp %1, %9, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"            \
+        "v_fmac_f32_dpp %2, %10, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"           \
+        "v_fmac_f32_dpp %3, %11, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"           \
+        "v_fmac_f32_dpp %4, %12, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"           \
+        "v_fmac_f32_dpp %5, %13, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"           \
+        "v_fmac_f32_dpp %6, %14, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"           \
+        "v_fmac_f32_dpp %7, %15, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"           \
+        : "+v"(ea[0]), "+v"(ea[1]), "+v"(ea[2]), "+v"(ea[3]), "+v"(ea[4]), "+v"(ea[5]), "+v"(ea[6]), "+v"(ea[7]) \
+        : "v"(q[0]), "v"(q[1]), "v"(q[2]), "v"(q[3]), "v"(q[4]), "v"(q[5]), "v"(q[6]), "v"(q[7]), "v"(u))
+
+template <int S>
+__device__ __forceinline__ void fmac_quad8(float (&ea)[8], const float (&q)[8], float u) {
+    if constexpr (S == 0) PDMK_FMAC_QUAD("0,0,0,0");
+    if constexpr (S == 1) PDMK_FMAC_QUAD("1,1,1,1");
+    if constexpr (S == 2) PDMK_FMAC_QUAD("2,2,2,2");
+    if constexpr (S == 3) PDMK_FMAC_QUAD("3,3,3,3");
+}
+#undef PDMK_FMAC_QUAD
+
+// k_near_rows for VC = 8 charge vectors at once: the residual of each (target, source) pair is
+// formed once and multiplied into VC sums.  pot_near[c * n + t] (c < vc) receives vector c's
+// near field of sorted target t; each vector's sums are those of k_near_rows on that vector,
+// bit for bit.
+template <int NC, int VC = 8>
+__global__ __launch_bounds__(1024) void k_near_rows_many(
+    const P4<float> *__restrict__ src, const float *__restrict__ qv, const P4<float> *__restrict__ trg,
+    const int4 *__restrict__ items, const int *__restrict__ l1_ptr, const P4<float> *__restrict__ geom,
+    const int *__restrict__ rs_base, const int *__restrict__ rslot, const long long *__restrict__ wave_base,
+    const int *__restrict__ steps, const float *__restrict__ coeffs, float *__restrict__ pot_near, int n, int vc,
+    int item0, int item1, int ne_max) {
+    static_assert(VC == 8, "the DPP FMA block takes 8 vectors");
+    extern __shared__ float4 nf_lds[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = item0 + xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + w;
+    if (item >= item1) return;  // whole wavefront (no workgroup barrier below)
+    const int lane = threadIdx.x & 63, g = lane >> 2;
+    const int4 it = items[item];
+    float4 *geo = nf_lds + (size_t)w * (ne_max + 1) * 6;
+    int *slots = reinterpret_cast<int *>(geo + 2 * (ne_max + 1));
+    const int eb = l1_ptr[it.z], ne = l1_ptr[it.z + 1] - eb;
+    for (int e = lane; e <= ne; e += 64) {
+        const P4<float> gm = e < ne ? geom[eb + e] : P4<float>{0.0f, 0.0f, 0.0f, 0.0f};
+        const float bs = gm.w;
+        geo[2 * e] = make_float4(gm.x, gm.y, gm.z, bs * bs);
+        geo[2 * e + 1] = make_float4(e < ne ? 2.0f / bs : 0.0f, -bs / 2.0f, 0.0f, 0.0f);
+    }
+    const int *rs = rslot + (size_t)rs_base[item] * NF_ROWS;
+    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) slots[k] = rs[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool active = lane < it.y;
+    const P4<float> me = active ? trg[it.x + lane] : P4<float>{1e18f, 1e18f, 1e18f, 0.0f};
+    float c[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    float tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
+    int end = 0, ns = 0;
+    double acc[VC];
+    float ea[VC];
+#pragma unroll
+    for (int v = 0; v < VC; ++v) {
+        acc[v] = 0;
+        ea[v] = 0;
+    }
+    auto enter = [&]() {
+#pragma unroll
+        for (int v = 0; v < VC; ++v) {
+            acc[v] += (double)ea[v];
+            ea[v] = 0;
+        }
+        const int sv = slots[ns * NF_ROWS + g];
+        ++ns;
+        const int o = sv & 255;
+        end = sv >> 8;
+        const float4 a = geo[2 * o], b = geo[2 * o + 1];
+        tx = me.x + a.x;
+        ty = me.y + a.y;
+        tz = me.z + a.z;
+        d2max = a.w;
+        rsc = b.x;
+        cen = b.y;
+    };
+    enter();
+    const int K = it.w;
+    const int *sp = steps + wave_base[item] + lane;
+    int id = sp[0];
+    P4<float> pc = src[id];
+    float qc[VC];
+#pragma unroll
+    for (int v = 0; v < VC; v += 4) *reinterpret_cast<float4 *>(&qc[v]) = *reinterpret_cast<const float4 *>(qv + (size_t)id * VC + v);
+    id = sp[64];
+    for (int k = 0; k < K; k += 4) {
+        const P4<float> pn = src[id];
+        float qn[VC];
+#pragma unroll
+        for (int v = 0; v < VC; v += 4) *reinterpret_cast<float4 *>(&qn[v]) = *reinterpret_cast<const float4 *>(qv + (size_t)id * VC + v);
+        id = sp[(size_t)(k + 8) * NF_ROWS];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (k + s >= end) enter();
+            float px, py, pz;
+            switch (s) {
+            case 0: px = quad_bcast<0>(pc.x); py = quad_bcast<0>(pc.y); pz = quad_bcast<0>(pc.z); break;
+            case 1: px = quad_bcast<1>(pc.x); py = quad_bcast<1>(pc.y); pz = quad_bcast<1>(pc.z); break;
+            case 2: px = quad_bcast<2>(pc.x); py = quad_bcast<2>(pc.y); pz = quad_bcast<2>(pc.z); break;
+            default: px = quad_bcast<3>(pc.x); py = quad_bcast<3>(pc.y); pz = quad_bcast<3>(pc.z); break;
+            }
+            const float dx = px - tx, dy = py - ty, dz = pz - tz;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            float u = 0;
+            if (r2 < d2max && r2 > 1e-30f) {
+                const float rinv = rsqrt_full<float>(r2);
+                const float x = fmaf(r2, rinv, cen) * rsc;
+                float p = c[NC - 1];
+#pragma unroll
+                for (int i = NC - 2; i >= 0; --i) p = fmaf(p, x, c[i]);
+                u = p * rinv;
+            }
+            switch (s) {
+            case 0: fmac_quad8<0>(ea, qc, u); break;
+            case 1: fmac_quad8<1>(ea, qc, u); break;
+            case 2: fmac_quad8<2>(ea, qc, u); break;
+            default: fmac_quad8<3>(ea, qc, u); break;
+            }
+        }
+        pc = pn;
+#pragma unroll
+        for (int v = 0; v < VC; ++v) qc[v] = qn[v];
+    }
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < VC; ++v)
+            if (v < vc) pot_near[(size_t)v * n + it.x + lane] = (float)(acc[v] + (double)ea[v]);
+    }
+}
+
 // The FP64 row kernel (NC coefficients, x form).  The same step lists and slots; a row's 4
 // points of a block reach its lanes through LDS (64-bit DPP has no quad broadcast): each lane
 // writes the point it loaded, then every lane reads its row's point of the step.  Per

@@ -1163,6 +1163,76 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     }
 }
 
+template <typename Real>
+void Plan<Real>::evaluate_many(int n_vec, const double *d_Q, Real *d_P, hipStream_t s) {
+    if (n_vec < 0) throw std::invalid_argument("evaluate_many: negative vector count");
+    if (!s) s = stream;
+    if (qp) throw std::logic_error("evaluate_many: quasi-periodic tree");
+    constexpr int VC = 8;
+    const bool batched = std::is_same<Real, float>::value && nf_rows && n_digits == 6 && n_nf_items > 0;
+    if (!batched) {
+        for (int v = 0; v < n_vec; ++v) evaluate(d_Q + (size_t)v * n, d_P + (size_t)v * n, s);
+        return;
+    }
+    if constexpr (std::is_same<Real, float>::value) {
+        if (!many_qv.p) {
+            many_qv.alloc(sizeof(float) * (size_t)nsall * VC);
+            many_near.alloc(sizeof(float) * (size_t)n * VC);
+            many_far.alloc(sizeof(double) * (size_t)n * VC);
+        }
+        const size_t per_wave = (size_t)(nf_ne_max + 1) * 6 * sizeof(float4);
+        int wpb = 4;
+        while (wpb > 1 && (size_t)wpb * per_wave > 65536) wpb /= 2;
+        for (int v0 = 0; v0 < n_vec; v0 += VC) {
+            const int vc = std::min(VC, n_vec - v0);
+            // near field of the pass's vectors on s_near, far fields on s_far, beside each other
+            check(hipEventRecord(e_in, s), "hipEventRecord");
+            check(hipStreamWaitEvent(s_near, e_in, 0), "hipStreamWaitEvent");
+            check(hipStreamWaitEvent(s_far, e_in, 0), "hipStreamWaitEvent");
+            hipLaunchKernelGGL(k_gather_many<VC>, dim3((nsall + 255) / 256), dim3(256), 0, s_near, d_Q, n_vec, v0, n,
+                               src_user.as<int>(), nsall, many_qv.as<float>());
+            check(hipGetLastError(), "k_gather_many");
+            hipLaunchKernelGGL((k_near_rows_many<12, VC>), dim3((n_nf_items + wpb - 1) / wpb), dim3(64 * wpb),
+                               (size_t)wpb * per_wave, s_near, src.as<P4<float>>(), many_qv.as<float>(),
+                               trg.as<P4<float>>(), nf_items.as<int4>(), nf_ptr.as<int>(), nf_geom.as<P4<float>>(),
+                               nf_rs_base.as<int>(), nf_rslot.as<int>(), nf_wave_base.as<long long>(),
+                               nf_row_steps.as<int>(), coeffs.as<float>(), many_near.as<float>(), n, vc, 0,
+                               n_nf_items, nf_ne_max);
+            check(hipGetLastError(), "k_near_rows_many");
+            check(hipEventRecord(e_near, s_near), "hipEventRecord");
+            for (int c = 0; c < vc; ++c) {
+                const double *q = d_Q + (size_t)(v0 + c) * n;
+                double *fo = many_far.as<double>() + (size_t)c * n;
+                if (far_nufft) nf.run(q, fo, s_far);
+                else ff.run(q, fo, s_far);
+            }
+            check(hipEventRecord(e_far, s_far), "hipEventRecord");
+            // the chain writes the sources' charges the near field reads beside them: it starts
+            // after the near-field pass
+            check(hipStreamWaitEvent(s, e_near, 0), "hipStreamWaitEvent");
+            e_upward = nullptr;
+            for (int c = 0; c < vc; ++c) {
+                const double *q = d_Q + (size_t)(v0 + c) * n;
+                const int m = std::max(nsall, n);
+                hipLaunchKernelGGL(k_gather_charges<Real>, dim3((m + 255) / 256), dim3(256), 0, s, q, src_user.as<int>(),
+                                   src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), pot_near.as<Real>(), n,
+                                   (const double2 *)nullptr, 0);
+                check(hipGetLastError(), "k_gather_charges");
+                switch (p) {
+                case 18: launch_transforms<Real, 18, 27>(*this, s); break;
+                default: throw std::logic_error("evaluate_many: unexpected proxy order");
+                }
+                if (c == 0) check(hipStreamWaitEvent(s, e_far, 0), "hipStreamWaitEvent");
+                hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(),
+                                   many_near.as<float>() + (size_t)c * n, trg_w0.as<double>(), trg_user.as<int>(), q,
+                                   many_far.as<double>() + (size_t)c * n, far_idx.as<int>(),
+                                   d_P + (size_t)(v0 + c) * n, n, 1, 1, 1);
+                check(hipGetLastError(), "k_finalize");
+            }
+        }
+    }
+}
+
 namespace {
 
 // Charges of the tuning runs: a fixed neutral-ish pattern (the timings do not depend on them).

@@ -208,6 +208,14 @@ struct Plan {
     // host buffers in, host buffers out
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
 
+    // evaluate() of n_vec charge vectors (d_Q, d_P: vector-major, n per vector) on `s`.  FP32
+    // plans with row step lists form the near field of 8 vectors per pass (each pair's residual
+    // once), beside the far fields, then run the proxy / plane-wave chain per vector; other
+    // plans call evaluate() per vector.  Each vector's potentials equal evaluate()'s on it.
+    // The per-vector buffers grow on first use to 8 vectors' worth.
+    void evaluate_many(int n_vec, const double *d_Q, Real *d_P, hipStream_t s);
+    DevBuf many_qv, many_near, many_far;
+
     // Launch shapes: set at construction from the per-device cache when a plan of the same
     // kind (order, precision, size buckets) was tuned before in this process (or in the file
     // PDMK_TUNE_CACHE names); autotune() times evaluate() on this plan over each knob's
