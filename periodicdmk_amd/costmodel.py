@@ -218,14 +218,14 @@ def choose(cell, positions, n_digits, blob=None, path=MODEL_PATH, precision=None
 
 
 # ------------------------------------------------------------------ solver-timed solves
-def random_problem(rng):
+def random_problem(rng, n_min: float = 1e4):
     """A random cell (cubic, or triclinic with off-diagonals U[0, 0.4] and diagonal U[0.7, 1.4]),
-    N log-uniform in [1e4, 4e6] (at most 1e6 at 12 digits, 2e6 at 9), positions from a mixture of
+    N log-uniform in [n_min, 4e6] (at most 1e6 at 12 digits, 2e6 at 9), positions from a mixture of
     1, 8, 64 or 512 Gaussians (sigma log-uniform in [0.005, 0.05] of the cell; 1 = uniform),
     neutral U[-1, 1] charges and n_digits in {3, 6, 9, 12}."""
     digits = int(rng.choice([3, 6, 9, 12]))
     n_max = {3: 4e6, 6: 4e6, 9: 2e6, 12: 1e6}[digits]
-    n = int(np.exp(rng.uniform(np.log(1e4), np.log(n_max))))
+    n = int(np.exp(rng.uniform(np.log(min(n_min, n_max)), np.log(n_max))))
     cell = np.eye(3)
     if rng.uniform() < 0.5:
         cell = np.diag(rng.uniform(0.7, 1.4, 3))
@@ -307,17 +307,19 @@ def generate(n_problems: int, seed: int, out_path: str, configs_per_problem: int
                   f"k={prob['clusters']} ({time.time() - t_start:.0f} s)", flush=True)
 
 
-def heldout(n_problems: int, seed: int, out_path: str, blob=None, budget_s: float = 1e9):
-    """Model choice against the defaults on new problems: time and error of both."""
+def heldout(n_problems: int, seed: int, out_path: str, blob=None, budget_s: float = 1e9, n_min: float = 1e4):
+    """Model choice against the defaults on new problems: time and error of both (one JSON
+    line per problem appended to out_path; returns the records)."""
     blob = load_model() if blob is None else blob
     rng = np.random.default_rng(seed)
     t_start = time.time()
+    recs = []
     with open(out_path, "a") as f:
         for i in range(n_problems):
             if time.time() - t_start > budget_s:
                 print(f"[costmodel] time budget reached after {i} problems", flush=True)
                 break
-            prob = random_problem(rng)
+            prob = random_problem(rng, n_min)
             ch = choose(prob["cell"], prob["pos"], prob["n_digits"], blob)
             tg, ew = _reference(prob)
             prec_d = "float" if prob["n_digits"] <= 6 else "double"
@@ -332,9 +334,11 @@ def heldout(n_problems: int, seed: int, out_path: str, blob=None, budget_s: floa
                 rec["failed"] = str(e)
             f.write(json.dumps(rec) + "\n")
             f.flush()
+            recs.append(rec)
             print(f"[costmodel] held-out {i}: {rec.get('choice')} "
                   f"default {rec.get('default', {}).get('ms', float('nan')):.2f} ms "
                   f"model {rec.get('model', {}).get('ms', float('nan')):.2f} ms", flush=True)
+    return recs
 
 
 def read_records(paths):

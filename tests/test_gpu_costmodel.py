@@ -60,3 +60,22 @@ def test_training_reduces_loss_and_is_deterministic(pdmk, cuda):
     # the reported loss is the loss of the parameters before each step
     loss_end, _ = costmodel.loss_grad(p1, X, T, M)
     assert loss_end <= h1[-1] * 1.05
+
+
+def test_trained_model_gain(pdmk, cuda, tmp_path):
+    """What the trained network gains over the defaults: 6 new random problems (generator seed
+    not used for the training solves, N >= 2e5 so that the times are stable), each solved
+    with the model's choice and with the defaults; the model's total time is at least 5% lower
+    and every choice keeps the error within 2x of the defaults' (against the FP64 Ewald sum
+    at sampled points)."""
+    from periodicdmk_amd import costmodel
+    blob = costmodel.load_model()
+    assert blob is not None, "the trained cost model ships with the package"
+    recs = costmodel.heldout(6, 271828, str(tmp_path / "gain.jsonl"), blob=blob, n_min=2e5)
+    assert len(recs) == 6 and all("failed" not in r for r in recs)
+    t_def = sum(r["default"]["ms"] for r in recs)
+    t_mod = sum(r["model"]["ms"] for r in recs)
+    print(f"model {t_mod:.2f} ms, defaults {t_def:.2f} ms", flush=True)
+    for r in recs:
+        assert r["model"]["err"] <= 2.0 * r["default"]["err"] + 1e-15, r
+    assert t_def >= 1.05 * t_mod
