@@ -165,9 +165,10 @@ int pdmk_gpu_build_info(void* h, double* out8);
 
 /// Algorithmic work per evaluate(): out[0..7] = {near-field pairs in support,
 /// near-field candidate pairs, near-field flops (in-support pairs x (2*NC+12)),
-/// proxy/PW transform flops, far-field mode-point products, far-field flops,
-/// bytes of O(N) traffic, near-field kernel HBM bytes}.  Near-field pair counts are
-/// computed on the host at build time.
+/// proxy/PW transform flops, far-field mode-point products (of the exact sum), far-field
+/// flops (of the exact sum, or of the NUFFT when the plan uses it: 4 W^3 per point plus the
+/// pruned axis DFTs), bytes of O(N) traffic, near-field kernel HBM bytes}.  Near-field pair
+/// counts are computed on the host at build time.
 int pdmk_gpu_work(void* h, double* out8);
 
 /// Kernel-group timing with HIP events on the evaluate stream (measurement aid; no

@@ -75,14 +75,22 @@ __global__ void k_gather_charges(const double *__restrict__ q, const int *__rest
 }
 
 // pot[stride u] = pot_sorted[i] - w0_i q_u + far[u], u = trg_user[i] (stride 2: one component
-// of an interleaved complex output)
+// of an interleaved complex output).  Vector v = blockIdx.z of a many-vector launch uses the
+// arrays offset by v times the vs_* element strides (all zero for one vector).
 template <class R>
 __global__ void k_finalize(const R *__restrict__ pot_sorted, const R *__restrict__ pot_near, const double *__restrict__ w0,
                            const int *__restrict__ trg_user, const double *__restrict__ q, const double *__restrict__ far,
                            const int *__restrict__ far_idx, R *__restrict__ pot, int n, int use_self, int use_far,
-                           int stride) {
+                           int stride, long long vs_sorted = 0, long long vs_near = 0, long long vs_q = 0,
+                           long long vs_far = 0, long long vs_pot = 0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const long long vz = blockIdx.z;
+    pot_sorted += vz * vs_sorted;
+    pot_near += vz * vs_near;
+    q += vz * vs_q;
+    far += vz * vs_far;
+    pot += vz * vs_pot;
     const int u = trg_user[i];
     double v = (double)pot_sorted[i] + (double)pot_near[i];
     if (use_self) v -= w0[i] * q[u];
@@ -113,9 +121,13 @@ template <class R, int P>
 __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
     const P4<R> *__restrict__ src, const int *__restrict__ grp_slot, const R *__restrict__ grp_geom,
     const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
-    const R *__restrict__ rng_off, R *__restrict__ proxy) {
+    const R *__restrict__ rng_off, R *__restrict__ proxy, const R *__restrict__ qv = nullptr, int qv_ld = 0,
+    long long vs_out = 0) {
     chain_priority();
     constexpr int RB = C2pCfg<P>::RB, NB = C2pCfg<P>::NB, NT = C2pCfg<P>::threads;
+    // many-vector launch: vector blockIdx.z takes its charges from qv[source * qv_ld + v]
+    // (src[].w otherwise) and writes its own proxy array, vs_out elements further on
+    proxy += (long long)blockIdx.z * vs_out;
     constexpr int PX = (P + 3) & ~3, PY = NB * RB;  // padded table rows
     __shared__ R Tx[C2P_TS * PX], Ty[C2P_TS * PY], Tz[C2P_TS * P];
     const int g = blockIdx.x;
@@ -141,7 +153,7 @@ __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
                     if (s < ns) {
                         const P4<R> pt = src[beg + s0 + s];
                         const R x = sc * ((d == 0 ? pt.x : (d == 1 ? pt.y : pt.z)) + rng_off[4 * r + d]);
-                        const R m = (d == 2) ? pt.w : R(1);
+                        const R m = (d == 2) ? (qv ? qv[(size_t)(beg + s0 + s) * qv_ld + blockIdx.z] : pt.w) : R(1);
                         R t0 = m, t1 = x * m;
                         T[0] = t0;
                         if (P > 1) T[1] = t1;
@@ -404,8 +416,11 @@ template <class R, int P, bool TRANSPOSE>
 __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
     const R *__restrict__ in_base, R *__restrict__ out_base, const int *__restrict__ item_out,
     const int *__restrict__ item_add, const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
-    const int *__restrict__ in_code, const R *__restrict__ mats_rm, int n_items, int kc) {
+    const int *__restrict__ in_code, const R *__restrict__ mats_rm, int n_items, int kc, long long vs_in = 0,
+    long long vs_out = 0) {
     chain_priority();
+    in_base += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    out_base += (long long)blockIdx.z * vs_out;
     using Cfg = T3Cfg<R, P>;
     constexpr int PP = P * P, TS = Cfg::TS, KC = Cfg::KC, NT = Cfg::threads, SG = Cfg::SG, PS = Cfg::PS;
     constexpr int PR = P;  // row stride of the matrices
@@ -498,8 +513,10 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
 template <class R, int P>
 __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, const R *__restrict__ scratch,
                                                        const int *__restrict__ parent_slot, const int *__restrict__ child_ptr,
-                                                       int n_parents) {
+                                                       int n_parents, long long vs_proxy = 0, long long vs_scratch = 0) {
     chain_priority();
+    proxy += (long long)blockIdx.z * vs_proxy;  // vector blockIdx.z of a many-vector launch
+    scratch += (long long)blockIdx.z * vs_scratch;
     constexpr int P3 = P * P * P;
     const int it = blockIdx.x, e = blockIdx.y * blockDim.x + threadIdx.x;
     if (it >= n_parents || e >= P3) return;
@@ -522,8 +539,11 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // slot slot0 + x and writes plane-wave slot x.
 template <class R, int P, int NPW, int BK = 0>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
-                                                   const Cx<R> *__restrict__ W, int slot0, int n_boxes) {
+                                                   const Cx<R> *__restrict__ W, int slot0, int n_boxes,
+                                                   long long vs_in = 0, long long vs_out = 0) {
     chain_priority();
+    proxy += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    pw_out += (long long)blockIdx.z * vs_out;
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, TS = OddStride<P>::v;
     constexpr int K3 = P2pwCfg<R, P, NPW, BK>::K3;
     __shared__ Cx<R> T1[K3 * PP];        // [k][b][a]
@@ -620,8 +640,11 @@ template <class R>
 __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_out, const Cx<R> *__restrict__ shift0,
                                                    R lscale, const int *__restrict__ nbr_ptr,
                                                    const int *__restrict__ nbr_slot, const int *__restrict__ nbr_sten,
-                                                   int slot0, int nm, int item0, Cx<R> *__restrict__ pin) {
+                                                   int slot0, int nm, int item0, Cx<R> *__restrict__ pin,
+                                                   long long vs_in = 0, long long vs_out = 0) {
     chain_priority();
+    pw_out += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    pin += (long long)blockIdx.z * vs_out;
     // x = target box (consecutive workgroups share neighbours; each XCD takes a contiguous,
     // spatially coherent range of boxes), y = mode chunk: the neighbours' chunk is read from
     // L2 by the 27 boxes around it instead of from memory
@@ -644,8 +667,11 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
 template <class R, int P, int NPW, int BK = 0>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
-                                                   int item0, int n_items, R *__restrict__ down_base) {
+                                                   int item0, int n_items, R *__restrict__ down_base,
+                                                   long long vs_in = 0, long long vs_out = 0) {
     chain_priority();
+    pin += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    down_base += (long long)blockIdx.z * vs_out;
     constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, half = NPW / 2, TS2 = OddStride<P>::v;
     constexpr int AC = Pw2pCfg<R, P, NPW, BK>::AC;
     __shared__ Cx<R> U1[AC * COL];       // [a][k3][k2]
@@ -784,8 +810,11 @@ constexpr int EV_THREADS = 128, EV_TPT = 2;
 template <class R, int P>
 __global__ __launch_bounds__(EV_THREADS) void k_eval_targets(const R *__restrict__ down_base, const int *__restrict__ leaf_down,
                                                               const R *__restrict__ leaf_geom, const int *__restrict__ leaf_trg,
-                                                              const P4<R> *__restrict__ trg, R *__restrict__ pot_sorted) {
+                                                              const P4<R> *__restrict__ trg, R *__restrict__ pot_sorted,
+                                                              long long vs_in = 0, long long vs_out = 0) {
     chain_priority();
+    down_base += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    pot_sorted += (long long)blockIdx.z * vs_out;
     constexpr int PP = P * P;
     __shared__ R plane[PP];
     const int lf = blockIdx.x;
@@ -877,6 +906,14 @@ template <int P> struct EvMfmaCfg {
     static constexpr int TT = 2;                   // target tiles per stage
     static constexpr int NT = 16 * TT;             // targets per stage
     static constexpr int PA = 32;
+    // LDS row strides, in elements: the A reads (16 targets x 2 a per half wavefront) at a
+    // stride of 34 and the T_b / T_c reads (2 targets x 16 (c, b) columns) at 48 fall on
+    // distinct banks; the stride of 32 put the 16 targets of an A read on the same banks
+    // (measured at N=1e6, 9 digits: 3.3x fewer LDS bank-conflict cycles, 20% fewer busy
+    // cycles; an XOR-swizzled layout removed more conflicts but its address arithmetic cost
+    // more than it saved)
+    static constexpr int LX = 34, LY = 48;
+    static_assert(P <= LY, "order above the table stride");
 };
 // One workgroup per leaf.  Wave w owns the (c, b) tiles w, w + NW, ...: their B operands
 // (the leaf's proxy rows) are loaded into registers once per leaf, and the targets pass by
@@ -889,7 +926,8 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
     using C = EvMfmaCfg<P>;
     constexpr int RT = C::RT, NW = C::NW, MT = C::MT, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
     static_assert(P <= PA, "order above the padded a count");
-    __shared__ double Tx[NT * PA], Ty[NT * P], Tz[NT * P], red[NW * NT];
+    constexpr int LX = C::LX, LY = C::LY;
+    __shared__ double Tx[NT * LX], Ty[NT * LY], Tz[NT * LY], red[NW * NT];
     const int lf = blockIdx.x, lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, jj = lane & 15;
@@ -911,7 +949,7 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
         __syncthreads();
         for (int task = threadIdx.x; task < 3 * NT; task += blockDim.x) {
             const int t = task % NT, d = task / NT;
-            double *T = d == 0 ? Tx + t * PA : (d == 1 ? Ty + t * P : Tz + t * P);
+            double *T = d == 0 ? Tx + t * LX : (d == 1 ? Ty + t * LY : Tz + t * LY);
             const int len = d == 0 ? PA : P;
             if (t0 + t < cnt) {
                 const P4<double> pt = trg[beg + t0 + t];
@@ -935,7 +973,7 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
 #pragma unroll
         for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
-            for (int ks = 0; ks < PA / 4; ++ks) A[tt][ks] = Tx[(tt * 16 + jj) * PA + 4 * ks + kk];
+            for (int ks = 0; ks < PA / 4; ++ks) A[tt][ks] = Tx[(tt * 16 + jj) * LX + 4 * ks + kk];
         double part[TT][4];
 #pragma unroll
         for (int tt = 0; tt < TT; ++tt)
@@ -961,7 +999,7 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int t = tt * 16 + kk + 4 * e;
-                    part[tt][e] = fma(D[tt][e], Ty[t * P + b] * Tz[t * P + c], part[tt][e]);
+                    part[tt][e] = fma(D[tt][e], Ty[t * LY + b] * Tz[t * LY + c], part[tt][e]);
                 }
         }
         // sum over the 16 lanes of each row (the (c, b) columns), then over the waves
@@ -1667,8 +1705,11 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__ theta, const double *__restrict__ q, int n,
                                                        int ms, int mt, int mu, const int *__restrict__ chunk_ijk,
                                                        const int *__restrict__ chunk_m0, const int *__restrict__ chunk_len,
-                                                       int n_chunks, int n_modes, double2 *__restrict__ Sp) {
+                                                       int n_chunks, int n_modes, double2 *__restrict__ Sp,
+                                                       long long vs_q = 0, long long vs_sp = 0) {
     extern __shared__ double2 ff_lds[];
+    q += (long long)blockIdx.z * vs_q;  // vector blockIdx.z of a many-vector launch
+    Sp += (long long)blockIdx.z * vs_sp;
     double2 *E1 = ff_lds, *E2 = E1 + FF_TP * ms, *E3 = E2 + FF_TP * mt, *Est = E3 + FF_TP * mu;
     const int c = blockIdx.y * blockDim.x + threadIdx.x;
     const bool active = c < n_chunks;
@@ -1724,8 +1765,11 @@ __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__
 
 // S[pidx[m]] = w[m] * sum_blk Sp[blk][m]: workgroup = 32 modes x 8 partial-sum lanes.
 __global__ __launch_bounds__(256) void k_ff_reduce(const double2 *__restrict__ Sp, int nblk, const double *__restrict__ w,
-                                                   const int *__restrict__ pidx, int n_modes, double2 *__restrict__ S) {
+                                                   const int *__restrict__ pidx, int n_modes, double2 *__restrict__ S,
+                                                   long long vs_sp = 0, long long vs_s = 0) {
     __shared__ double2 part[8][32];
+    Sp += (long long)blockIdx.z * vs_sp;  // vector blockIdx.z of a many-vector launch
+    S += (long long)blockIdx.z * vs_s;
     const int ml = threadIdx.x & 31, g = threadIdx.x >> 5;
     const int m = blockIdx.x * 32 + ml;
     double sr = 0, si = 0;
@@ -1767,7 +1811,9 @@ __device__ __forceinline__ void ff_advance(double2 &e, int d, double2 z) {
 // line by (small) exponent steps.
 __global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ theta, int n, const int4 *__restrict__ lines,
                                                  const int *__restrict__ base, int n_lines, const double2 *__restrict__ S,
-                                                 double *__restrict__ far) {
+                                                 double *__restrict__ far, long long vs_s = 0, long long vs_far = 0) {
+    S += (long long)blockIdx.z * vs_s;  // vector blockIdx.z of a many-vector launch
+    far += (long long)blockIdx.z * vs_far;
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     const int uc = u < n ? u : n - 1;
     double2 z1, z2, z3;
@@ -2047,29 +2093,53 @@ __global__ __launch_bounds__(64) void k_nufft_gather(const double *__restrict__ 
 }
 
 // out[a][k][b] = sum_l in[a][l][b] mat[k L + l] (complex matrix; real or complex input and
-// output, the real output keeping the real part).
+// output, the real output keeping the real part).  A lane owns one line (a, b) (b fastest, so
+// the loads of a wavefront are contiguous when B > 1) and DFT_KC outputs k of it, grid y =
+// the chunk of k: each input element read feeds DFT_KC complex multiply-adds, and the matrix
+// entries are wave-uniform loads.  Each output is summed over l in order.
+constexpr int DFT_KC = 8;
+
 template <bool RIN, bool ROUT>
 __global__ __launch_bounds__(256) void k_dft_axis(const void *__restrict__ in, void *__restrict__ out,
                                                   const double2 *__restrict__ mat, int A, int L, int K, int B) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)A * K * B) return;
-    const int b = (int)(idx % B), k = (int)((idx / B) % K), a = (int)(idx / ((long)B * K));
-    double sr = 0, si = 0;
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long)A * B) return;
+    const int b = (int)(col % B);
+    const long a = col / B;
+    const int k0 = blockIdx.y * DFT_KC;
+    int row[DFT_KC];
+#pragma unroll
+    for (int j = 0; j < DFT_KC; ++j) row[j] = min(k0 + j, K - 1) * L;  // clamped rows are not stored
+    double sr[DFT_KC], si[DFT_KC];
+#pragma unroll
+    for (int j = 0; j < DFT_KC; ++j) sr[j] = si[j] = 0.0;
     for (int l = 0; l < L; ++l) {
-        const double2 m = mat[(size_t)k * L + l];
         const size_t ii = ((size_t)a * L + l) * B + b;
         if (RIN) {
             const double x = static_cast<const double *>(in)[ii];
-            sr = fma(x, m.x, sr);
-            si = fma(x, m.y, si);
+#pragma unroll
+            for (int j = 0; j < DFT_KC; ++j) {
+                const double2 m = mat[row[j] + l];
+                sr[j] = fma(x, m.x, sr[j]);
+                si[j] = fma(x, m.y, si[j]);
+            }
         } else {
             const double2 x = static_cast<const double2 *>(in)[ii];
-            sr = fma(x.x, m.x, fma(-x.y, m.y, sr));
-            si = fma(x.x, m.y, fma(x.y, m.x, si));
+#pragma unroll
+            for (int j = 0; j < DFT_KC; ++j) {
+                const double2 m = mat[row[j] + l];
+                sr[j] = fma(x.x, m.x, fma(-x.y, m.y, sr[j]));
+                si[j] = fma(x.x, m.y, fma(x.y, m.x, si[j]));
+            }
         }
     }
-    if (ROUT) static_cast<double *>(out)[idx] = sr;
-    else static_cast<double2 *>(out)[idx] = make_double2(sr, si);
+#pragma unroll
+    for (int j = 0; j < DFT_KC; ++j) {
+        if (k0 + j >= K) break;
+        const size_t o = ((size_t)a * K + k0 + j) * B + b;
+        if (ROUT) static_cast<double *>(out)[o] = sr[j];
+        else static_cast<double2 *>(out)[o] = make_double2(sr[j], si[j]);
+    }
 }
 
 // C[pos[i]] = cw[i] B[pos[i]] for the listed modes (the other entries of C stay zero).

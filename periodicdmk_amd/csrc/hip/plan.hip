@@ -417,7 +417,10 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         poly2pw.upload(a);
         pw2poly.upload(b);
         shift0.upload(s);
-        work.transform_flops += (double)n_up * 8.0 * ((double)p * p * H * p + (double)p * n_pw * H * p + (double)NM * p);
+        // per box: z step from the real proxy (one multiply-add per term), y and x steps on
+        // complex values against matrix entries that are real or imaginary (two per term)
+        work.transform_flops += (double)n_up * (2.0 * (double)p * p * H * p + 4.0 * (double)p * n_pw * H * p +
+                                                4.0 * (double)NM * p);
     }
 
     // ---------------- downward per level
@@ -468,8 +471,10 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         E.nbr_slot.upload(nslot);
         E.nbr_sten.upload(nsten);
         work.transform_flops += 8.0 * (double)nslot.size() * NM;
-        work.transform_flops += (double)E.n_items * 8.0 *
-                                ((double)p * n_pw * H * n_pw + (double)p * p * H * n_pw + (double)P3 * H / 2.0);
+        // per box: x and y steps complex against real-or-imaginary entries (two multiply-adds
+        // per term), z step to a real output (one)
+        work.transform_flops += (double)E.n_items *
+                                (4.0 * (double)p * n_pw * H * n_pw + 4.0 * (double)p * p * H * n_pw + 2.0 * (double)P3 * H);
         eval_levels.push_back(std::move(E));
         p2c_levels.push_back(make_t3(out, add, ptr, in, code));
     }
@@ -743,7 +748,9 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             nf.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w, std::max(4, std::min(14, n_digits + 2)));
             for (int i = 0; i < n; ++i) fidx[i] = nf.position[tu[i]];
         } else {
-            ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w);
+            // at most 512 point blocks: each writes a partial sum per mode that the reduction
+            // reads back (2048 blocks at N = 2e5, 6 digits, made 95 MB of partial sums per call)
+            ff.setup(th, n, ms, mt, mu, T.mode_ijk, T.mode_w, nullptr, 512);
         }
         far_idx.upload(fidx);
         if (qp) {
@@ -781,6 +788,15 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         build_ms[2] = ms_since(t_far);
         work.ff_mode_points = 2.0 * (double)n * n_ff_modes;
         work.ff_flops = 8.0 * (double)n * n_ff_modes + 4.0 * (double)n * n_ff_modes;
+        if (far_nufft) {
+            // the NUFFT's own work: spreading and interpolation (a multiply-add per point and
+            // grid cell of the W^3 support), the pruned axis DFTs both ways (a complex
+            // multiply-add per input and output pair along each axis), the mode weights
+            const double W3 = (double)nf.W * nf.W * nf.W;
+            const double g0 = nf.g[0], g1 = nf.g[1], g2 = nf.g[2], M0 = nf.M[0], M1 = nf.M[1], M2 = nf.M[2];
+            const double dft = 8.0 * (g2 * g1 * g0 * M0 + g2 * g1 * M0 * M1 + g2 * M1 * M0 * M2);
+            work.ff_flops = 4.0 * (double)n * W3 + 2.0 * dft + 8.0 * (double)n_ff_modes;
+        }
     }
 
     // ---------------- incoming plane-wave scratch: one level's boxes at a time, in batches
@@ -860,6 +876,10 @@ void Plan<Real>::release_handles() {
         if (*e) (void)hipEventDestroy(*e);
         *e = nullptr;
     }
+    for (auto &e : many_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     for (hipStream_t *st : {&stream, &s_far, &s_near}) {
         if (*st) (void)hipStreamDestroy(*st);
         *st = nullptr;
@@ -873,58 +893,84 @@ Plan<Real>::~Plan() {
 
 namespace {
 
+// The arrays one pass of the proxy / plane-wave chain works on: the plan's own (one vector,
+// charges in src[].w), or nv vectors' copies, vector v at v times the element strides, with
+// charges qv[source * qv_ld + v] (the kernels take v from blockIdx.z).
+template <class Real> struct ChainBufs {
+    Real *up, *c2p, *down, *pot;
+    Cx<Real> *pw, *pin;
+    const Real *qv = nullptr;
+    int qv_ld = 0, nv = 1;
+    long long s_up = 0, s_c2p = 0, s_pw = 0, s_pin = 0, s_down = 0, s_pot = 0;
+};
+
+template <class Real> ChainBufs<Real> own_chain(Plan<Real> &pl) {
+    ChainBufs<Real> b;
+    b.up = pl.proxy_up.template as<Real>();
+    b.c2p = pl.c2p_buffer();
+    b.down = pl.proxy_down.template as<Real>();
+    b.pot = pl.pot_sorted.template as<Real>();
+    b.pw = pl.pw_out.template as<Cx<Real>>();
+    b.pin = pl.pin.template as<Cx<Real>>();
+    return b;
+}
+
 // c2p / p2c of one level: output planes are split into slices so that a level with few
-// boxes still spreads over the whole GPU (about 1024 workgroups or more)
+// boxes still spreads over the whole GPU (about 1024 workgroups or more); nv vectors along z
 template <class Real, int P, bool TRANSPOSE>
 void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launch &L, const Real *mats_rm,
-                    hipStream_t s, int div) {
+                    hipStream_t s, int div, int nv = 1, long long vs_in = 0, long long vs_out = 0) {
     if (L.n_items == 0) return;
     constexpr int KC = T3Cfg<Real, P>::KC;
     int kc = std::max(1, KC / std::max(1, div));
     while (kc > 1 && (long)L.n_items * ((P + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
-    const dim3 grid(L.n_items * ((P + kc - 1) / kc));  // (item, slice) pairs
+    const dim3 grid(L.n_items * ((P + kc - 1) / kc), 1, nv);  // (item, slice) pairs
     hipLaunchKernelGGL((k_tensor3<Real, P, TRANSPOSE>), grid, dim3(T3Cfg<Real, P>::threads), 0, s, in, out,
                        L.out.template as<int>(), L.add.template as<int>(), L.ptr.template as<int>(),
-                       L.in_slot.template as<int>(), L.in_code.template as<int>(), mats_rm, L.n_items, kc);
+                       L.in_slot.template as<int>(), L.in_code.template as<int>(), mats_rm, L.n_items, kc, vs_in,
+                       vs_out);
     check(hipGetLastError(), "k_tensor3");
 }
 
 template <class Real, int P, int NPW>
-void launch_transforms(Plan<Real> &pl, hipStream_t s) {
+void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) {
     using C = Cx<Real>;
     const int NM = NPW * NPW * ((NPW + 1) / 2);
+    const int nv = B.nv;
+    if (nv < 1 || nv > 65535) throw std::invalid_argument("launch_transforms: vector count");
     if (pl.n_up > 0) {
         // FP64 at P = 28 (9 digits): matrix cores (a padded to 32 costs 1/8).  FP64 at P = 18
         // pads too much.  FP32 at P = 18 on the matrix cores is faster alone (395 -> 307 us at
         // N=1e6) but slower inside evaluate() (3.02 -> 3.24 ms): its 8-wave workgroups take
-        // wave slots from the near field beside it; it stays on the vector ALUs.
+        // wave slots from the near field beside it; it stays on the vector ALUs (and the
+        // many-vector chain with it: the same sums in the same order as one vector's).
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
+            if (nv != 1 || B.qv) throw std::logic_error("launch_transforms: one vector on the matrix-core kernels");
             hipLaunchKernelGGL((k_charge2proxy_mfma<Real, P>), dim3(pl.n_up), dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
                                pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
                                pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
-                               pl.rng_off.template as<Real>(), pl.proxy_up.template as<Real>());
+                               pl.rng_off.template as<Real>(), B.up);
             check(hipGetLastError(), "k_charge2proxy_mfma");
         } else {
-            hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up), dim3(C2pCfg<P>::threads), 0, s,
+            hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up, 1, nv), dim3(C2pCfg<P>::threads), 0, s,
                                pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
                                pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
-                               pl.rng_off.template as<Real>(), pl.proxy_up.template as<Real>());
+                               pl.rng_off.template as<Real>(), B.up, B.qv, B.qv_ld, B.s_up);
             check(hipGetLastError(), "k_charge2proxy");
         }
     }
     for (auto &C : pl.c2p_levels) {
-        launch_tensor3<Real, P, true>(pl.proxy_up.template as<Real>(), pl.c2p_buffer(), C.children,
-                                      pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div);
-        hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256), dim3(256), 0, s,
-                           pl.proxy_up.template as<Real>(), pl.c2p_buffer(),
-                           C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents);
+        launch_tensor3<Real, P, true>(B.up, B.c2p, C.children, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, nv,
+                                      B.s_up, B.s_c2p);
+        hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256, nv), dim3(256), 0, s,
+                           B.up, B.c2p, C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents,
+                           B.s_up, B.s_c2p);
         check(hipGetLastError(), "k_sum_children");
     }
     auto proxy2pw_bk = [&](auto bk, int slot0, int count) {
         constexpr int BK = decltype(bk)::value;
-        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW, BK>), dim3(count * P2pwCfg<Real, P, NPW, BK>::slices), dim3(256), 0,
-                           s, pl.proxy_up.template as<Real>(), pl.pw_out.template as<C>(), pl.poly2pw.template as<C>(),
-                           slot0, count);
+        hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW, BK>), dim3(count * P2pwCfg<Real, P, NPW, BK>::slices, 1, nv),
+                           dim3(256), 0, s, B.up, B.pw, pl.poly2pw.template as<C>(), slot0, count, B.s_up, B.s_pw);
         check(hipGetLastError(), "k_proxy2pw");
     };
     auto proxy2pw = [&](int slot0, int count) {
@@ -937,9 +983,9 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
     };
     auto pw2proxy_bk = [&](auto bk, const typename Plan<Real>::EvalLevel &E, int i0, int nb) {
         constexpr int BK = decltype(bk)::value;
-        hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW, BK>), dim3(nb * Pw2pCfg<Real, P, NPW, BK>::slices), dim3(256), 0, s,
-                           pl.pin.template as<C>(), pl.pw2poly.template as<C>(), E.down.template as<int>(),
-                           E.zero.template as<int>(), i0, nb, pl.proxy_down.template as<Real>());
+        hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW, BK>), dim3(nb * Pw2pCfg<Real, P, NPW, BK>::slices, 1, nv), dim3(256),
+                           0, s, B.pin, pl.pw2poly.template as<C>(), E.down.template as<int>(),
+                           E.zero.template as<int>(), i0, nb, B.down, B.s_pin, B.s_down);
         check(hipGetLastError(), "k_pw2proxy");
     };
     auto pw2proxy = [&](const typename Plan<Real>::EvalLevel &E, int i0, int nb) {
@@ -959,29 +1005,27 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s) {
         const int slot0 = pl.pw_by_level ? E.pw_lo : 0;
         for (int i0 = 0; i0 < E.n_items; i0 += pl.pin_batch) {
             const int nb = std::min(pl.pin_batch, E.n_items - i0);
-            hipLaunchKernelGGL(k_pw_shift<Real>, dim3(nb, (NM + 255) / 256), dim3(256), 0, s, pl.pw_out.template as<C>(),
+            hipLaunchKernelGGL(k_pw_shift<Real>, dim3(nb, (NM + 255) / 256, nv), dim3(256), 0, s, B.pw,
                                pl.shift0.template as<C>(), lscale, E.nbr_ptr.template as<int>(),
-                               E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0,
-                               pl.pin.template as<C>());
+                               E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0, B.pin,
+                               B.s_pw, B.s_pin);
             check(hipGetLastError(), "k_pw_shift");
             pw2proxy(E, i0, nb);
         }
         auto &L = pl.p2c_levels[l];
-        launch_tensor3<Real, P, false>(pl.proxy_down.template as<Real>(), pl.proxy_down.template as<Real>(), L,
-                                       pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div);
+        launch_tensor3<Real, P, false>(B.down, B.down, L, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, nv,
+                                       B.s_down, B.s_down);
     }
     if (pl.n_leaf_eval > 0) {
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
-            hipLaunchKernelGGL((k_eval_targets_mfma<P>), dim3(pl.n_leaf_eval), dim3(64 * EvMfmaCfg<P>::NW), 0, s,
-                               pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
-                               pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(),
-                               pl.trg.template as<P4<Real>>(), pl.pot_sorted.template as<Real>());
+            hipLaunchKernelGGL((k_eval_targets_mfma<P>), dim3(pl.n_leaf_eval), dim3(64 * EvMfmaCfg<P>::NW), 0, s, B.down,
+                               pl.leaf_down.template as<int>(), pl.leaf_geom.template as<Real>(),
+                               pl.leaf_trg.template as<int>(), pl.trg.template as<P4<Real>>(), B.pot);
             check(hipGetLastError(), "k_eval_targets_mfma");
         } else {
-            hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval), dim3(EV_THREADS), 0, s,
-                               pl.proxy_down.template as<Real>(), pl.leaf_down.template as<int>(),
-                               pl.leaf_geom.template as<Real>(), pl.leaf_trg.template as<int>(),
-                               pl.trg.template as<P4<Real>>(), pl.pot_sorted.template as<Real>());
+            hipLaunchKernelGGL((k_eval_targets<Real, P>), dim3(pl.n_leaf_eval, 1, nv), dim3(EV_THREADS), 0, s, B.down,
+                               pl.leaf_down.template as<int>(), pl.leaf_geom.template as<Real>(),
+                               pl.leaf_trg.template as<int>(), pl.trg.template as<P4<Real>>(), B.pot, B.s_down, B.s_pot);
             check(hipGetLastError(), "k_eval_targets");
         }
     }
@@ -1119,16 +1163,17 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
     // (p < 28); at p >= 28 the chain is several times longer and its first kernels (proxy
     // formation on the matrix cores, proxy -> plane wave) fill the GPU, so the near field
     // starts after them and overlaps the rest.  (Measured at N=1e6: a later start, or the near
-    // field cut into two launches, is slower at 6 digits; the late start is faster at 9.)
+    // field cut into two launches, is slower at 6 digits; the late start is faster at 9; a
+    // start after the whole chain was slower at 6 and 9 digits, FP32 and FP64.)
     e_upward = near_late ? e_up : nullptr;
     if (!serial && !near_late) near_field(0, n_nf_items);
     mark(2, 0, s);
     if (stages & 1) {
         switch (p) {
-        case 9: launch_transforms<Real, 9, 13>(*this, s); break;
-        case 18: launch_transforms<Real, 18, 27>(*this, s); break;
-        case 28: launch_transforms<Real, 28, 39>(*this, s); break;
-        case 38: launch_transforms<Real, 38, 55>(*this, s); break;
+        case 9: launch_transforms<Real, 9, 13>(*this, s, own_chain(*this)); break;
+        case 18: launch_transforms<Real, 18, 27>(*this, s, own_chain(*this)); break;
+        case 28: launch_transforms<Real, 28, 39>(*this, s, own_chain(*this)); break;
+        case 38: launch_transforms<Real, 38, 55>(*this, s, own_chain(*this)); break;
         default: throw std::logic_error("unsupported proxy order");
         }
     }
@@ -1164,71 +1209,130 @@ void Plan<Real>::evaluate(const double *d_q, Real *d_pot, hipStream_t s, int sta
 }
 
 template <typename Real>
+bool Plan<Real>::many_ensure() {
+    if (many_vc_chain != 0) return many_vc_chain > 0;
+    constexpr int VC = kManyVec;
+    // per-vector bytes of the chain copies and of the exact far field's partial sums
+    const size_t chain_v = proxy_up.bytes + (pw_by_level ? 0 : c2p_scratch.bytes) + pw_out.bytes + pin.bytes +
+                           proxy_down.bytes;
+    const size_t sp_v = far_nufft ? 0 : sizeof(double2) * ff.sp_elems();
+    // at most 256 MiB of chain copies and 128 MiB of partial sums per plan: a large tree
+    // already fills the GPU with one vector per launch
+    many_vc_chain = (int)std::max<size_t>(1, std::min<size_t>(VC, (size_t(256) << 20) / std::max<size_t>(1, chain_v)));
+    many_vc_far = far_nufft ? 0 : (int)std::max<size_t>(1, std::min<size_t>(VC, (size_t(128) << 20) / std::max<size_t>(1, sp_v)));
+    const size_t pass = 2 * ((size_t)nsall * VC * sizeof(float) + (size_t)n * VC * (sizeof(float) + sizeof(double)));
+    const size_t total = pass + chain_v * many_vc_chain + (size_t)n * VC * sizeof(Real) +
+                         (far_nufft ? 0 : sp_v * many_vc_far + sizeof(double2) * ff.s_elems() * VC);
+    size_t free_b = 0, total_b = 0;
+    check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (pw_by_level || total > free_b / 4) {
+        many_vc_chain = many_vc_far = -1;
+        return false;
+    }
+    for (int b = 0; b < 2; ++b) {
+        many_qv[b].alloc((size_t)nsall * VC * sizeof(float));
+        many_near[b].alloc((size_t)n * VC * sizeof(float));
+        many_far[b].alloc((size_t)n * VC * sizeof(double));
+    }
+    many_up.alloc(proxy_up.bytes * many_vc_chain);
+    if (!pw_by_level) many_c2p.alloc(c2p_scratch.bytes * many_vc_chain);
+    many_pw.alloc(pw_out.bytes * many_vc_chain);
+    many_pin.alloc(pin.bytes * many_vc_chain);
+    many_down.alloc(proxy_down.bytes * many_vc_chain);
+    many_pot.alloc((size_t)n * VC * sizeof(Real));
+    if (!far_nufft) {
+        many_sp.alloc(sp_v * many_vc_far);
+        many_s.alloc(sizeof(double2) * ff.s_elems() * VC);
+        check(hipMemset(many_s.p, 0, many_s.bytes), "hipMemset");  // padding of the weighted sums stays zero
+    }
+    for (auto &e : many_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    return true;
+}
+
+template <typename Real>
 void Plan<Real>::evaluate_many(int n_vec, const double *d_Q, Real *d_P, hipStream_t s) {
     if (n_vec < 0) throw std::invalid_argument("evaluate_many: negative vector count");
     if (!s) s = stream;
     if (qp) throw std::logic_error("evaluate_many: quasi-periodic tree");
-    constexpr int VC = 8;
-    const bool batched = std::is_same<Real, float>::value && nf_rows && n_digits == 6 && n_nf_items > 0;
-    if (!batched) {
+    constexpr int VC = kManyVec;
+    const bool batched = std::is_same<Real, float>::value && nf_rows && n_digits == 6 && n_nf_items > 0 && p == 18;
+    if (!batched || n_vec < 2 || (size_t)n * VC > (size_t)INT_MAX || !many_ensure()) {
         for (int v = 0; v < n_vec; ++v) evaluate(d_Q + (size_t)v * n, d_P + (size_t)v * n, s);
         return;
     }
     if constexpr (std::is_same<Real, float>::value) {
-        if (!many_qv.p) {
-            many_qv.alloc(sizeof(float) * (size_t)nsall * VC);
-            many_near.alloc(sizeof(float) * (size_t)n * VC);
-            many_far.alloc(sizeof(double) * (size_t)n * VC);
-        }
         const size_t per_wave = (size_t)(nf_ne_max + 1) * 6 * sizeof(float4);
         int wpb = 4;
         while (wpb > 1 && (size_t)wpb * per_wave > 65536) wpb /= 2;
-        for (int v0 = 0; v0 < n_vec; v0 += VC) {
-            const int vc = std::min(VC, n_vec - v0);
-            // near field of the pass's vectors on s_near, far fields on s_far, beside each other
-            check(hipEventRecord(e_in, s), "hipEventRecord");
-            check(hipStreamWaitEvent(s_near, e_in, 0), "hipStreamWaitEvent");
-            check(hipStreamWaitEvent(s_far, e_in, 0), "hipStreamWaitEvent");
+        // the chain copies: vector c of a chain launch at c times these element strides
+        ChainBufs<float> cb;
+        cb.up = many_up.as<float>();
+        cb.c2p = many_c2p.as<float>();
+        cb.down = many_down.as<float>();
+        cb.pw = many_pw.as<Cx<float>>();
+        cb.pin = many_pin.as<Cx<float>>();
+        cb.qv_ld = VC;
+        cb.s_up = (long long)(proxy_up.bytes / sizeof(float));
+        cb.s_c2p = (long long)(c2p_scratch.bytes / sizeof(float));
+        cb.s_pw = (long long)(pw_out.bytes / sizeof(Cx<float>));
+        cb.s_pin = (long long)(pin.bytes / sizeof(Cx<float>));
+        cb.s_down = (long long)(proxy_down.bytes / sizeof(float));
+        cb.s_pot = n;
+        hipEvent_t *e_gat = many_ev, *e_nr = many_ev + 2, *e_fr = many_ev + 4, *e_fin = many_ev + 6;
+        check(hipEventRecord(e_in, s), "hipEventRecord");
+        check(hipStreamWaitEvent(s_near, e_in, 0), "hipStreamWaitEvent");
+        check(hipStreamWaitEvent(s_far, e_in, 0), "hipStreamWaitEvent");
+        e_upward = nullptr;
+        for (int v0 = 0, pass = 0; v0 < n_vec; v0 += VC, ++pass) {
+            const int vc = std::min(VC, n_vec - v0), b = pass & 1;
+            if (pass >= 2) {
+                // slot b's buffers are free once pass - 2 has finished on s
+                check(hipStreamWaitEvent(s_near, e_fin[b], 0), "hipStreamWaitEvent");
+                check(hipStreamWaitEvent(s_far, e_fin[b], 0), "hipStreamWaitEvent");
+            }
+            // near field of the pass's vectors on s_near
             hipLaunchKernelGGL(k_gather_many<VC>, dim3((nsall + 255) / 256), dim3(256), 0, s_near, d_Q, n_vec, v0, n,
-                               src_user.as<int>(), nsall, many_qv.as<float>());
+                               src_user.as<int>(), nsall, many_qv[b].as<float>());
             check(hipGetLastError(), "k_gather_many");
+            check(hipEventRecord(e_gat[b], s_near), "hipEventRecord");
             hipLaunchKernelGGL((k_near_rows_many<12, VC>), dim3((n_nf_items + wpb - 1) / wpb), dim3(64 * wpb),
-                               (size_t)wpb * per_wave, s_near, src.as<P4<float>>(), many_qv.as<float>(),
+                               (size_t)wpb * per_wave, s_near, src.as<P4<float>>(), many_qv[b].as<float>(),
                                trg.as<P4<float>>(), nf_items.as<int4>(), nf_ptr.as<int>(), nf_geom.as<P4<float>>(),
                                nf_rs_base.as<int>(), nf_rslot.as<int>(), nf_wave_base.as<long long>(),
-                               nf_row_steps.as<int>(), coeffs.as<float>(), many_near.as<float>(), n, vc, 0,
+                               nf_row_steps.as<int>(), coeffs.as<float>(), many_near[b].as<float>(), n, vc, 0,
                                n_nf_items, nf_ne_max);
             check(hipGetLastError(), "k_near_rows_many");
-            check(hipEventRecord(e_near, s_near), "hipEventRecord");
-            for (int c = 0; c < vc; ++c) {
-                const double *q = d_Q + (size_t)(v0 + c) * n;
-                double *fo = many_far.as<double>() + (size_t)c * n;
-                if (far_nufft) nf.run(q, fo, s_far);
-                else ff.run(q, fo, s_far);
+            check(hipEventRecord(e_nr[b], s_near), "hipEventRecord");
+            // far fields on s_far
+            const double *Q0 = d_Q + (size_t)v0 * n;
+            double *far0 = many_far[b].as<double>();
+            if (far_nufft) {
+                for (int c = 0; c < vc; ++c) nf.run(Q0 + (size_t)c * n, far0 + (size_t)c * n, s_far);
+            } else {
+                ff.run_many(Q0, n, vc, far0, n, many_sp.as<double2>(), many_vc_far, many_s.as<double2>(), s_far);
             }
-            check(hipEventRecord(e_far, s_far), "hipEventRecord");
-            // the chain writes the sources' charges the near field reads beside them: it starts
-            // after the near-field pass
-            check(hipStreamWaitEvent(s, e_near, 0), "hipStreamWaitEvent");
-            e_upward = nullptr;
-            for (int c = 0; c < vc; ++c) {
-                const double *q = d_Q + (size_t)(v0 + c) * n;
-                const int m = std::max(nsall, n);
-                hipLaunchKernelGGL(k_gather_charges<Real>, dim3((m + 255) / 256), dim3(256), 0, s, q, src_user.as<int>(),
-                                   src.as<P4<Real>>(), nsall, pot_sorted.as<Real>(), pot_near.as<Real>(), n,
-                                   (const double2 *)nullptr, 0);
-                check(hipGetLastError(), "k_gather_charges");
-                switch (p) {
-                case 18: launch_transforms<Real, 18, 27>(*this, s); break;
-                default: throw std::logic_error("evaluate_many: unexpected proxy order");
-                }
-                if (c == 0) check(hipStreamWaitEvent(s, e_far, 0), "hipStreamWaitEvent");
-                hipLaunchKernelGGL(k_finalize<Real>, dim3((n + 255) / 256), dim3(256), 0, s, pot_sorted.as<Real>(),
-                                   many_near.as<float>() + (size_t)c * n, trg_w0.as<double>(), trg_user.as<int>(), q,
-                                   many_far.as<double>() + (size_t)c * n, far_idx.as<int>(),
-                                   d_P + (size_t)(v0 + c) * n, n, 1, 1, 1);
-                check(hipGetLastError(), "k_finalize");
+            check(hipEventRecord(e_fr[b], s_far), "hipEventRecord");
+            // the chain on s, many_vc_chain vectors per launch, charges from the gathered copy
+            check(hipStreamWaitEvent(s, e_gat[b], 0), "hipStreamWaitEvent");
+            hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)(((size_t)n * vc + 255) / 256)), dim3(256), 0, s,
+                               many_pot.as<float>(), 0.0f, n * vc);
+            check(hipGetLastError(), "k_fill");
+            for (int c0 = 0; c0 < vc; c0 += many_vc_chain) {
+                ChainBufs<float> B = cb;
+                B.nv = std::min(many_vc_chain, vc - c0);
+                B.qv = many_qv[b].as<float>() + c0;
+                B.pot = many_pot.as<float>() + (size_t)c0 * n;
+                launch_transforms<float, 18, 27>(*this, s, B);
             }
+            // potentials of the pass once its near and far fields are in
+            check(hipStreamWaitEvent(s, e_nr[b], 0), "hipStreamWaitEvent");
+            check(hipStreamWaitEvent(s, e_fr[b], 0), "hipStreamWaitEvent");
+            hipLaunchKernelGGL(k_finalize<float>, dim3((n + 255) / 256, 1, vc), dim3(256), 0, s, many_pot.as<float>(),
+                               many_near[b].as<float>(), trg_w0.as<double>(), trg_user.as<int>(), Q0, far0,
+                               far_idx.as<int>(), d_P + (size_t)v0 * n, n, 1, 1, 1, (long long)n, (long long)n,
+                               (long long)n, (long long)n, (long long)n);
+            check(hipGetLastError(), "k_finalize");
+            check(hipEventRecord(e_fin[b], s), "hipEventRecord");
         }
     }
 }
@@ -1481,6 +1585,7 @@ void FourierSum::setup(const std::vector<double> &theta_host, int n_points, int 
     pidx_d = put(pidx, pix);
     blocks = std::max(1, std::min(std::min(2048, max_blocks), (n + FF_TP - 1) / FF_TP));
     const size_t sp_n = (size_t)blocks * std::max(1, n_modes), s_n = (size_t)std::max(1, npad);
+    s_len = s_n;
     if (arena) {
         Sp_d = arena->take_n<double2>(sp_n);
         S_d = arena->take_n<double2>(s_n);
@@ -1513,6 +1618,32 @@ void FourierSum::run_at(const double *d_q, const double *d_theta_eval, int n_eva
     check(hipGetLastError(), "k_ff_reduce");
     hipLaunchKernelGGL(k_ff_eval, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_theta_eval, n_eval, lines_d,
                        line_base_d, n_lines, S_d, d_out);
+    check(hipGetLastError(), "k_ff_eval");
+}
+
+void FourierSum::run_many(const double *d_Q, long long ldq, int nv, double *d_out, long long ldo, double2 *Sp_many,
+                          int vc, double2 *S_many, hipStream_t s) const {
+    if (nv <= 0 || n <= 0) return;
+    if (n_modes == 0) {
+        for (int v = 0; v < nv; ++v) run(d_Q + v * ldq, d_out + v * ldo, s);
+        return;
+    }
+    if (vc < 1 || vc > 65535 || nv > 65535) throw std::invalid_argument("FourierSum::run_many: vector count");
+    const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1);
+    const long long spv = (long long)sp_elems(), sv = (long long)s_len;
+    // the same launches as run(), with the vectors along z: per-vector sums in the same order
+    for (int v0 = 0; v0 < nv; v0 += vc) {
+        const int c = std::min(vc, nv - v0);
+        hipLaunchKernelGGL(k_ff_structure, dim3(blocks, (n_chunks + 255) / 256, c), dim3(256), lds, s, theta_d,
+                           d_Q + v0 * ldq, n, ms, mt, mu, chunk_ijk_d, chunk_m0_d, chunk_len_d, n_chunks, n_modes,
+                           Sp_many, (long long)ldq, spv);
+        check(hipGetLastError(), "k_ff_structure");
+        hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32, 1, c), dim3(256), 0, s, Sp_many, blocks, w_d, pidx_d,
+                           n_modes, S_many + v0 * sv, spv, sv);
+        check(hipGetLastError(), "k_ff_reduce");
+    }
+    hipLaunchKernelGGL(k_ff_eval, dim3((n + 255) / 256, 1, nv), dim3(256), 0, s, theta_d, n, lines_d, line_base_d,
+                       n_lines, S_many, d_out, sv, (long long)ldo);
     check(hipGetLastError(), "k_ff_eval");
 }
 
@@ -1672,9 +1803,9 @@ void nufft_dispatch(const NufftFarField &F, double *d_out, hipStream_t s, bool s
 
 template <bool RIN, bool ROUT>
 void dft_axis(const void *in, void *out, const DevBuf &mat, int A, int L, int K, int B, hipStream_t s) {
-    const long total = (long)A * K * B;
-    hipLaunchKernelGGL((k_dft_axis<RIN, ROUT>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out,
-                       mat.as<double2>(), A, L, K, B);
+    const long lines = (long)A * B;
+    hipLaunchKernelGGL((k_dft_axis<RIN, ROUT>), dim3((unsigned)((lines + 255) / 256), (K + DFT_KC - 1) / DFT_KC),
+                       dim3(256), 0, s, in, out, mat.as<double2>(), A, L, K, B);
     check(hipGetLastError(), "k_dft_axis");
 }
 

@@ -89,6 +89,14 @@ struct FourierSum {
     // S(m) from the setup points, then out[t] at n_eval other points (theta_eval[3t + d],
     // device, same convention); enqueue only
     void run_at(const double *d_q, const double *d_theta_eval, int n_eval, double *d_out, hipStream_t s) const;
+    // run() for nv vectors (charges d_Q + v ldq, output d_out + v ldo), each equal to run() on
+    // it: partial sums of vc vectors per launch in Sp_many (vc * sp_elems()), the weighted
+    // sums of all nv in S_many (nv * s_elems(), padding zero), one evaluation launch
+    void run_many(const double *d_Q, long long ldq, int nv, double *d_out, long long ldo, double2 *Sp_many, int vc,
+                  double2 *S_many, hipStream_t s) const;
+    size_t sp_elems() const { return (size_t)blocks * (size_t)std::max(1, n_modes); }
+    size_t s_elems() const { return s_len; }
+    size_t s_len = 1;
 };
 
 // The same Fourier sum by a non-uniform FFT (spreading with an exponential-of-semicircle
@@ -209,12 +217,25 @@ struct Plan {
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
 
     // evaluate() of n_vec charge vectors (d_Q, d_P: vector-major, n per vector) on `s`.  FP32
-    // plans with row step lists form the near field of 8 vectors per pass (each pair's residual
-    // once), beside the far fields, then run the proxy / plane-wave chain per vector; other
-    // plans call evaluate() per vector.  Each vector's potentials equal evaluate()'s on it.
-    // The per-vector buffers grow on first use to 8 vectors' worth.
+    // plans with row step lists take the vectors in passes of 8: the near field of the pass
+    // (each pair's residual formed once for the 8 vectors) on s_near, the far fields on s_far
+    // (the exact Fourier sum of several vectors per launch) and the proxy / plane-wave chain of
+    // several vectors per launch on s, the three beside each other and beside the next pass;
+    // other plans call evaluate() per vector.  Each vector's potentials equal evaluate()'s on
+    // it bit for bit.  The per-vector buffers are allocated on first use (many_ensure).
     void evaluate_many(int n_vec, const double *d_Q, Real *d_P, hipStream_t s);
-    DevBuf many_qv, many_near, many_far;
+    static constexpr int kManyVec = 8;  // vectors per near-field pass
+    // vectors per chain launch and per exact far-field launch (sized to a memory budget);
+    // 0 until the buffers exist, -1 when they do not fit (evaluate_many then calls evaluate())
+    int many_vc_chain = 0, many_vc_far = 0;
+    bool many_ensure();
+    // two pass slots (b = pass & 1): charges [nsall][8] (FP32), near [8][n], far [8][n]
+    DevBuf many_qv[2], many_near[2], many_far[2];
+    // chain copies for many_vc_chain vectors, potentials of a pass, exact far-field partial
+    // sums for many_vc_far vectors and weighted sums for 8
+    DevBuf many_up, many_c2p, many_pw, many_pin, many_down, many_pot, many_sp, many_s;
+    // per slot: charges gathered, near field done, far field done, pass finished
+    hipEvent_t many_ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
     // Launch shapes: set at construction from the per-device cache when a plan of the same
     // kind (order, precision, size buckets) was tuned before in this process (or in the file

@@ -4,6 +4,7 @@ stage on its own (no other stream competing for the GPU):
     python -m periodicdmk_amd.stages --paths mixed6,fp64_9 --stages 1 --calls 3
     python -m periodicdmk_amd.stages --paths fp64_6 --stages 15 --calls 4 --bloch 2,-1,0.5
     python -m periodicdmk_amd.stages --workload clustered --n 4000000 --paths mixed6,fp64_9,fp64_12 --stages 15
+    python -m periodicdmk_amd.stages --workload clustered --n 4000000 --paths fp64_9 --stages 15 --autotune
 
 Stage bits: 1 proxy / plane-wave chain, 2 near field, 4 self correction, 8 far field.
 --stages 15 calls evaluate() itself (device pointers; far field, near field and chain side
@@ -34,6 +35,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--bloch", default="", help="bx,by,bz: quasi-periodic trees")
     ap.add_argument("--workload", default="uniform", choices=("uniform", "clustered"))
+    ap.add_argument("--autotune", action="store_true", help="run the launch-shape autotuner on each tree first")
     args = ap.parse_args()
     if P.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
@@ -51,6 +53,10 @@ def main():
         bloch = [float(v) for v in args.bloch.split(",")] if args.bloch else None
         s = P.PDMKSolver(cell, pos, n_digits=nd, n_per_leaf=300, nz=4, precision=prec, bloch=bloch)
         out = P.DeviceArray(2 * args.n if bloch else args.n, s.dtype)
+        if args.autotune:
+            r = s.autotune(2)
+            print(f"[stages] {name}: autotune {r['ms_default']:.3f} -> {r['ms_tuned']:.3f} ms, knobs {r['knobs']}",
+                  flush=True)
         times = []
         for _ in range(args.calls):
             t0 = time.perf_counter()

@@ -1,8 +1,10 @@
 """evaluate() of many charge vectors per call (pdmk_gpu_evaluate_many) against one call per
-vector: FP32 plans at 6 digits form the near field of 8 vectors per pass with each residual
-computed once, the rest per vector; every vector's potentials equal pdmk_gpu_evaluate_float's
-bit for bit (same sums in the same order).  FP64 plans call evaluate() per vector.  Vector
-counts below, at and above a pass of 8, and 0."""
+vector: FP32 plans at 6 digits take the vectors in passes of 8 (the near field of a pass with
+each residual computed once, the chain and the exact far field of several vectors per launch,
+passes overlapping); every vector's potentials equal pdmk_gpu_evaluate_float's bit for bit
+(same sums in the same order).  FP64 plans call evaluate() per vector.  Vector counts below,
+at and above a pass of 8, three passes (the pass buffers reused), and 0; a tree large enough
+for the NUFFT far field and for chain launches of fewer than 8 vectors."""
 import numpy as np
 import pytest
 
@@ -23,10 +25,21 @@ def test_many_equals_single(pdmk, cuda, name, precision):
     cell, pos, npl = CASES[name]()
     n = pos.shape[0]
     s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, nz=4, precision=precision)
-    for nv in (1, 8, 11):
+    for nv in (1, 2, 8, 11, 20):
         Q = np.stack([W.neutral_uniform_charges(n, 500 + 7 * nv + v) for v in range(nv)])
         got = s.evaluate_many(Q)
         for v in range(nv):
             np.testing.assert_array_equal(got[v], s.evaluate(Q[v]), err_msg=f"{name} {precision} vector {v} of {nv}")
     assert s.evaluate_many(np.zeros((0, n))).shape == (0, n)
+    s.close()
+
+
+def test_many_large_tree(pdmk, cuda):
+    n = 400_000
+    cell = W.cubic_cell()
+    s = pdmk.PDMKSolver(cell, W.uniform_positions(n, 24, cell), n_digits=6, n_per_leaf=300, nz=4, precision="float")
+    Q = np.stack([W.neutral_uniform_charges(n, 900 + v) for v in range(9)])
+    got = s.evaluate_many(Q)
+    for v in range(9):
+        np.testing.assert_array_equal(got[v], s.evaluate(Q[v]), err_msg=f"vector {v} of 9")
     s.close()
