@@ -283,7 +283,8 @@ def _reference(prob, m=64):
     return tg, ewald_targets(prob["cell"], prob["pos"], prob["q"], tg, s=5.5)
 
 
-def generate(n_problems: int, seed: int, out_path: str, configs_per_problem: int = 5, budget_s: float = 1e9):
+def generate(n_problems: int, seed: int, out_path: str, configs_per_problem: int = 5, budget_s: float = 1e9,
+             n_min: float = 1e4):
     rng = np.random.default_rng(seed)
     t_start = time.time()
     with open(out_path, "a") as f:
@@ -291,7 +292,7 @@ def generate(n_problems: int, seed: int, out_path: str, configs_per_problem: int
             if time.time() - t_start > budget_s:
                 print(f"[costmodel] time budget reached after {i} problems", flush=True)
                 break
-            prob = random_problem(rng)
+            prob = random_problem(rng, n_min)
             f4 = features(prob["cell"], prob["pos"], prob["n_digits"])
             tg, ew = _reference(prob)
             for npl, nz, fl in _configs_for(rng, prob["n_digits"], configs_per_problem):
@@ -360,6 +361,7 @@ def main(argv=None):
     g.add_argument("--seed", type=int, default=1)
     g.add_argument("--configs", type=int, default=5)
     g.add_argument("--budget-s", type=float, default=1e9)
+    g.add_argument("--n-min", type=float, default=1e4, help="smallest N of the random problems")
     g.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
@@ -371,9 +373,10 @@ def main(argv=None):
     h.add_argument("--seed", type=int, default=1000003)
     h.add_argument("--out", required=True)
     h.add_argument("--budget-s", type=float, default=1e9)
+    h.add_argument("--n-min", type=float, default=1e4, help="smallest N of the random problems")
     a = ap.parse_args(argv)
     if a.cmd == "generate":
-        generate(a.problems, a.seed, a.out, a.configs, a.budget_s)
+        generate(a.problems, a.seed, a.out, a.configs, a.budget_s, a.n_min)
     elif a.cmd == "train":
         rows = read_records(a.data)
         blob, hist = fit(rows, a.steps, lr=a.lr)
@@ -381,7 +384,7 @@ def main(argv=None):
                                  "data": [os.path.basename(d) for d in a.data]})
         print(f"[costmodel] trained on {len(rows)} solves: loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
     else:
-        heldout(a.problems, a.seed, a.out, budget_s=a.budget_s)
+        heldout(a.problems, a.seed, a.out, budget_s=a.budget_s, n_min=a.n_min)
 
 
 if __name__ == "__main__":

@@ -51,9 +51,17 @@ __device__ __forceinline__ int vgpr_zero() {
     return z;
 }
 
+// 1 / sqrt(x) for the near-field distances: every caller passes 1e-30 < x < 1e30 (a positive
+// normal number).  FP64: the hardware estimate y = v_rsq_f64(x) and one third-order
+// correction y (1 + e/2 + 3e^2/8), e = 1 - x y^2 -- the library rsqrt's own arithmetic, bit
+// for bit on this range, without its branches for zero, infinity and NaN inputs.
 template <class R> __device__ __forceinline__ R rsqrt_full(R x);
 template <> __device__ __forceinline__ float rsqrt_full<float>(float x) { return __frsqrt_rn(x); }
-template <> __device__ __forceinline__ double rsqrt_full<double>(double x) { return rsqrt(x); }
+template <> __device__ __forceinline__ double rsqrt_full<double>(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(y * -x, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
 
 // ------------------------------------------------------------------ O(N) stages
 // Charges of the sorted sources.  Quasi-periodic passes multiply each by the real (comp 0)
@@ -1250,6 +1258,21 @@ __device__ __forceinline__ float quad_bcast(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), S * 0x55, 0xF, 0xF, true));
 }
 
+// ea += q(lane 4 g + S) * u in one DPP-broadcast FMA (the same fused multiply-add as
+// fmaf(quad_bcast<S>(q), u, ea)), led by s_nop 1: the two wait states a DPP read needs after a
+// VALU write of its source (the compiler does not check hazards inside inline asm).
+#define PDMK_FMAC_QUAD1(SEL) \
+    asm("s_nop 1\n"          \
+        "v_fmac_f32_dpp %0, %1, %2 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1" : "+v"(ea) : "v"(q), "v"(u))
+template <int S>
+__device__ __forceinline__ void fmac_quad1(float &ea, float q, float u) {
+    if constexpr (S == 0) PDMK_FMAC_QUAD1("0,0,0,0");
+    if constexpr (S == 1) PDMK_FMAC_QUAD1("1,1,1,1");
+    if constexpr (S == 2) PDMK_FMAC_QUAD1("2,2,2,2");
+    if constexpr (S == 3) PDMK_FMAC_QUAD1("3,3,3,3");
+}
+#undef PDMK_FMAC_QUAD1
+
 // PASS 0: steps of row g in the k-th entry of item i -> cnt[(ie_base[i] + k) * 16 + g], and the
 // in-support pairs added to *pairs.  PASS 1: the source indices into the row blocks from
 // steps + wave_base[i] (the item's K, as built on the host, in items[i].w).
@@ -1353,6 +1376,11 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
     float c[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    // the second-highest coefficient in a vector register: the first Horner step then takes
+    // the highest from a scalar register and needs no copy per pair (one scalar operand per
+    // FMA)
+    float c_next;
+    asm("v_mov_b32 %0, %1" : "=v"(c_next) : "s"(c[NC - 2]));
     float tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
     int end = 0, ns = 0;
     double acc = 0;
@@ -1398,19 +1426,17 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
             if (r2 < d2max && r2 > 1e-30f) {
                 const float rinv = rsqrt_full<float>(r2);
                 const float x = fmaf(r2, rinv, cen) * rsc;
-                float p = c[NC - 1];
+                float p = fmaf(c[NC - 1], x, c_next);
 #pragma unroll
-                for (int i = NC - 2; i >= 0; --i) p = fmaf(p, x, c[i]);
+                for (int i = NC - 3; i >= 0; --i) p = fmaf(p, x, c[i]);
                 u = p * rinv;
             }
-            float pq;
             switch (s) {
-            case 0: pq = quad_bcast<0>(pc.w); break;
-            case 1: pq = quad_bcast<1>(pc.w); break;
-            case 2: pq = quad_bcast<2>(pc.w); break;
-            default: pq = quad_bcast<3>(pc.w); break;
+            case 0: fmac_quad1<0>(ea, pc.w, u); break;
+            case 1: fmac_quad1<1>(ea, pc.w, u); break;
+            case 2: fmac_quad1<2>(ea, pc.w, u); break;
+            default: fmac_quad1<3>(ea, pc.w, u); break;
             }
-            ea = fmaf(pq, u, ea);
         }
         pc = pn;
     }
@@ -1493,6 +1519,8 @@ __global__ __launch_bounds__(1024) void k_near_rows_many(
     float c[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
+    float c_next;  // as in k_near_rows
+    asm("v_mov_b32 %0, %1" : "=v"(c_next) : "s"(c[NC - 2]));
     float tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
     int end = 0, ns = 0;
     double acc[VC];
@@ -1551,9 +1579,9 @@ __global__ __launch_bounds__(1024) void k_near_rows_many(
             if (r2 < d2max && r2 > 1e-30f) {
                 const float rinv = rsqrt_full<float>(r2);
                 const float x = fmaf(r2, rinv, cen) * rsc;
-                float p = c[NC - 1];
+                float p = fmaf(c[NC - 1], x, c_next);
 #pragma unroll
-                for (int i = NC - 2; i >= 0; --i) p = fmaf(p, x, c[i]);
+                for (int i = NC - 3; i >= 0; --i) p = fmaf(p, x, c[i]);
                 u = p * rinv;
             }
             switch (s) {
