@@ -374,6 +374,7 @@ def main(argv=None):
     h.add_argument("--out", required=True)
     h.add_argument("--budget-s", type=float, default=1e9)
     h.add_argument("--n-min", type=float, default=1e4, help="smallest N of the random problems")
+    h.add_argument("--model", default=MODEL_PATH, help="model file (default: the shipped one)")
     a = ap.parse_args(argv)
     if a.cmd == "generate":
         generate(a.problems, a.seed, a.out, a.configs, a.budget_s, a.n_min)
@@ -384,7 +385,10 @@ def main(argv=None):
                                  "data": [os.path.basename(d) for d in a.data]})
         print(f"[costmodel] trained on {len(rows)} solves: loss {hist[0]:.4f} -> {hist[-1]:.4f}", flush=True)
     else:
-        heldout(a.problems, a.seed, a.out, budget_s=a.budget_s, n_min=a.n_min)
+        blob = load_model(a.model)
+        if blob is None:
+            sys.exit(f"no model at {a.model}")
+        heldout(a.problems, a.seed, a.out, blob=blob, budget_s=a.budget_s, n_min=a.n_min)
 
 
 if __name__ == "__main__":
