@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 
 namespace pdmk::gpu {
@@ -236,34 +237,47 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             return o < N ? o : g.img_src[o - N];
         };
         const double bs0 = t.boxsize[0], tol = 1e-12 * bs0;
-        std::vector<int> ub, up;
-        std::function<bool(int, int)> same = [&](int b, int q) {
+        // (user index, position relative to the box centre) of a box's points, sorted by index
+        using Pt = std::pair<int, std::array<double, 3>>;
+        auto points_of = [&](int box, std::vector<Pt> &v) {
+            v.clear();
+            for (int i = 0; i < t.src_count[box]; ++i) {
+                const int j = t.src_begin[box] + i;
+                std::array<double, 3> x;
+                for (int d = 0; d < 3; ++d) x[d] = t.src_sorted[3 * (size_t)j + d] - t.center[3 * box + d];
+                v.emplace_back(user_of(j), x);
+            }
+            std::sort(v.begin(), v.end(), [](const Pt &u, const Pt &w) { return u.first < w.first; });
+        };
+        struct Scratch { std::vector<int> ub, up; std::vector<Pt> pb, pq; };
+        std::function<bool(int, int, Scratch &)> same = [&](int b, int q, Scratch &sc) {
             if (t.level[b] != t.level[q] || t.leaf[b] != t.leaf[q] || t.pwexp[b] != t.pwexp[q] ||
                 t.src_count[b] != t.src_count[q] || t.trg_count[b] != 0)
                 return false;
             if (!t.leaf[b]) {
                 for (int k = 0; k < 8; ++k)
-                    if (!same(t.first_child[b] + k, t.first_child[q] + k)) return false;
+                    if (!same(t.first_child[b] + k, t.first_child[q] + k, sc)) return false;
                 return true;
             }
-            ub.clear();
-            up.clear();
-            for (int i = 0; i < t.src_count[b]; ++i) ub.push_back(user_of(t.src_begin[b] + i));
-            for (int i = 0; i < t.src_count[q]; ++i) up.push_back(user_of(t.src_begin[q] + i));
-            std::sort(ub.begin(), ub.end());
-            std::sort(up.begin(), up.end());
-            return ub == up;
+            sc.ub.clear();
+            sc.up.clear();
+            for (int i = 0; i < t.src_count[b]; ++i) sc.ub.push_back(user_of(t.src_begin[b] + i));
+            for (int i = 0; i < t.src_count[q]; ++i) sc.up.push_back(user_of(t.src_begin[q] + i));
+            std::sort(sc.ub.begin(), sc.ub.end());
+            std::sort(sc.up.begin(), sc.up.end());
+            return sc.ub == sc.up;
         };
-        std::function<void(int, int)> mark = [&](int b, int q) {
+        std::function<int(int, int)> mark = [&](int b, int q) {
             fold[b] = q;
-            n_folded++;
+            int m = 1;
             if (!t.leaf[b])
-                for (int k = 0; k < 8; ++k) mark(t.first_child[b] + k, t.first_child[q] + k);
+                for (int k = 0; k < 8; ++k) m += mark(t.first_child[b] + k, t.first_child[q] + k);
+            return m;
         };
-        std::unordered_map<int, std::array<double, 3>> local;  // user index -> position in its level-0 box
-        for (int b : (enabled ? t.by_level[0] : std::vector<int>())) {
+        // b: a level-0 box of images; its counterpart q, or -1
+        auto counterpart = [&](int b, Scratch &sc) {
             const int cnt = t.src_count[b], beg = t.src_begin[b];
-            if (cnt == 0 || t.src_order[beg] < N) continue;
+            if (cnt == 0 || t.src_order[beg] < N) return -1;
             // the translation of the first point; every point of b must be an image under it
             const int u0 = user_of(beg);
             double R[3];
@@ -271,26 +285,36 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
             int qc[3];
             for (int d = 0; d < 3; ++d) qc[d] = (int)std::lround((t.center[3 * b + d] - R[d] - t.xelo[d]) / bs0 - 0.5);
             const int q = t.find(0, qc[0], qc[1], qc[2]);
-            if (q < 0 || t.src_count[q] != cnt) continue;
-            bool ok = true;
-            for (int d = 0; d < 3; ++d) ok = ok && std::fabs(t.center[3 * q + d] - (t.center[3 * b + d] - R[d])) <= tol;
-            local.clear();
-            for (int i = 0; ok && i < cnt; ++i) {
-                const int j = t.src_begin[q] + i;
-                ok = t.src_order[j] < N;
-                std::array<double, 3> x;
-                for (int d = 0; d < 3; ++d) x[d] = t.src_sorted[3 * (size_t)j + d] - t.center[3 * q + d];
-                ok = ok && local.emplace(t.src_order[j], x).second;
+            if (q < 0 || t.src_count[q] != cnt) return -1;
+            for (int d = 0; d < 3; ++d)
+                if (!(std::fabs(t.center[3 * q + d] - (t.center[3 * b + d] - R[d])) <= tol)) return -1;
+            // q holds originals only, each once; b holds images only, of the same points at the
+            // same positions relative to the box centres
+            for (int i = 0; i < cnt; ++i)
+                if (t.src_order[t.src_begin[q] + i] >= N || t.src_order[beg + i] < N) return -1;
+            points_of(q, sc.pq);
+            points_of(b, sc.pb);
+            for (int i = 0; i < cnt; ++i) {
+                if (i > 0 && sc.pq[i].first == sc.pq[i - 1].first) return -1;
+                if (sc.pb[i].first != sc.pq[i].first) return -1;
+                for (int d = 0; d < 3; ++d)
+                    if (!(std::fabs(sc.pb[i].second[d] - sc.pq[i].second[d]) <= tol)) return -1;
             }
-            for (int i = 0; ok && i < cnt; ++i) {
-                const int j = beg + i;
-                const auto it = local.find(user_of(j));
-                ok = t.src_order[j] >= N && it != local.end();
-                for (int d = 0; ok && d < 3; ++d)
-                    ok = std::fabs(t.src_sorted[3 * (size_t)j + d] - t.center[3 * b + d] - it->second[d]) <= tol;
-            }
-            if (ok && same(b, q)) mark(b, q);
-        }
+            return same(b, q, sc) ? q : -1;
+        };
+        // level-0 image boxes are independent (disjoint subtrees): split over host threads
+        const std::vector<int> roots = enabled ? t.by_level[0] : std::vector<int>();
+        const int nthr = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<int> found(roots.size(), -1);
+        std::vector<std::thread> pool;
+        for (int w = 0; w < nthr; ++w)
+            pool.emplace_back([&, w] {
+                Scratch sc;
+                for (size_t i = w; i < roots.size(); i += nthr) found[i] = counterpart(roots[i], sc);
+            });
+        for (auto &th : pool) th.join();
+        for (size_t i = 0; i < roots.size(); ++i)
+            if (found[i] >= 0) n_folded += mark(roots[i], found[i]);
     }
 
     // ---------------- proxy slots
