@@ -53,6 +53,23 @@ namespace {
 
 constexpr size_t kPinBytes = size_t(512) << 20;  // incoming plane-wave scratch cap
 
+// f(i0, i1) over contiguous chunks of [0, n) on up to 16 host threads (construction work
+// whose chunks write disjoint data)
+template <class F> void host_parallel(int n, F f) {
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int chunk = (n + nt - 1) / nt;
+    for (int w = 0; w < nt; ++w) {
+        const int i0 = w * chunk, i1 = std::min(n, i0 + chunk);
+        if (i0 < i1) pool.emplace_back([&f, i0, i1] { f(i0, i1); });
+    }
+    for (auto &th : pool) th.join();
+}
+
 // Tuned knobs per plan kind, shared by the plans of the process; optionally mirrored in the
 // text file named by PDMK_TUNE_CACHE (one "key near_wpb near_late p2pw_bk pw2p_bk t3_div"
 // line per kind, later lines win).
@@ -141,40 +158,56 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     // in the leaf, so that runs of consecutive points are spatially compact (near-field
     // work items and source chunks); box ranges are unchanged by this.
     std::vector<int> src_leaf(nsall, -1), trg_leaf(n, -1), src_ord(nsall), trg_ord(n);
-    auto morton_order = [&](const std::vector<double> &xyz, int beg, int cnt, int b, int *ord) {
-        std::vector<std::pair<unsigned long long, int>> key((size_t)cnt);
+    // 10 bits of q spread to every third bit
+    auto spread3 = [](unsigned long long q) {
+        q &= 0x3ffull;
+        q = (q | (q << 16)) & 0x030000ffull;
+        q = (q | (q << 8)) & 0x0300f00full;
+        q = (q | (q << 4)) & 0x030c30c3ull;
+        q = (q | (q << 2)) & 0x09249249ull;
+        return q;
+    };
+    auto morton_order = [&](const std::vector<double> &xyz, int beg, int cnt, int b, int *ord,
+                            std::vector<std::pair<unsigned long long, int>> &key) {
+        key.resize((size_t)cnt);
         const double inv = 1.0 / t.boxsize[t.level[b]];
         for (int i = 0; i < cnt; ++i) {
             unsigned long long code = 0;
             for (int d = 0; d < 3; ++d) {
                 const double u = (xyz[3 * (size_t)(beg + i) + d] - t.center[3 * (size_t)b + d]) * inv + 0.5;
                 const unsigned long long q = (unsigned long long)std::min(1023.0, std::max(0.0, std::floor(u * 1024.0)));
-                for (int bit = 0; bit < 10; ++bit) code |= ((q >> bit) & 1ull) << (3 * bit + d);
+                code |= spread3(q) << d;
             }
             key[i] = {code, beg + i};
         }
-        std::stable_sort(key.begin(), key.end(),
-                         [](const auto &a, const auto &c) { return a.first < c.first; });
+        // (the index breaks ties: the order of a stable sort by code)
+        std::sort(key.begin(), key.end());
         for (int i = 0; i < cnt; ++i) ord[beg + i] = key[i].second;
     };
-    for (int b = 0; b < nboxes; ++b) {
-        if (!t.leaf[b]) continue;
-        for (int i = 0; i < t.src_count[b]; ++i) src_leaf[t.src_begin[b] + i] = b;
-        for (int i = 0; i < t.trg_count[b]; ++i) trg_leaf[t.trg_begin[b] + i] = b;
-        morton_order(t.src_sorted, t.src_begin[b], t.src_count[b], b, src_ord.data());
-        morton_order(t.trg_sorted, t.trg_begin[b], t.trg_count[b], b, trg_ord.data());
-    }
+    // leaves own disjoint point ranges: split over the host threads
+    host_parallel(nboxes, [&](int b0, int b1) {
+        std::vector<std::pair<unsigned long long, int>> key;
+        for (int b = b0; b < b1; ++b) {
+            if (!t.leaf[b]) continue;
+            for (int i = 0; i < t.src_count[b]; ++i) src_leaf[t.src_begin[b] + i] = b;
+            for (int i = 0; i < t.trg_count[b]; ++i) trg_leaf[t.trg_begin[b] + i] = b;
+            morton_order(t.src_sorted, t.src_begin[b], t.src_count[b], b, src_ord.data(), key);
+            morton_order(t.trg_sorted, t.trg_begin[b], t.trg_count[b], b, trg_ord.data(), key);
+        }
+    });
     auto local = [&](const std::vector<double> &xyz, size_t i, int b, int d) {
         return (Real)(xyz[3 * i + d] - t.center[3 * (size_t)b + d]);
     };
     std::vector<P4<Real>> srcv(nsall);
     std::vector<int> su(nsall);
-    for (int i = 0; i < nsall; ++i) {
-        const int b = src_leaf[i], j = src_ord[i];
-        srcv[i] = {local(t.src_sorted, j, b, 0), local(t.src_sorted, j, b, 1), local(t.src_sorted, j, b, 2), Real(0)};
-        const int o = t.src_order[j];
-        su[i] = o < N ? o : g.img_src[o - N];
-    }
+    host_parallel(nsall, [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const int b = src_leaf[i], j = src_ord[i];
+            srcv[i] = {local(t.src_sorted, j, b, 0), local(t.src_sorted, j, b, 1), local(t.src_sorted, j, b, 2), Real(0)};
+            const int o = t.src_order[j];
+            su[i] = o < N ? o : g.img_src[o - N];
+        }
+    });
     if (bloch_in) {
         // quasi-periodic image phases e^{i b.R}, R = image position - original position in the
         // lattice frame (the translation of generate_images, cpp/include/dmk/pbc.hpp:221-229)
@@ -194,11 +227,14 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
     std::vector<P4<Real>> trgv(n);
     std::vector<int> tu(n);
     std::vector<double> tw0(n, 0.0);
-    for (int i = 0; i < n; ++i) {
-        const int b = trg_leaf[i], j = trg_ord[i];
-        trgv[i] = {local(t.trg_sorted, j, b, 0), local(t.trg_sorted, j, b, 1), local(t.trg_sorted, j, b, 2), Real(0)};
-        tu[i] = t.trg_order[j];
-    }
+    host_parallel(n, [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i) {
+            const int b = trg_leaf[i], j = trg_ord[i];
+            trgv[i] = {local(t.trg_sorted, j, b, 0), local(t.trg_sorted, j, b, 1), local(t.trg_sorted, j, b, 2),
+                       Real(0)};
+            tu[i] = t.trg_order[j];
+        }
+    });
     for (int b = 0; b < nboxes; ++b)
         if (t.leaf[b] && t.trg_count[b] > 0) {
             const double w = T.w0[t.level[b] + t.pwexp[b]];
@@ -655,25 +691,34 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 nf_ne_max = std::max(nf_ne_max, ne);
                 rsb[i] = n_slots;
                 n_slots += ne + 1;
-                // slot = (end step << 8) | entry ordinal; unused slots hold the terminator
-                rend.resize((size_t)n_slots * NF_ROWS, (0x7fffff << 8) | std::min(ne, 255));
-                int kmax = 0;
-                for (int g = 0; g < NF_ROWS; ++g) {
-                    int k = 0, sl = 0;
-                    for (int e = 0; e < ne; ++e) {
-                        const int ce = c[(size_t)(ieb[i] + e) * NF_ROWS + g];
-                        if (ce == 0) continue;
-                        k += ce;
-                        rend[(size_t)(rsb[i] + sl) * NF_ROWS + g] = (std::min(k, 0x7fffff) << 8) | e;
-                        ++sl;
+            }
+            rend.assign((size_t)n_slots * NF_ROWS, 0);
+            // items own disjoint slot ranges: rows of all items on the host threads
+            host_parallel(n_nf_items, [&](int i0, int i1) {
+                for (int i = i0; i < i1; ++i) {
+                    const int ne = ptr[itv[i].z + 1] - ptr[itv[i].z];
+                    // slot = (end step << 8) | entry ordinal; unused slots hold the terminator
+                    std::fill(rend.begin() + (size_t)rsb[i] * NF_ROWS, rend.begin() + (size_t)(rsb[i] + ne + 1) * NF_ROWS,
+                              (0x7fffff << 8) | std::min(ne, 255));
+                    int kmax = 0;
+                    for (int g = 0; g < NF_ROWS; ++g) {
+                        int k = 0, sl = 0;
+                        for (int e = 0; e < ne; ++e) {
+                            const int ce = c[(size_t)(ieb[i] + e) * NF_ROWS + g];
+                            if (ce == 0) continue;
+                            k += ce;
+                            rend[(size_t)(rsb[i] + sl) * NF_ROWS + g] = (std::min(k, 0x7fffff) << 8) | e;
+                            ++sl;
+                        }
+                        kmax = std::max(kmax, k);
                     }
-                    kmax = std::max(kmax, k);
+                    itv[i].w = (kmax + 3) & ~3;  // K
                 }
-                const int K = (kmax + 3) & ~3;
-                k_item_max = std::max(k_item_max, K);
-                itv[i].w = K;
+            });
+            for (int i = 0; i < n_nf_items; ++i) {
+                k_item_max = std::max(k_item_max, itv[i].w);
                 wbase[i] = total;
-                total += (long long)K * NF_ROWS;
+                total += (long long)itv[i].w * NF_ROWS;
             }
             n_nf_steps = total / NF_ROWS;  // steps per row summed over the items
             // beyond int indexing, 8-bit entry ordinals or 23-bit step counts: wavefront lists
