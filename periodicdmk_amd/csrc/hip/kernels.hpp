@@ -360,6 +360,10 @@ constexpr int tf_slices(int total, int per_slice, int reserved = 0, int budget =
 // workgroup of its own (grid y), so a level with few boxes still fills the GPU.
 // BK: the LDS budget in KiB per workgroup (0 = the default below); the autotuner picks
 // among 0, 16, 32 and 64 per plan.
+// slices per box from which the proxy <-> plane-wave transforms take their first contraction
+// of whole boxes from HBM (two passes) rather than re-reading each box's input per slice
+constexpr int kTwoPassSlices = 8;
+
 template <class R, int P, int NPW, int BK = 0> struct P2pwCfg {
     static constexpr int H = (NPW + 1) / 2;
     // (FP64 at p = 28: one k3 plane per workgroup, 31 KB, five workgroups per CU; two planes at
@@ -547,10 +551,39 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // (W(k, j) at W[k + n j]); one workgroup per (box, slice of k3), 1-D grid, (box, slice) at
 // xcd_block index box * slices + slice (a box's slices share one L2).  Box x reads proxy
 // slot slot0 + x and writes plane-wave slot x.
-template <class R, int P, int NPW, int BK = 0>
+// The z step of whole boxes into HBM, for orders where the slices of k_proxy2pw would each
+// re-read the box's proxy charges (20 slices per box for FP64 at p = 28, 28 at p = 38):
+// t1[box][k][b][a] = sum_c W(k, c) C[c][b][a] for k < H, one thread per column (b, a), the
+// same sums as k_proxy2pw's z step.  k_proxy2pw<..., true> then reads its slice of t1.
+template <class R, int P, int NPW>
+__global__ __launch_bounds__(256) void k_p2pw_z(const R *__restrict__ proxy, Cx<R> *__restrict__ t1,
+                                                const Cx<R> *__restrict__ W, int slot0, int n_boxes) {
+    chain_priority();
+    constexpr int H = (NPW + 1) / 2, PP = P * P, PER = (PP + 255) / 256;  // workgroups per box
+    const int lg = xcd_block(blockIdx.x, gridDim.x), bx = lg / PER, j = (lg - bx * PER) * 256 + (int)threadIdx.x;
+    if (bx >= n_boxes || j >= PP) return;
+    const R *C = proxy + (size_t)(slot0 + bx) * P * PP;
+    Cx<R> *out = t1 + (size_t)bx * H * PP;
+    R x[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) x[c] = C[c * PP + j];
+    for (int k = 0; k < H; ++k) {
+        R sr = 0, si = 0;
+#pragma unroll
+        for (int c = 0; c < P; c += 2) sr = fma(W[k + NPW * c].re, x[c], sr);
+#pragma unroll
+        for (int c = 1; c < P; c += 2) si = fma(W[k + NPW * c].im, x[c], si);
+        out[(size_t)k * PP + j] = {sr, si};
+    }
+}
+
+// T1G: the z step comes from k_p2pw_z's t1 (box bx at t1 + bx H p^2) instead of the proxy
+// charges.
+template <class R, int P, int NPW, int BK = 0, bool T1G = false>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
                                                    const Cx<R> *__restrict__ W, int slot0, int n_boxes,
-                                                   long long vs_in = 0, long long vs_out = 0) {
+                                                   long long vs_in = 0, long long vs_out = 0,
+                                                   const Cx<R> *__restrict__ t1g = nullptr) {
     chain_priority();
     proxy += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
     pw_out += (long long)blockIdx.z * vs_out;
@@ -567,17 +600,22 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
         const int k30 = slice * K3;  // this workgroup's slice of k3
         const int nk = min(K3, H - k30);
         // z: T1[k][b][a] = sum_c W(k30 + k, c) C[c][b][a]
-        for (int j = threadIdx.x; j < PP; j += blockDim.x) {
-            R x[P];
+        if constexpr (T1G) {
+            const Cx<R> *src = t1g + ((size_t)bx * H + k30) * PP;
+            for (int j = threadIdx.x; j < nk * PP; j += blockDim.x) T1[j] = src[j];
+        } else {
+            for (int j = threadIdx.x; j < PP; j += blockDim.x) {
+                R x[P];
 #pragma unroll
-            for (int c = 0; c < P; ++c) x[c] = C[c * PP + j];
-            for (int k = 0; k < nk; ++k) {
-                R sr = 0, si = 0;
+                for (int c = 0; c < P; ++c) x[c] = C[c * PP + j];
+                for (int k = 0; k < nk; ++k) {
+                    R sr = 0, si = 0;
 #pragma unroll
-                for (int c = 0; c < P; c += 2) sr = fma(W[k30 + k + NPW * c].re, x[c], sr);
+                    for (int c = 0; c < P; c += 2) sr = fma(W[k30 + k + NPW * c].re, x[c], sr);
 #pragma unroll
-                for (int c = 1; c < P; c += 2) si = fma(W[k30 + k + NPW * c].im, x[c], si);
-                T1[k * PP + j] = {sr, si};
+                    for (int c = 1; c < P; c += 2) si = fma(W[k30 + k + NPW * c].im, x[c], si);
+                    T1[k * PP + j] = {sr, si};
+                }
             }
         }
         __syncthreads();
@@ -674,11 +712,49 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
 // (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  One
 // workgroup per (box, slice of a).
-template <class R, int P, int NPW, int BK = 0>
+// The x step of whole boxes into HBM, for orders where the slices of k_pw2proxy would each
+// re-read the box's incoming expansion (14 slices per box for FP64 at p = 28, 38 at p = 38):
+// u1[it][a][k3][k2] = sum_k1 V(k1, a) pin[it][k1][k3][k2] for every a, one thread per column
+// (k3, k2), the same sums as k_pw2proxy's x step.  k_pw2proxy<..., true> then reads its
+// slice of u1.
+template <class R, int P, int NPW>
+__global__ __launch_bounds__(256) void k_pw2p_x(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
+                                                int n_items, Cx<R> *__restrict__ u1) {
+    chain_priority();
+    constexpr int H = (NPW + 1) / 2, COL = H * NPW, PER = (COL + 255) / 256;  // workgroups per box
+    const int lg = xcd_block(blockIdx.x, gridDim.x), it = lg / PER, col = (lg - it * PER) * 256 + (int)threadIdx.x;
+    if (it >= n_items || col >= COL) return;
+    const Cx<R> *X = pin + (size_t)it * NPW * COL;
+    Cx<R> acc[P];
+#pragma unroll
+    for (int a = 0; a < P; ++a) acc[a] = {R(0), R(0)};
+    for (int k1 = 0; k1 < NPW; ++k1) {
+        const Cx<R> x = X[(size_t)k1 * COL + col];
+#pragma unroll
+        for (int a = 0; a < P; ++a) {
+            if (a % 2 == 0) {
+                const R m = V[k1 + NPW * a].re;
+                acc[a].re = fma(m, x.re, acc[a].re);
+                acc[a].im = fma(m, x.im, acc[a].im);
+            } else {
+                const R m = V[k1 + NPW * a].im;
+                acc[a].re = fma(-m, x.im, acc[a].re);
+                acc[a].im = fma(m, x.re, acc[a].im);
+            }
+        }
+    }
+    Cx<R> *out = u1 + (size_t)it * P * COL + col;
+#pragma unroll
+    for (int a = 0; a < P; ++a) out[(size_t)a * COL] = acc[a];
+}
+
+// U1G: the x step comes from k_pw2p_x's u1 (box it at u1 + it p H n) instead of pin.
+template <class R, int P, int NPW, int BK = 0, bool U1G = false>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
                                                    int item0, int n_items, R *__restrict__ down_base,
-                                                   long long vs_in = 0, long long vs_out = 0) {
+                                                   long long vs_in = 0, long long vs_out = 0,
+                                                   const Cx<R> *__restrict__ u1g = nullptr) {
     chain_priority();
     pin += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
     down_base += (long long)blockIdx.z * vs_out;
@@ -731,8 +807,14 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
                     if (a < na) U1[a * COL + col] = acc[a];
             }
         };
-        if (a0 & 1) xstep(Odd{});
-        else xstep(Even{});
+        if constexpr (U1G) {
+            const Cx<R> *src = u1g + ((size_t)it * P + a0) * COL;
+            for (int j = threadIdx.x; j < na * COL; j += blockDim.x) U1[j] = src[j];
+        } else if (a0 & 1) {
+            xstep(Odd{});
+        } else {
+            xstep(Even{});
+        }
         __syncthreads();
         // y: U2[a][k3][b] = h(k3) sum_k2 V(k2, b) U1[a][k3][k2]; slot = (column w = a H + k3,
         // segment of b, uniform across a wavefront)

@@ -52,6 +52,7 @@ void *Arena::take(size_t nbytes) {
 namespace {
 
 constexpr size_t kPinBytes = size_t(512) << 20;  // incoming plane-wave scratch cap
+constexpr size_t kTwoPassBytes = size_t(512) << 20;  // two-pass transform scratch cap
 
 // f(i0, i1) over contiguous chunks of [0, n) on up to 16 host threads (construction work
 // whose chunks write disjoint data)
@@ -874,6 +875,18 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         for (auto &E : eval_levels) items = std::max(items, E.n_items);
         pin_batch = std::max(1, std::min(items, (int)(kPinBytes / ((size_t)NM * 2 * sizeof(Real)))));
         pin.alloc((size_t)pin_batch * NM * 2 * sizeof(Real));
+        // two-pass proxy <-> plane-wave transforms (FP64 at p >= 28, whose slices re-read a
+        // box's input kTwoPassSlices times or more): the first contraction of a chunk of boxes
+        const int Hh = (n_pw + 1) / 2;
+        if (sizeof(Real) == 8 && p >= 28) {
+            const size_t z_box = (size_t)Hh * p * p * 2 * sizeof(Real);  // H p^2 complex
+            const size_t x_box = (size_t)p * Hh * n_pw * 2 * sizeof(Real);      // p H n complex
+            const size_t want = std::max((size_t)std::max(1, n_up) * z_box, (size_t)pin_batch * x_box);
+            const size_t bytes = std::min(want, std::max(kTwoPassBytes, std::max(z_box, x_box)));
+            tp_scratch.alloc(bytes);
+            tp_boxes_p2pw = (int)std::max<size_t>(1, bytes / z_box);
+            tp_boxes_pw2p = (int)std::max<size_t>(1, bytes / x_box);
+        }
     }
     // ---------------- outgoing plane waves and the child -> parent scratch: all up slots when
     // they fit in the free device memory (less 1 GiB), else one level's slot range at a time
@@ -1036,8 +1049,27 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
                            B.s_up, B.s_c2p);
         check(hipGetLastError(), "k_sum_children");
     }
+    // Where a box's slices would each re-read its input kTwoPassSlices times or more, the first
+    // contraction of whole boxes goes to HBM first (k_p2pw_z, k_pw2p_x; the same sums), in
+    // chunks of boxes that fit the plan's two-pass scratch.
     auto proxy2pw_bk = [&](auto bk, int slot0, int count) {
         constexpr int BK = decltype(bk)::value;
+        if constexpr (P2pwCfg<Real, P, NPW, BK>::slices >= kTwoPassSlices) {
+            if (pl.tp_scratch.p && nv == 1) {
+                C *t1 = pl.tp_scratch.template as<C>();
+                for (int c0 = 0; c0 < count; c0 += pl.tp_boxes_p2pw) {
+                    const int m = std::min(pl.tp_boxes_p2pw, count - c0);
+                    hipLaunchKernelGGL((k_p2pw_z<Real, P, NPW>), dim3(m * ((P * P + 255) / 256)), dim3(256), 0, s, B.up, t1,
+                                       pl.poly2pw.template as<C>(), slot0 + c0, m);
+                    check(hipGetLastError(), "k_p2pw_z");
+                    hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW, BK, true>), dim3(m * P2pwCfg<Real, P, NPW, BK>::slices),
+                                       dim3(256), 0, s, B.up, B.pw + (size_t)c0 * NM, pl.poly2pw.template as<C>(),
+                                       slot0 + c0, m, 0LL, 0LL, (const C *)t1);
+                    check(hipGetLastError(), "k_proxy2pw");
+                }
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW, BK>), dim3(count * P2pwCfg<Real, P, NPW, BK>::slices, 1, nv),
                            dim3(256), 0, s, B.up, B.pw, pl.poly2pw.template as<C>(), slot0, count, B.s_up, B.s_pw);
         check(hipGetLastError(), "k_proxy2pw");
@@ -1052,6 +1084,24 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
     };
     auto pw2proxy_bk = [&](auto bk, const typename Plan<Real>::EvalLevel &E, int i0, int nb) {
         constexpr int BK = decltype(bk)::value;
+        if constexpr (Pw2pCfg<Real, P, NPW, BK>::slices >= kTwoPassSlices) {
+            if (pl.tp_scratch.p && nv == 1) {
+                C *u1 = pl.tp_scratch.template as<C>();
+                constexpr int COL = NPW * ((NPW + 1) / 2);
+                for (int c0 = 0; c0 < nb; c0 += pl.tp_boxes_pw2p) {
+                    const int m = std::min(pl.tp_boxes_pw2p, nb - c0);
+                    hipLaunchKernelGGL((k_pw2p_x<Real, P, NPW>), dim3(m * ((COL + 255) / 256)), dim3(256), 0, s,
+                                       B.pin + (size_t)c0 * NM, pl.pw2poly.template as<C>(), m, u1);
+                    check(hipGetLastError(), "k_pw2p_x");
+                    hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW, BK, true>), dim3(m * Pw2pCfg<Real, P, NPW, BK>::slices),
+                                       dim3(256), 0, s, B.pin + (size_t)c0 * NM, pl.pw2poly.template as<C>(),
+                                       E.down.template as<int>(), E.zero.template as<int>(), i0 + c0, m, B.down, 0LL, 0LL,
+                                       (const C *)u1);
+                    check(hipGetLastError(), "k_pw2proxy");
+                }
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW, BK>), dim3(nb * Pw2pCfg<Real, P, NPW, BK>::slices, 1, nv), dim3(256),
                            0, s, B.pin, pl.pw2poly.template as<C>(), E.down.template as<int>(),
                            E.zero.template as<int>(), i0, nb, B.down, B.s_pin, B.s_down);

@@ -199,6 +199,10 @@ struct Plan {
     // incoming plane waves of up to pin_batch boxes of one level
     DevBuf pin;
     int pin_batch = 0;
+    // first contractions of the two-pass proxy <-> plane-wave transforms, for up to
+    // tp_boxes_p2pw / tp_boxes_pw2p boxes at a time (empty when the plan does not use them)
+    DevBuf tp_scratch;
+    int tp_boxes_p2pw = 0, tp_boxes_pw2p = 0;
 
     // bloch (optional, 3 values): quasi-periodic tree; then only evaluate_complex applies
     Plan(const double *cell_colmajor, const double *pos, int N, int n_digits, int n_per_leaf, int nz,
