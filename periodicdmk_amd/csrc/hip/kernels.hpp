@@ -554,7 +554,8 @@ __global__ __launch_bounds__(256) void k_sum_children(R *__restrict__ proxy, con
 // The z step of whole boxes into HBM, for orders where the slices of k_proxy2pw would each
 // re-read the box's proxy charges (20 slices per box for FP64 at p = 28, 28 at p = 38):
 // t1[box][k][b][a] = sum_c W(k, c) C[c][b][a] for k < H, one thread per column (b, a), the
-// same sums as k_proxy2pw's z step.  k_proxy2pw<..., true> then reads its slice of t1.
+// same sums as k_proxy2pw's z step.  k_p2pw_yx then takes each plane of t1 through the y and
+// x steps.
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_p2pw_z(const R *__restrict__ proxy, Cx<R> *__restrict__ t1,
                                                 const Cx<R> *__restrict__ W, int slot0, int n_boxes) {
@@ -577,13 +578,10 @@ __global__ __launch_bounds__(256) void k_p2pw_z(const R *__restrict__ proxy, Cx<
     }
 }
 
-// T1G: the z step comes from k_p2pw_z's t1 (box bx at t1 + bx H p^2) instead of the proxy
-// charges.
-template <class R, int P, int NPW, int BK = 0, bool T1G = false>
+template <class R, int P, int NPW, int BK = 0>
 __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, Cx<R> *__restrict__ pw_out,
                                                    const Cx<R> *__restrict__ W, int slot0, int n_boxes,
-                                                   long long vs_in = 0, long long vs_out = 0,
-                                                   const Cx<R> *__restrict__ t1g = nullptr) {
+                                                   long long vs_in = 0, long long vs_out = 0) {
     chain_priority();
     proxy += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
     pw_out += (long long)blockIdx.z * vs_out;
@@ -600,22 +598,17 @@ __global__ __launch_bounds__(256) void k_proxy2pw(const R *__restrict__ proxy, C
         const int k30 = slice * K3;  // this workgroup's slice of k3
         const int nk = min(K3, H - k30);
         // z: T1[k][b][a] = sum_c W(k30 + k, c) C[c][b][a]
-        if constexpr (T1G) {
-            const Cx<R> *src = t1g + ((size_t)bx * H + k30) * PP;
-            for (int j = threadIdx.x; j < nk * PP; j += blockDim.x) T1[j] = src[j];
-        } else {
-            for (int j = threadIdx.x; j < PP; j += blockDim.x) {
-                R x[P];
+        for (int j = threadIdx.x; j < PP; j += blockDim.x) {
+            R x[P];
 #pragma unroll
-                for (int c = 0; c < P; ++c) x[c] = C[c * PP + j];
-                for (int k = 0; k < nk; ++k) {
-                    R sr = 0, si = 0;
+            for (int c = 0; c < P; ++c) x[c] = C[c * PP + j];
+            for (int k = 0; k < nk; ++k) {
+                R sr = 0, si = 0;
 #pragma unroll
-                    for (int c = 0; c < P; c += 2) sr = fma(W[k30 + k + NPW * c].re, x[c], sr);
+                for (int c = 0; c < P; c += 2) sr = fma(W[k30 + k + NPW * c].re, x[c], sr);
 #pragma unroll
-                    for (int c = 1; c < P; c += 2) si = fma(W[k30 + k + NPW * c].im, x[c], si);
-                    T1[k * PP + j] = {sr, si};
-                }
+                for (int c = 1; c < P; c += 2) si = fma(W[k30 + k + NPW * c].im, x[c], si);
+                T1[k * PP + j] = {sr, si};
             }
         }
         __syncthreads();
@@ -707,6 +700,95 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
         si = fma(a.re, s.im, fma(a.im, s.re, si));
     }
     pin[(size_t)bi * nm + m] = {sr * lscale, si * lscale};
+}
+
+// Folded matrix table of the plane-wave transforms: F[j * 64 + k] = W(k, j).re for even j,
+// W(k, j).im for odd j (the entry's only non-zero part), 0 for k >= n; k in a row of 64.
+//
+// Second pass of the two-pass proxy -> plane-wave transform (after k_p2pw_z), FP64 on the
+// matrix cores: the y and x steps of one k3 plane per wavefront.  With the parity folded into
+// the operands, each step is a pair of real products with F:
+//   y:  T2t[a][k2] = sum_b X[b][a] F[b][k2],  X = (T.re, T.im) for even b, (-T.im, T.re) for odd b
+//   x:  pw[k1][k2] = sum_a F[a][k1] Z[a][k2], Z = (T2.re, T2.im) for even a, (-T2.im, T2.re) odd a
+// (T[b][a] = the plane of t1).  v_mfma_f64_16x16x4f64 leaves y's output tile with lane l
+// holding rows a = 4 e + l/16 of column l%16, which is the B operand x needs for the
+// 4-row steps e = 0..3 of that tile, so the plane goes from y to x in registers, one 16-column
+// tile of k2 at a time.  F sits in LDS (A and B fragments of both steps); the plane's t1 is
+// read once per wavefront into registers.  Each output sums its terms in the same order as
+// k_proxy2pw (b, then a, ascending).
+template <int P, int NPW>
+__global__ __launch_bounds__(256) void k_p2pw_yx(const Cx<double> *__restrict__ t1, const double *__restrict__ F,
+                                                 Cx<double> *__restrict__ pw_out, int n_boxes) {
+    chain_priority();
+    static_assert(NPW <= 64, "k2 and k1 within a table row of 64");
+    using M = Mfma16<double>;
+    typedef M::acc_t acc_t;
+    constexpr int H = (NPW + 1) / 2, COL = H * NPW, PP = P * P, PG = (H + 3) / 4;  // 4 planes per workgroup
+    constexpr int KS = (P + 3) / 4, AT = (P + 15) / 16, CT = (NPW + 15) / 16;     // 4-steps, a tiles, k tiles
+    __shared__ double Fs[4 * KS * 64];
+    for (int i = threadIdx.x; i < 4 * KS * 64; i += 256) Fs[i] = i < P * 64 ? F[i] : 0.0;
+    __syncthreads();
+    const int lg = xcd_block(blockIdx.x, gridDim.x), bx = lg / PG;
+    const int k = (lg - bx * PG) * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    if (bx >= n_boxes || k >= H) return;
+    const int l = threadIdx.x & 63, kk = l >> 4, jj = l & 15;
+    const bool odd = kk & 1;  // parity of b = 4 ks + kk (y) and of a = 16 at + 4 e + kk (x)
+    const Cx<double> *T = t1 + ((size_t)bx * H + k) * PP;
+    // y's A fragments: X[b = 4 ks + kk][a = 16 at + jj], real-output and imaginary-output parts
+    double xr[KS][AT], xi[KS][AT];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int at = 0; at < AT; ++at) {
+            const int b = 4 * ks + kk, a = 16 * at + jj;
+            Cx<double> x = {0.0, 0.0};
+            if (b < P && a < P) x = T[b * P + a];
+            xr[ks][at] = odd ? -x.im : x.re;
+            xi[ks][at] = odd ? x.re : x.im;
+        }
+    Cx<double> *dst = pw_out + (size_t)bx * NPW * COL + (size_t)k * NPW;
+    for (int ct = 0; ct < CT; ++ct) {  // k2 tile
+        acc_t yr[AT], yi[AT];
+#pragma unroll
+        for (int at = 0; at < AT; ++at) yr[at] = yi[at] = acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const double f = Fs[(4 * ks + kk) * 64 + 16 * ct + jj];  // B[b][k2] = F[b][k2]
+#pragma unroll
+            for (int at = 0; at < AT; ++at) {
+                yr[at] = M::mma(xr[ks][at], f, yr[at]);
+                yi[at] = M::mma(xi[ks][at], f, yi[at]);
+            }
+        }
+        // yr/yi[at][e] = T2 (re, im) at a = 16 at + 4 e + kk, k2 = 16 ct + jj: fold into Z
+#pragma unroll
+        for (int at = 0; at < AT; ++at)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double re = yr[at][e], im = yi[at][e];
+                yr[at][e] = odd ? -im : re;
+                yi[at][e] = odd ? re : im;
+            }
+        for (int mt = 0; mt < CT; ++mt) {  // k1 tile
+            acc_t pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int at = 0; at < AT; ++at)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (4 * at + e < KS) {
+                        const double fa = Fs[(4 * (4 * at + e) + kk) * 64 + 16 * mt + jj];  // A[k1][a] = F[a][k1]
+                        pr = M::mma(fa, yr[at][e], pr);
+                        pi = M::mma(fa, yi[at][e], pi);
+                    }
+                }
+            const int k2 = 16 * ct + jj;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k1 = 16 * mt + M::row(kk, e);
+                if (k1 < NPW && k2 < NPW) dst[(size_t)k1 * COL + k2] = {pr[e], pi[e]};
+            }
+        }
+    }
 }
 
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly

@@ -478,6 +478,19 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         poly2pw.upload(a);
         pw2poly.upload(b);
         shift0.upload(s);
+        // folded tables F[j * 64 + k] = the non-zero part of W(k, j) (real for even j,
+        // imaginary for odd j), for the two-pass transforms
+        if (n_pw <= 64) {
+            std::vector<Real> fa((size_t)p * 64, Real(0)), fb((size_t)p * 64, Real(0));
+            for (int j = 0; j < p; ++j)
+                for (int k = 0; k < n_pw; ++k) {
+                    const Cx<Real> wa = a[k + (size_t)n_pw * j], wb = b[k + (size_t)n_pw * j];
+                    fa[(size_t)j * 64 + k] = (j % 2 == 0) ? wa.re : wa.im;
+                    fb[(size_t)j * 64 + k] = (j % 2 == 0) ? wb.re : wb.im;
+                }
+            poly2pw_f.upload(fa);
+            pw2poly_f.upload(fb);
+        }
         // per box: z step from the real proxy (one multiply-add per term), y and x steps on
         // complex values against matrix entries that are real or imaginary (two per term)
         work.transform_flops += (double)n_up * (2.0 * (double)p * p * H * p + 4.0 * (double)p * n_pw * H * p +
@@ -877,12 +890,16 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         pin.alloc((size_t)pin_batch * NM * 2 * sizeof(Real));
         // two-pass proxy <-> plane-wave transforms (FP64 at p >= 28, whose slices re-read a
         // box's input kTwoPassSlices times or more): the first contraction of a chunk of boxes
+        // (PDMK_TWO_PASS=0 turns them off, PDMK_TWO_PASS=<MiB> caps the chunk scratch)
         const int Hh = (n_pw + 1) / 2;
-        if (sizeof(Real) == 8 && p >= 28) {
+        long long tp_mib = -1;
+        if (const char *e = std::getenv("PDMK_TWO_PASS")) tp_mib = std::atoll(e);
+        if (sizeof(Real) == 8 && p >= 28 && tp_mib != 0) {
             const size_t z_box = (size_t)Hh * p * p * 2 * sizeof(Real);  // H p^2 complex
             const size_t x_box = (size_t)p * Hh * n_pw * 2 * sizeof(Real);      // p H n complex
             const size_t want = std::max((size_t)std::max(1, n_up) * z_box, (size_t)pin_batch * x_box);
-            const size_t bytes = std::min(want, std::max(kTwoPassBytes, std::max(z_box, x_box)));
+            const size_t cap = tp_mib > 0 ? (size_t)tp_mib << 20 : kTwoPassBytes;
+            const size_t bytes = std::min(want, std::max(cap, std::max(z_box, x_box)));
             tp_scratch.alloc(bytes);
             tp_boxes_p2pw = (int)std::max<size_t>(1, bytes / z_box);
             tp_boxes_pw2p = (int)std::max<size_t>(1, bytes / x_box);
@@ -1054,7 +1071,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
     // chunks of boxes that fit the plan's two-pass scratch.
     auto proxy2pw_bk = [&](auto bk, int slot0, int count) {
         constexpr int BK = decltype(bk)::value;
-        if constexpr (P2pwCfg<Real, P, NPW, BK>::slices >= kTwoPassSlices) {
+        if constexpr (sizeof(Real) == 8 && P >= 28 && P2pwCfg<Real, P, NPW, BK>::slices >= kTwoPassSlices) {
             if (pl.tp_scratch.p && nv == 1) {
                 C *t1 = pl.tp_scratch.template as<C>();
                 for (int c0 = 0; c0 < count; c0 += pl.tp_boxes_p2pw) {
@@ -1062,10 +1079,10 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
                     hipLaunchKernelGGL((k_p2pw_z<Real, P, NPW>), dim3(m * ((P * P + 255) / 256)), dim3(256), 0, s, B.up, t1,
                                        pl.poly2pw.template as<C>(), slot0 + c0, m);
                     check(hipGetLastError(), "k_p2pw_z");
-                    hipLaunchKernelGGL((k_proxy2pw<Real, P, NPW, BK, true>), dim3(m * P2pwCfg<Real, P, NPW, BK>::slices),
-                                       dim3(256), 0, s, B.up, B.pw + (size_t)c0 * NM, pl.poly2pw.template as<C>(),
-                                       slot0 + c0, m, 0LL, 0LL, (const C *)t1);
-                    check(hipGetLastError(), "k_proxy2pw");
+                    hipLaunchKernelGGL((k_p2pw_yx<P, NPW>), dim3(m * (((NPW + 1) / 2 + 3) / 4)), dim3(256), 0, s,
+                                       (const Cx<double> *)t1,
+                                       pl.poly2pw_f.template as<double>(), (Cx<double> *)(B.pw + (size_t)c0 * NM), m);
+                    check(hipGetLastError(), "k_p2pw_yx");
                 }
                 return;
             }
@@ -1084,7 +1101,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
     };
     auto pw2proxy_bk = [&](auto bk, const typename Plan<Real>::EvalLevel &E, int i0, int nb) {
         constexpr int BK = decltype(bk)::value;
-        if constexpr (Pw2pCfg<Real, P, NPW, BK>::slices >= kTwoPassSlices) {
+        if constexpr (sizeof(Real) == 8 && P >= 28 && Pw2pCfg<Real, P, NPW, BK>::slices >= kTwoPassSlices) {
             if (pl.tp_scratch.p && nv == 1) {
                 C *u1 = pl.tp_scratch.template as<C>();
                 constexpr int COL = NPW * ((NPW + 1) / 2);

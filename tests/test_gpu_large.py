@@ -100,3 +100,35 @@ def test_clustered_triclinic_4m(pdmk, cuda, clustered_4m, clustered_4m_ewald, di
         np.testing.assert_array_equal(s.evaluate(q).astype(np.float64), pot)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("digits,n,npl", [(9, 100_000, 40), (12, 40_000, 40)])
+def test_two_pass_transforms_identical(pdmk, cuda, monkeypatch, digits, n, npl):
+    """FP64 at 9 and 12 digits (p = 28 and 38) forms the first contraction of the proxy <->
+    plane-wave transforms for whole boxes in an HBM scratch and takes each box's slices from it,
+    several slices per workgroup when the boxes are many.  The sums are the one-pass kernels':
+    the potentials must equal those with the two passes off (PDMK_TWO_PASS=0), also with a 1 MiB
+    scratch that splits every level into chunks of a few boxes, and the many-vector path (which
+    runs the one-pass kernels) must give each vector's single-vector potentials."""
+    cell = W.cubic_cell()
+    pos = W.uniform_positions(n, 31, cell)
+    Q = np.stack([W.neutral_uniform_charges(n, 40 + k) for k in range(2)])
+    out = {}
+    for mode in ("0", None, "1"):
+        if mode is None:
+            monkeypatch.delenv("PDMK_TWO_PASS", raising=False)
+        else:
+            monkeypatch.setenv("PDMK_TWO_PASS", mode)
+        s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=npl, nz=4, precision="double")
+        monkeypatch.delenv("PDMK_TWO_PASS", raising=False)
+        try:
+            out[mode] = s.evaluate(Q[0])
+            if mode is None:
+                many = s.evaluate_many(Q)
+                np.testing.assert_array_equal(many[0], out[mode])
+                np.testing.assert_array_equal(many[1], s.evaluate(Q[1]))
+                print(f"{digits} digits, N={n}: {s.build_info()['proxy_slots']} proxy slots", flush=True)
+        finally:
+            s.close()
+    np.testing.assert_array_equal(out[None], out["0"])
+    np.testing.assert_array_equal(out["1"], out["0"])
