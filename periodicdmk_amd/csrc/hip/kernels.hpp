@@ -460,6 +460,15 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
         for (int c = 0; c < P; ++c) s = fma(urow[c], x[c], s);
         return s;
     };
+    // the same sums for four consecutive rows at once (four independent dependency chains)
+    auto row_dot4 = [](const R *urow, const R (&x)[P], R (&o)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[i] = R(0);
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o[i] = fma(urow[i * PR + c], x[c], o[i]);
+    };
     // x-step accumulators: work slot (column (k, b'), segment of a') -- persistent over children
     R acc[XC][PS];
 #pragma unroll
@@ -477,7 +486,14 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
             R x[P];
 #pragma unroll
             for (int c = 0; c < P; ++c) x[c] = in[c * PP + j];
-            for (int k = 0; k < nk; ++k) T1[k * PP + j] = row_dot(U2 + (k0 + k) * PR, x, R(0));
+            int k = 0;
+            for (; k + 4 <= nk; k += 4) {  // four independent sums at a time
+                R o[4];
+                row_dot4(U2 + (k0 + k) * PR, x, o);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) T1[(k + i) * PP + j] = o[i];
+            }
+            for (; k < nk; ++k) T1[k * PP + j] = row_dot(U2 + (k0 + k) * PR, x, R(0));
         }
         __syncthreads();
         // y: T2[k][b'][a] = sum_b U1(b', b) T1[k][b][a]; slot = (column (k, a), segment of b')
@@ -489,7 +505,15 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
             R x[P];
 #pragma unroll
             for (int b = 0; b < P; ++b) x[b] = T1[k * PP + b * P + a];
-            for (int b2 = sg * PS; b2 < min(P, sg * PS + PS); ++b2) T2[(k * P + b2) * TS + a] = row_dot(U1 + b2 * PR, x, R(0));
+            const int b2e = min(P, sg * PS + PS);
+            int b2 = sg * PS;
+            for (; b2 + 4 <= b2e; b2 += 4) {
+                R o[4];
+                row_dot4(U1 + b2 * PR, x, o);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) T2[(k * P + b2 + i) * TS + a] = o[i];
+            }
+            for (; b2 < b2e; ++b2) T2[(k * P + b2) * TS + a] = row_dot(U1 + b2 * PR, x, R(0));
         }
         __syncthreads();
         // x: acc[a'] += sum_a U0(a', a) T2[k][b'][a]; slot = (column (k, b'), segment of a')
@@ -519,6 +543,135 @@ __global__ __launch_bounds__((T3Cfg<R, P>::threads)) void k_tensor3(
             for (int i = 0; i < PS; ++i)
                 if (sg * PS + i < P) dst[i] = add ? dst[i] + acc[q][i] : acc[q][i];
         }
+    }
+}
+
+// The same transform for FP64 at p = 28 on the matrix cores (v_mfma_f64_16x16x4f64,
+// operand and accumulator layout as in k_charge2proxy_mfma).  Workgroup (item, ct) computes
+// the output planes c' in [16 ct, 16 ct + 16), the item's inputs taken in slices of 4 values
+// of a:
+//   z:  T1[c'][(b, al)]  = sum_c U2(c', c) in[c][b][4 s + al]        (16 rows, 4 p columns)
+//   y:  T2[(c', b')][al] = sum_b U1(b', b) T1[c'][(b, al)]            (rows b', 64 columns)
+//   x:  out[(c', b')][a'] += sum_al U0(a', 4 s + al) T2[(c', b')][al] (rows a', 16 p columns)
+// z and y go through LDS, x into accumulators that persist over the slices and the item's
+// inputs (a wave owns x's column tiles w, w + NW, ...).  Every output sums its terms in the
+// order of k_tensor3 (c, then b, then a ascending, inputs in order), so the two kernels agree
+// to the last bit.  LDS holds both matrices transposed (Ut[c][r], zero-padded, so fragment
+// reads need no bounds checks) and the two slice buffers (zero where c' or b is past p).
+template <int P> struct T3MfmaCfg {
+    static constexpr int PP = P * P;
+    static constexpr int KS = (P + 3) / 4;        // 4-steps of a contraction = slices of a
+    static constexpr int CT = (P + 15) / 16;      // 16-row tiles of an output index
+    static constexpr int NW = P >= 38 ? 8 : 4;    // waves per workgroup
+    static constexpr int XT = (P + NW - 1) / NW;  // x column tiles per wave (16 p columns = p tiles)
+    static constexpr int LP = 16 * CT + 8;        // row stride of Ut (8 mod 16: a fragment's rows
+    static constexpr int S1 = 16 * KS + 8;        // and those of T1 fall on different banks)
+};
+
+template <int P, bool TRANSPOSE>
+__global__ __launch_bounds__(64 * T3MfmaCfg<P>::NW) void k_tensor3_mfma(
+    const double *__restrict__ in_base, double *__restrict__ out_base, const int *__restrict__ item_out,
+    const int *__restrict__ item_add, const int *__restrict__ item_ptr, const int *__restrict__ in_slot,
+    const int *__restrict__ in_code, const double *__restrict__ mats_rm, int n_items, long long vs_in = 0,
+    long long vs_out = 0) {
+    chain_priority();
+    in_base += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    out_base += (long long)blockIdx.z * vs_out;
+    using Cfg = T3MfmaCfg<P>;
+    using M = Mfma16<double>;
+    typedef M::acc_t acc_t;
+    constexpr int PP = Cfg::PP, KS = Cfg::KS, CT = Cfg::CT, NW = Cfg::NW, XT = Cfg::XT, LP = Cfg::LP, S1 = Cfg::S1,
+                  NT = 64 * NW;
+    __shared__ double Ut[2 * 4 * KS * LP];  // Ut[s][c][r] = U_s(r, c), zero outside r, c < p
+    __shared__ double T1[16 * S1];          // [c' - 16 ct][(b, al)]
+    __shared__ double T2[16 * P * 4];       // [(c' - 16 ct, b')][al]
+    const int lg = xcd_block(blockIdx.x, gridDim.x), it = lg / CT, ct = lg - it * CT;
+    if (it >= n_items) return;
+    const double *Ubase = mats_rm + (TRANSPOSE ? 2 : 0) * PP;
+    for (int i = threadIdx.x; i < 2 * 4 * KS * LP; i += NT) {
+        const int sd = i / (4 * KS * LP), c = (i / LP) % (4 * KS), r = i % LP;
+        Ut[i] = (r < P && c < P) ? Ubase[sd * PP + r * P + c] : 0.0;
+    }
+    for (int i = threadIdx.x; i < 16 * S1; i += NT) T1[i] = 0.0;
+    for (int i = threadIdx.x; i < 16 * P * 4; i += NT) T2[i] = 0.0;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), l = threadIdx.x & 63, kk = l >> 4,
+              jj = l & 15;
+    acc_t acc[XT][CT];
+#pragma unroll
+    for (int t = 0; t < XT; ++t)
+#pragma unroll
+        for (int mt = 0; mt < CT; ++mt) acc[t][mt] = acc_t{0.0, 0.0, 0.0, 0.0};
+    const int e0 = item_ptr[it], e1 = item_ptr[it + 1];
+    for (int e = e0; e < e1; ++e) {
+        const double *in = in_base + (size_t)in_slot[e] * P * PP;
+        const int code = in_code[e];
+        const double *U0 = Ut + (code & 1) * 4 * KS * LP, *U1 = Ut + ((code >> 1) & 1) * 4 * KS * LP,
+                     *U2 = Ut + ((code >> 2) & 1) * 4 * KS * LP;
+        for (int s = 0; s < KS; ++s) {
+            __syncthreads();  // Ut and the zero padding written; the previous y done with T1
+            // z: one job per 16 columns (b, al)
+            for (int nt = w; nt < KS; nt += NW) {
+                const int n = 16 * nt + jj, b = n >> 2, a = 4 * s + (n & 3);
+                acc_t d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int c = 4 * ks + kk;
+                    const double bv = (c < P && b < P && a < P) ? in[(size_t)c * PP + b * P + a] : 0.0;
+                    d = M::mma(U2[c * LP + 16 * ct + jj], bv, d);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int cl = M::row(kk, q);
+                    if (16 * ct + cl < P && b < P) T1[cl * S1 + n] = d[q];
+                }
+            }
+            __syncthreads();
+            // y: job = (row tile of b', one of the 4 column tiles of (c', al))
+            for (int job = w; job < 4 * CT; job += NW) {
+                const int mt = job % CT, nt = job / CT;
+                const int n = 16 * nt + jj, cl = n >> 2, al = n & 3;
+                acc_t d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    const int b = 4 * ks + kk;
+                    d = M::mma(U1[b * LP + 16 * mt + jj], T1[cl * S1 + 4 * b + al], d);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int bp = 16 * mt + M::row(kk, q);
+                    if (bp < P && 16 * ct + cl < P) T2[(cl * P + bp) * 4 + al] = d[q];
+                }
+            }
+            __syncthreads();
+            // x: this wave's column tiles of (c', b'), one 4-step over a = 4 s + al
+#pragma unroll
+            for (int t = 0; t < XT; ++t) {
+                const int nt = w + NW * t;
+                if (nt < P) {
+                    const double bv = T2[(16 * nt + jj) * 4 + kk];
+#pragma unroll
+                    for (int mt = 0; mt < CT; ++mt)
+                        acc[t][mt] = M::mma(U0[(4 * s + kk) * LP + 16 * mt + jj], bv, acc[t][mt]);
+                }
+            }
+        }
+    }
+    double *out = out_base + (size_t)item_out[it] * P * PP + (size_t)16 * ct * PP;
+    const bool add = item_add[it] != 0;
+#pragma unroll
+    for (int t = 0; t < XT; ++t) {
+        const int n = 16 * (w + NW * t) + jj;  // (c' - 16 ct) p + b'
+        if (w + NW * t >= P || 16 * ct + n / P >= P) continue;
+#pragma unroll
+        for (int mt = 0; mt < CT; ++mt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int ap = 16 * mt + M::row(kk, q);
+                if (ap < P) {
+                    double *dst = out + (size_t)n * P + ap;
+                    *dst = add ? *dst + acc[t][mt][q] : acc[t][mt][q];
+                }
+            }
     }
 }
 

@@ -918,6 +918,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         const size_t all_bytes = pw_slot * (size_t)std::max(1, n_up) + c2p_bytes;
         pw_by_level = all_bytes + (size_t(1) << 30) > free_b;
         if (const char *e = std::getenv("PDMK_PW_LEVELS")) pw_by_level = e[0] == '1';
+        if (const char *e = std::getenv("PDMK_T3_MFMA")) t3_mfma = e[0] != '0';
         if (pw_by_level) {
             pw_out.alloc(std::max(pw_slot * (size_t)lvl_cnt, c2p_bytes));
         } else {
@@ -1018,8 +1019,22 @@ template <class Real> ChainBufs<Real> own_chain(Plan<Real> &pl) {
 // boxes still spreads over the whole GPU (about 1024 workgroups or more); nv vectors along z
 template <class Real, int P, bool TRANSPOSE>
 void launch_tensor3(const Real *in, Real *out, typename Plan<Real>::Tensor3Launch &L, const Real *mats_rm,
-                    hipStream_t s, int div, int nv = 1, long long vs_in = 0, long long vs_out = 0) {
+                    hipStream_t s, int div, bool mfma, int nv = 1, long long vs_in = 0, long long vs_out = 0) {
     if (L.n_items == 0) return;
+    // FP64 at p = 28 on the matrix cores (clustered N = 4e6, 9 digits: child -> parent 22.3 ->
+    // 21.1 ms, parent -> child 12.6 -> 10.5 ms per evaluate).  At p = 38 the matrix-core kernel's
+    // a-slices of the input re-read each box several times and ran 1.8x slower (168.8 against
+    // 94.4 ms); p = 38 stays on the vector ALUs.
+    if constexpr (std::is_same_v<Real, double> && P == 28) {
+        if (mfma) {
+            hipLaunchKernelGGL((k_tensor3_mfma<P, TRANSPOSE>), dim3(L.n_items * T3MfmaCfg<P>::CT, 1, nv), dim3(64 * T3MfmaCfg<P>::NW), 0, s,
+                               in, out, L.out.template as<int>(), L.add.template as<int>(), L.ptr.template as<int>(),
+                               L.in_slot.template as<int>(), L.in_code.template as<int>(), mats_rm, L.n_items, vs_in,
+                               vs_out);
+            check(hipGetLastError(), "k_tensor3_mfma");
+            return;
+        }
+    }
     constexpr int KC = T3Cfg<Real, P>::KC;
     int kc = std::max(1, KC / std::max(1, div));
     while (kc > 1 && (long)L.n_items * ((P + kc - 1) / kc) < 1024) kc = (kc + 1) / 2;
@@ -1059,8 +1074,8 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
         }
     }
     for (auto &C : pl.c2p_levels) {
-        launch_tensor3<Real, P, true>(B.up, B.c2p, C.children, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, nv,
-                                      B.s_up, B.s_c2p);
+        launch_tensor3<Real, P, true>(B.up, B.c2p, C.children, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, pl.t3_mfma,
+                                      nv, B.s_up, B.s_c2p);
         hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256, nv), dim3(256), 0, s,
                            B.up, B.c2p, C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents,
                            B.s_up, B.s_c2p);
@@ -1149,8 +1164,8 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
             pw2proxy(E, i0, nb);
         }
         auto &L = pl.p2c_levels[l];
-        launch_tensor3<Real, P, false>(B.down, B.down, L, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, nv,
-                                       B.s_down, B.s_down);
+        launch_tensor3<Real, P, false>(B.down, B.down, L, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div,
+                                       pl.t3_mfma, nv, B.s_down, B.s_down);
     }
     if (pl.n_leaf_eval > 0) {
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {

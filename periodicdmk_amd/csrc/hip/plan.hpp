@@ -167,6 +167,7 @@ struct Plan {
     // the level's shifts); c2p_scratch then shares pw_out's memory (the upward pass is done
     // before the first plane wave is formed, on the same stream).
     bool pw_by_level = false;
+    bool t3_mfma = true;  // FP64 tensor transforms at p = 28 on the matrix cores (PDMK_T3_MFMA=0: vector ALUs)
     // evaluate() on the caller's stream only (the stages one after another): for callers that
     // run many trees side by side, where the trees, not the stages, fill the GPU
     bool single_stream = false;

@@ -132,3 +132,27 @@ def test_two_pass_transforms_identical(pdmk, cuda, monkeypatch, digits, n, npl):
             s.close()
     np.testing.assert_array_equal(out[None], out["0"])
     np.testing.assert_array_equal(out["1"], out["0"])
+
+
+@pytest.mark.parametrize("digits,n,npl", [(9, 60_000, 40)])
+def test_tensor3_matrix_cores_identical(pdmk, cuda, monkeypatch, digits, n, npl):
+    """FP64 at 9 digits (p = 28) runs the child <-> parent tensor transforms on the matrix
+    cores, each output summed in the order of the vector-ALU kernel: the potentials must equal
+    those with the vector-ALU kernel (PDMK_T3_MFMA=0), on a clustered triclinic tree whose
+    levels have parents with up to 8 children."""
+    cell = W.triclinic_cell()
+    pos = W.gaussian_mixture(n, 8, 0.03, 7, cell)
+    q = W.neutral_uniform_charges(n, 44)
+    out = {}
+    for mode in ("0", None):
+        if mode is None:
+            monkeypatch.delenv("PDMK_T3_MFMA", raising=False)
+        else:
+            monkeypatch.setenv("PDMK_T3_MFMA", mode)
+        s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=npl, nz=4, precision="double")
+        monkeypatch.delenv("PDMK_T3_MFMA", raising=False)
+        try:
+            out[mode] = s.evaluate(q)
+        finally:
+            s.close()
+    np.testing.assert_array_equal(out[None], out["0"])
