@@ -950,8 +950,8 @@ __global__ __launch_bounds__(256) void k_p2pw_yx(const Cx<double> *__restrict__ 
 // The x step of whole boxes into HBM, for orders where the slices of k_pw2proxy would each
 // re-read the box's incoming expansion (14 slices per box for FP64 at p = 28, 38 at p = 38):
 // u1[it][a][k3][k2] = sum_k1 V(k1, a) pin[it][k1][k3][k2] for every a, one thread per column
-// (k3, k2), the same sums as k_pw2proxy's x step.  k_pw2proxy<..., true> then reads its
-// slice of u1.
+// (k3, k2), the same sums as k_pw2proxy's x step.  k_pw2p_yz then takes each plane a of u1
+// through the y and z steps.
 template <class R, int P, int NPW>
 __global__ __launch_bounds__(256) void k_pw2p_x(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                 int n_items, Cx<R> *__restrict__ u1) {
@@ -983,13 +983,132 @@ __global__ __launch_bounds__(256) void k_pw2p_x(const Cx<R> *__restrict__ pin, c
     for (int a = 0; a < P; ++a) out[(size_t)a * COL] = acc[a];
 }
 
-// U1G: the x step comes from k_pw2p_x's u1 (box it at u1 + it p H n) instead of pin.
-template <class R, int P, int NPW, int BK = 0, bool U1G = false>
+// Second pass of the two-pass plane-wave -> proxy transform (after k_pw2p_x), FP64 on the
+// matrix cores: the y and z steps of one plane a per wavefront.  With G[k][j] the folded
+// table (the non-zero part of V(k, j): real for even j, imaginary for odd j):
+//   y:  R[k3][b] = sum_k2 u1.re[a][k3][k2] G[k2][b],  I[k3][b] = the same of u1.im;
+//       U2[k3][b] = h(k3) (R, I) for even b, h(k3) (-I, R) for odd b
+//   z:  Zr[c][b] = sum_k3 G[k3][c] U2.re[k3][b],  Zi[c][b] = the same of U2.im;
+//       down[c][b][a] (+)= 2 Zr for even c, 2 (-Zi) for odd c
+// (the parity of the output index picks the part, so each step forms both products and
+// keeps one; a negated sum of fused multiply-adds is the sum of the negated terms, bit for
+// bit).  y's output tile, lane l holding rows k3 = 4 q + l/16 of column b = l%16, is z's B
+// operand for the 4-steps q of that tile, so U2 stays in registers.  LDS holds G transposed
+// (Gt[k][j], zero-padded).  Every output sums its terms in the order of k_pw2proxy (k2, then
+// k3 ascending).
+template <int P, int NPW> struct Pw2pMfmaCfg {
+    static constexpr int H = (NPW + 1) / 2;
+    static constexpr int KS2 = (NPW + 3) / 4;     // y's 4-steps over k2
+    static constexpr int MT = (H + 15) / 16;      // y's row tiles over k3
+    static constexpr int KSZ = (H + 3) / 4;       // z's 4-steps over k3
+    static constexpr int BT = (P + 15) / 16;      // column tiles over b, row tiles over c
+    static constexpr int GR = 4 * (KS2 > 4 * MT ? KS2 : 4 * MT);  // rows of Gt
+    static constexpr int LG = 16 * BT + 8;        // row stride of Gt (8 mod 16)
+};
+
+template <int P, int NPW>
+__global__ __launch_bounds__(256) void k_pw2p_yz(const Cx<double> *__restrict__ u1, const double *__restrict__ F,
+                                                 const int *__restrict__ box_down, const int *__restrict__ box_zero,
+                                                 int item0, int n_items, double *__restrict__ down_base) {
+    chain_priority();
+    using Cfg = Pw2pMfmaCfg<P, NPW>;
+    using M = Mfma16<double>;
+    typedef M::acc_t acc_t;
+    constexpr int H = Cfg::H, COL = H * NPW, PP = P * P, KS2 = Cfg::KS2, MT = Cfg::MT, KSZ = Cfg::KSZ, BT = Cfg::BT,
+                  GR = Cfg::GR, LG = Cfg::LG, AG = (P + 3) / 4;  // 4 planes a per workgroup
+    static_assert(NPW <= 64 && 16 * BT <= 64, "table rows of 64");
+    __shared__ double Gt[GR * LG];  // Gt[k][j] = G[k][j] for k < n, j < p, else 0
+    for (int i = threadIdx.x; i < GR * LG; i += 256) {
+        const int k = i / LG, j = i - k * LG;
+        Gt[i] = (k < NPW && j < P) ? F[j * 64 + k] : 0.0;
+    }
+    __syncthreads();
+    const int lg = xcd_block(blockIdx.x, gridDim.x), it = lg / AG;
+    const int a = (lg - it * AG) * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    if (it >= n_items || a >= P) return;
+    const int l = threadIdx.x & 63, kk = l >> 4, jj = l & 15;
+    const Cx<double> *X = u1 + ((size_t)it * P + a) * COL;  // X[k3][k2]
+    // y: R, I tiles (k3 tile mt, b tile nt)
+    acc_t yr[MT][BT], yi[MT][BT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < BT; ++nt) yr[mt][nt] = yi[mt][nt] = acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < KS2; ++ks) {
+        const int k2 = 4 * ks + kk;
+        double g[BT];
+#pragma unroll
+        for (int nt = 0; nt < BT; ++nt) g[nt] = Gt[k2 * LG + 16 * nt + jj];  // B[k2][b]
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int k3 = 16 * mt + jj;
+            Cx<double> x = {0.0, 0.0};
+            if (k3 < H && k2 < NPW) x = X[k3 * NPW + k2];  // A[k3][k2]
+#pragma unroll
+            for (int nt = 0; nt < BT; ++nt) {
+                yr[mt][nt] = M::mma(x.re, g[nt], yr[mt][nt]);
+                yi[mt][nt] = M::mma(x.im, g[nt], yi[mt][nt]);
+            }
+        }
+    }
+    // U2 in place: element q of tile (mt, nt) is k3 = 16 mt + 4 q + kk, b = 16 nt + jj
+    const bool bodd = jj & 1;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double hw = (16 * mt + M::row(kk, q) >= NPW / 2) ? 0.5 : 1.0;
+#pragma unroll
+            for (int nt = 0; nt < BT; ++nt) {
+                const double r = yr[mt][nt][q], i = yi[mt][nt][q];
+                yr[mt][nt][q] = (bodd ? -i : r) * hw;
+                yi[mt][nt][q] = (bodd ? r : i) * hw;
+            }
+        }
+    // z: c tile ct, b tile nt; 4-step ks = 4 mt + q over k3
+    double *down = down_base + (size_t)box_down[item0 + it] * P * PP;
+    const bool zero = box_zero[item0 + it] != 0;
+    const bool codd = kk & 1;  // parity of c = 16 ct + 4 q + kk
+#pragma unroll
+    for (int ct = 0; ct < BT; ++ct) {
+        acc_t zr[BT], zi[BT];
+#pragma unroll
+        for (int nt = 0; nt < BT; ++nt) zr[nt] = zi[nt] = acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (4 * mt + q < KSZ) {
+                    const double gz = Gt[(16 * mt + 4 * q + kk) * LG + 16 * ct + jj];  // A[c][k3] = G[k3][c]
+#pragma unroll
+                    for (int nt = 0; nt < BT; ++nt) {
+                        zr[nt] = M::mma(gz, yr[mt][nt][q], zr[nt]);
+                        zi[nt] = M::mma(gz, yi[mt][nt][q], zi[nt]);
+                    }
+                }
+            }
+#pragma unroll
+        for (int nt = 0; nt < BT; ++nt) {
+            const int b = 16 * nt + jj;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = 16 * ct + M::row(kk, q);
+                if (c < P && b < P) {
+                    double *dst = down + ((size_t)c * P + b) * P + a;
+                    const double v = codd ? -zi[nt][q] : zr[nt][q];
+                    *dst = (zero ? 0.0 : *dst) + 2.0 * v;
+                }
+            }
+        }
+    }
+}
+
+template <class R, int P, int NPW, int BK = 0>
 __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin, const Cx<R> *__restrict__ V,
                                                    const int *__restrict__ box_down, const int *__restrict__ box_zero,
                                                    int item0, int n_items, R *__restrict__ down_base,
-                                                   long long vs_in = 0, long long vs_out = 0,
-                                                   const Cx<R> *__restrict__ u1g = nullptr) {
+                                                   long long vs_in = 0, long long vs_out = 0) {
     chain_priority();
     pin += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
     down_base += (long long)blockIdx.z * vs_out;
@@ -1042,10 +1161,7 @@ __global__ __launch_bounds__(256) void k_pw2proxy(const Cx<R> *__restrict__ pin,
                     if (a < na) U1[a * COL + col] = acc[a];
             }
         };
-        if constexpr (U1G) {
-            const Cx<R> *src = u1g + ((size_t)it * P + a0) * COL;
-            for (int j = threadIdx.x; j < na * COL; j += blockDim.x) U1[j] = src[j];
-        } else if (a0 & 1) {
+        if (a0 & 1) {
             xstep(Odd{});
         } else {
             xstep(Even{});

@@ -1125,11 +1125,10 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
                     hipLaunchKernelGGL((k_pw2p_x<Real, P, NPW>), dim3(m * ((COL + 255) / 256)), dim3(256), 0, s,
                                        B.pin + (size_t)c0 * NM, pl.pw2poly.template as<C>(), m, u1);
                     check(hipGetLastError(), "k_pw2p_x");
-                    hipLaunchKernelGGL((k_pw2proxy<Real, P, NPW, BK, true>), dim3(m * Pw2pCfg<Real, P, NPW, BK>::slices),
-                                       dim3(256), 0, s, B.pin + (size_t)c0 * NM, pl.pw2poly.template as<C>(),
-                                       E.down.template as<int>(), E.zero.template as<int>(), i0 + c0, m, B.down, 0LL, 0LL,
-                                       (const C *)u1);
-                    check(hipGetLastError(), "k_pw2proxy");
+                    hipLaunchKernelGGL((k_pw2p_yz<P, NPW>), dim3(m * ((P + 3) / 4)), dim3(256), 0, s, (const Cx<double> *)u1,
+                                       pl.pw2poly_f.template as<double>(), E.down.template as<int>(),
+                                       E.zero.template as<int>(), i0 + c0, m, (double *)B.down);
+                    check(hipGetLastError(), "k_pw2p_yz");
                 }
                 return;
             }
