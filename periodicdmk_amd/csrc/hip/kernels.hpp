@@ -208,12 +208,14 @@ __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
     }
 }
 
-// The same proxy charges on the FP64 matrix cores, for 28 <= P <= 32.  Per destination box
+// The same proxy charges on the FP64 matrix cores, for p >= 28.  Per destination box
 //   C[(c, b), a] = sum_s A[(c, b), s] B[s, a],  A = q_s T_c(z_s) T_b(y_s),  B = T_a(x_s),
-// is a GEMM of P^2 rows by 32 columns (a padded) over the box's sources, computed with
+// is a GEMM of P^2 rows by PA columns (a padded to 16s) over the box's sources, computed with
 // v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
-// D[l / 16 + 4 r][l % 16] in accumulator element r (layout read off gfx950 hardware).  Wave w of the workgroup owns the row
-// tiles w, w + NW, ...  and both column tiles.
+// D[l / 16 + 4 r][l % 16] in accumulator element r (layout read off gfx950 hardware).  The
+// row tiles of a box are split over RS workgroups (two at p = 38, whose accumulators would
+// not fit one workgroup's registers); wave w of a workgroup owns its row tiles w, w + NW, ...
+// and every column tile.
 typedef double c2p_d4 __attribute__((ext_vector_type(4)));
 typedef float c2p_f4 __attribute__((ext_vector_type(4)));
 
@@ -237,10 +239,13 @@ template <> struct Mfma16<float> {
 };
 template <int P> struct C2pMfmaCfg {
     static constexpr int RT = (P * P + 15) / 16;      // row tiles
+    static constexpr int RS = P >= 38 ? 2 : 1;        // workgroups per box
+    static constexpr int RTS = (RT + RS - 1) / RS;    // row tiles per workgroup
     static constexpr int NW = 8;                      // waves per workgroup (two per SIMD)
-    static constexpr int TPW = (RT + NW - 1) / NW;    // row tiles per wave
+    static constexpr int TPW = (RTS + NW - 1) / NW;   // row tiles per wave
     static constexpr int TS = 64;                     // sources per LDS table
-    static constexpr int PA = 32;                     // a, padded
+    static constexpr int PA = (P + 15) / 16 * 16;     // a, padded
+    static constexpr int CT = PA / 16;                // column tiles
 };
 
 template <class R, int P>
@@ -251,10 +256,11 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
     chain_priority();
     using C = C2pMfmaCfg<P>;
     using M = Mfma16<R>;
-    constexpr int TS = C::TS, PA = C::PA, RT = C::RT, NW = C::NW, TPW = C::TPW;
+    constexpr int TS = C::TS, PA = C::PA, RT = C::RT, RS = C::RS, RTS = C::RTS, NW = C::NW, TPW = C::TPW, CT = C::CT;
     static_assert(P <= PA, "order above the padded column count");
     __shared__ R Tx[TS * PA], Ty[TS * P], Tz[TS * P];
-    const int g = blockIdx.x, lane = threadIdx.x & 63;
+    const int g = blockIdx.x / RS, part = blockIdx.x - g * RS, lane = threadIdx.x & 63;
+    const int rt0 = part * RTS, rt1 = min(RT, rt0 + RTS);  // this workgroup's row tiles
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, jj = lane & 15;
     const R sc = grp_geom[4 * g + 3];
@@ -262,15 +268,17 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
     R rv[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-        const int row = (w + NW * t) * 16 + jj;
+        const int row = (rt0 + w + NW * t) * 16 + jj;
         const bool ok = row < P * P;
         rv[t] = ok ? R(1) : R(0);
         ci[t] = ok ? row / P : 0;
         bi[t] = ok ? row % P : 0;
     }
-    typename M::acc_t acc[TPW][2];
+    typename M::acc_t acc[TPW][CT];
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) acc[t][0] = acc[t][1] = typename M::acc_t{R(0), R(0), R(0), R(0)};
+    for (int t = 0; t < TPW; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[t][ct] = typename M::acc_t{R(0), R(0), R(0), R(0)};
     for (int r = grp_rng[g]; r < grp_rng[g + 1]; ++r) {
         const int beg = rng_begin[r], cnt = rng_count[r];
         for (int s0 = 0; s0 < cnt; s0 += TS) {
@@ -301,13 +309,15 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
             __syncthreads();
             for (int k0 = 0; k0 < ns; k0 += 4) {
                 const int s = k0 + kk;
-                const R b0 = Tx[s * PA + jj], b1 = Tx[s * PA + 16 + jj];
+                R bv[CT];
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) bv[ct] = Tx[s * PA + 16 * ct + jj];
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
-                    if (w + NW * t < RT) {
+                    if (rt0 + w + NW * t < rt1) {
                         const R a = Tz[s * P + ci[t]] * Ty[s * P + bi[t]] * rv[t];
-                        acc[t][0] = M::mma(a, b0, acc[t][0]);
-                        acc[t][1] = M::mma(a, b1, acc[t][1]);
+#pragma unroll
+                        for (int ct = 0; ct < CT; ++ct) acc[t][ct] = M::mma(a, bv[ct], acc[t][ct]);
                     }
                 }
             }
@@ -316,13 +326,13 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
     R *out = proxy + (size_t)grp_slot[g] * (P * P * P);
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
-        if (w + NW * t >= RT) continue;
+        if (rt0 + w + NW * t >= rt1) continue;
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < CT; ++ct) {
             const int a = ct * 16 + jj;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int row = (w + NW * t) * 16 + M::row(kk, e);
+                const int row = (rt0 + w + NW * t) * 16 + M::row(kk, e);
                 if (row < P * P && a < P) out[(size_t)row * P + a] = acc[t][ct][e];
             }
         }
@@ -1332,30 +1342,33 @@ __global__ __launch_bounds__(EV_THREADS) void k_eval_targets(const R *__restrict
     }
 }
 
-// The same evaluation on the FP64 matrix cores, for 28 <= P <= 32.  Per leaf and stage of
-// 64 targets
-//   R[t, (c, b)] = sum_a T_a(x_t) C[(c, b), a]          (GEMM, a padded to 32)
+// The same evaluation on the FP64 matrix cores, for p >= 28.  Per leaf and stage of
+// targets
+//   R[t, (c, b)] = sum_a T_a(x_t) C[(c, b), a]          (GEMM, a padded to 16s)
 //   pot_t += sum_(c,b) R[t, (c, b)] T_b(y_t) T_c(z_t)
 // with v_mfma_f64_16x16x4f64 (operand layout as in k_charge2proxy_mfma: A[l % 16][l / 16],
 // B[l / 16][l % 16], D[l / 16 + 4 r][l % 16]).  Wave w takes the (c, b) tiles w, w + NW, ...,
 // whose B operands stay in registers for the whole leaf; each stage runs the TT target tiles
 // as independent accumulators.  Lane partials are reduced over each 16-lane row, then over
 // the waves in a fixed order.  (Reloading the B operands per stage of 64 targets read each
-// leaf's proxy 4 times: 1846 us at N=1e6, 9 digits.)
+// leaf's proxy 4 times: 1846 us at N=1e6, 9 digits.)  At p = 38 a wave's B operands would not
+// fit its registers: the leaf's (c, b) tiles are taken in RH = 2 halves, each half a pass over
+// the leaf's targets that adds its part of the sum to the potentials.
 template <int P> struct EvMfmaCfg {
     static constexpr int RT = (P * P + 15) / 16;
     static constexpr int NW = 8;
-    static constexpr int MT = (RT + NW - 1) / NW;  // (c, b) tiles per wave
+    static constexpr int RH = P >= 38 ? 2 : 1;              // passes over the leaf's targets
+    static constexpr int MT = (RT + NW * RH - 1) / (NW * RH);  // (c, b) tiles per wave and pass
     static constexpr int TT = 2;                   // target tiles per stage
     static constexpr int NT = 16 * TT;             // targets per stage
-    static constexpr int PA = 32;
+    static constexpr int PA = (P + 15) / 16 * 16;
     // LDS row strides, in elements: the A reads (16 targets x 2 a per half wavefront) at a
     // stride of 34 and the T_b / T_c reads (2 targets x 16 (c, b) columns) at 48 fall on
     // distinct banks; the stride of 32 put the 16 targets of an A read on the same banks
     // (measured at N=1e6, 9 digits: 3.3x fewer LDS bank-conflict cycles, 20% fewer busy
     // cycles; an XOR-swizzled layout removed more conflicts but its address arithmetic cost
     // more than it saved)
-    static constexpr int LX = 34, LY = 48;
+    static constexpr int LX = PA + 2, LY = 48;
     static_assert(P <= LY, "order above the table stride");
 };
 // One workgroup per leaf.  Wave w owns the (c, b) tiles w, w + NW, ...: their B operands
@@ -1367,7 +1380,7 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
     const int *__restrict__ leaf_trg, const P4<double> *__restrict__ trg, double *__restrict__ pot_sorted) {
     chain_priority();
     using C = EvMfmaCfg<P>;
-    constexpr int RT = C::RT, NW = C::NW, MT = C::MT, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
+    constexpr int RT = C::RT, NW = C::NW, RH = C::RH, MT = C::MT, TT = C::TT, NT = C::NT, PA = C::PA, PP = P * P;
     static_assert(P <= PA, "order above the padded a count");
     constexpr int LX = C::LX, LY = C::LY;
     __shared__ double Tx[NT * LX], Ty[NT * LY], Tz[NT * LY], red[NW * NT];
@@ -1377,90 +1390,92 @@ __global__ __launch_bounds__(64 * EvMfmaCfg<P>::NW) void k_eval_targets_mfma(
     const double *Cm = down_base + (size_t)leaf_down[lf] * P * PP;
     const double cx = leaf_geom[4 * lf], cy = leaf_geom[4 * lf + 1], cz = leaf_geom[4 * lf + 2], sc = leaf_geom[4 * lf + 3];
     const int beg = leaf_trg[2 * lf], cnt = leaf_trg[2 * lf + 1];
-    // B operands of tile rt = w + NW m: lane (kk, jj) supplies C[rt 16 + jj][4 ks + kk]
-    double B[MT][PA / 4];
-#pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        const int row = (w + NW * m) * 16 + jj;
-#pragma unroll
-        for (int ks = 0; ks < PA / 4; ++ks) {
-            const int a = 4 * ks + kk;
-            B[m][ks] = (row < PP && a < P) ? Cm[(size_t)row * P + a] : 0.0;
-        }
-    }
-    for (int t0 = 0; t0 < cnt; t0 += NT) {
-        __syncthreads();
-        for (int task = threadIdx.x; task < 3 * NT; task += blockDim.x) {
-            const int t = task % NT, d = task / NT;
-            double *T = d == 0 ? Tx + t * LX : (d == 1 ? Ty + t * LY : Tz + t * LY);
-            const int len = d == 0 ? PA : P;
-            if (t0 + t < cnt) {
-                const P4<double> pt = trg[beg + t0 + t];
-                const double x = sc * (d == 0 ? pt.x - cx : (d == 1 ? pt.y - cy : pt.z - cz));
-                double ta = 1.0, tb = x;
-                T[0] = 1.0;
-                T[1] = x;
-                for (int k = 2; k < P; ++k) {
-                    const double tn = 2.0 * x * tb - ta;
-                    T[k] = tn;
-                    ta = tb;
-                    tb = tn;
-                }
-                for (int k = P; k < len; ++k) T[k] = 0.0;
-            } else {
-                for (int k = 0; k < len; ++k) T[k] = 0.0;
-            }
-        }
-        __syncthreads();
-        double A[TT][PA / 4];
-#pragma unroll
-        for (int tt = 0; tt < TT; ++tt)
-#pragma unroll
-            for (int ks = 0; ks < PA / 4; ++ks) A[tt][ks] = Tx[(tt * 16 + jj) * LX + 4 * ks + kk];
-        double part[TT][4];
-#pragma unroll
-        for (int tt = 0; tt < TT; ++tt)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) part[tt][e] = 0.0;
+    for (int h = 0; h < RH; ++h) {
+        // B operands of tile rt = w + NW (h MT + m): lane (kk, jj) supplies C[rt 16 + jj][4 ks + kk]
+        double B[MT][PA / 4];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
-            const int rt = w + NW * m;
-            if (rt >= RT) break;  // wave-uniform
-            const int row = rt * 16 + jj;  // (c, b) of the B column this lane supplies
-            c2p_d4 D[TT];
+            const int row = (w + NW * (h * MT + m)) * 16 + jj;
 #pragma unroll
-            for (int tt = 0; tt < TT; ++tt) D[tt] = c2p_d4{0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < PA / 4; ++ks) {
+                const int a = 4 * ks + kk;
+                B[m][ks] = (row < PP && a < P) ? Cm[(size_t)row * P + a] : 0.0;
+            }
+        }
+        for (int t0 = 0; t0 < cnt; t0 += NT) {
+            __syncthreads();
+            for (int task = threadIdx.x; task < 3 * NT; task += blockDim.x) {
+                const int t = task % NT, d = task / NT;
+                double *T = d == 0 ? Tx + t * LX : (d == 1 ? Ty + t * LY : Tz + t * LY);
+                const int len = d == 0 ? PA : P;
+                if (t0 + t < cnt) {
+                    const P4<double> pt = trg[beg + t0 + t];
+                    const double x = sc * (d == 0 ? pt.x - cx : (d == 1 ? pt.y - cy : pt.z - cz));
+                    double ta = 1.0, tb = x;
+                    T[0] = 1.0;
+                    T[1] = x;
+                    for (int k = 2; k < P; ++k) {
+                        const double tn = 2.0 * x * tb - ta;
+                        T[k] = tn;
+                        ta = tb;
+                        tb = tn;
+                    }
+                    for (int k = P; k < len; ++k) T[k] = 0.0;
+                } else {
+                    for (int k = 0; k < len; ++k) T[k] = 0.0;
+                }
+            }
+            __syncthreads();
+            double A[TT][PA / 4];
 #pragma unroll
-            for (int ks = 0; ks < PA / 4; ++ks)
+            for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                for (int ks = 0; ks < PA / 4; ++ks) A[tt][ks] = Tx[(tt * 16 + jj) * LX + 4 * ks + kk];
+            double part[TT][4];
+#pragma unroll
+            for (int tt = 0; tt < TT; ++tt)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) part[tt][e] = 0.0;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                const int rt = w + NW * (h * MT + m);
+                if (rt >= RT) break;  // wave-uniform
+                const int row = rt * 16 + jj;  // (c, b) of the B column this lane supplies
+                c2p_d4 D[TT];
+#pragma unroll
+                for (int tt = 0; tt < TT; ++tt) D[tt] = c2p_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < PA / 4; ++ks)
+#pragma unroll
+                    for (int tt = 0; tt < TT; ++tt)
+                        D[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[tt][ks], B[m][ks], D[tt], 0, 0, 0);
+                // D[tt][e] = R[target tt 16 + kk + 4 e, (c, b) = rt 16 + jj]
+                const int cb = row < PP ? row : 0, c = cb / P, b = cb % P;
 #pragma unroll
                 for (int tt = 0; tt < TT; ++tt)
-                    D[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[tt][ks], B[m][ks], D[tt], 0, 0, 0);
-            // D[tt][e] = R[target tt 16 + kk + 4 e, (c, b) = rt 16 + jj]
-            const int cb = row < PP ? row : 0, c = cb / P, b = cb % P;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int t = tt * 16 + kk + 4 * e;
+                        part[tt][e] = fma(D[tt][e], Ty[t * LY + b] * Tz[t * LY + c], part[tt][e]);
+                    }
+            }
+            // sum over the 16 lanes of each row (the (c, b) columns), then over the waves
 #pragma unroll
             for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int t = tt * 16 + kk + 4 * e;
-                    part[tt][e] = fma(D[tt][e], Ty[t * LY + b] * Tz[t * LY + c], part[tt][e]);
+                    double v = part[tt][e];
+#pragma unroll
+                    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
+                    if (jj == 0) red[w * NT + tt * 16 + kk + 4 * e] = v;
                 }
-        }
-        // sum over the 16 lanes of each row (the (c, b) columns), then over the waves
-#pragma unroll
-        for (int tt = 0; tt < TT; ++tt)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                double v = part[tt][e];
-#pragma unroll
-                for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, 16);
-                if (jj == 0) red[w * NT + tt * 16 + kk + 4 * e] = v;
-            }
-        __syncthreads();
-        for (int t = threadIdx.x; t < NT; t += blockDim.x) {
-            if (t0 + t < cnt) {
-                double v = 0;
-                for (int q = 0; q < NW; ++q) v += red[q * NT + t];
-                pot_sorted[beg + t0 + t] += v;
+            __syncthreads();
+            for (int t = threadIdx.x; t < NT; t += blockDim.x) {
+                if (t0 + t < cnt) {
+                    double v = 0;
+                    for (int q = 0; q < NW; ++q) v += red[q * NT + t];
+                    pot_sorted[beg + t0 + t] += v;
+                }
             }
         }
     }

@@ -1053,14 +1053,17 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
     const int nv = B.nv;
     if (nv < 1 || nv > 65535) throw std::invalid_argument("launch_transforms: vector count");
     if (pl.n_up > 0) {
-        // FP64 at P = 28 (9 digits): matrix cores (a padded to 32 costs 1/8).  FP64 at P = 18
-        // pads too much.  FP32 at P = 18 on the matrix cores is faster alone (395 -> 307 us at
+        // FP64 at P = 28 (9 digits): matrix cores (a padded to 32 costs 1/8).  At P = 38 (a
+        // padded to 48, each box's row tiles split over two workgroups) the matrix-core kernel
+        // ran slower than the vector ALUs (clustered N = 4e6: 90.5 against 78.7 ms).  FP64 at
+        // P = 18 pads too much.  FP32 at P = 18 on the matrix cores is faster alone (395 -> 307 us at
         // N=1e6) but slower inside evaluate() (3.02 -> 3.24 ms): its 8-wave workgroups take
         // wave slots from the near field beside it; it stays on the vector ALUs (and the
         // many-vector chain with it: the same sums in the same order as one vector's).
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
             if (nv != 1 || B.qv) throw std::logic_error("launch_transforms: one vector on the matrix-core kernels");
-            hipLaunchKernelGGL((k_charge2proxy_mfma<Real, P>), dim3(pl.n_up), dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
+            hipLaunchKernelGGL((k_charge2proxy_mfma<Real, P>), dim3(pl.n_up * C2pMfmaCfg<P>::RS),
+                               dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
                                pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
                                pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
                                pl.rng_off.template as<Real>(), B.up);
@@ -1167,7 +1170,7 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
                                        pl.t3_mfma, nv, B.s_down, B.s_down);
     }
     if (pl.n_leaf_eval > 0) {
-        if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
+        if constexpr (std::is_same_v<Real, double> && P >= 28) {
             hipLaunchKernelGGL((k_eval_targets_mfma<P>), dim3(pl.n_leaf_eval), dim3(64 * EvMfmaCfg<P>::NW), 0, s, B.down,
                                pl.leaf_down.template as<int>(), pl.leaf_geom.template as<Real>(),
                                pl.leaf_trg.template as<int>(), pl.trg.template as<P4<Real>>(), B.pot);
