@@ -880,7 +880,7 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
 // read once per wavefront into registers.  Each output sums its terms in the same order as
 // k_proxy2pw (b, then a, ascending).
 template <int P, int NPW>
-__global__ __launch_bounds__(256) void k_p2pw_yx(const Cx<double> *__restrict__ t1, const double *__restrict__ F,
+__global__ __launch_bounds__(256, 2) void k_p2pw_yx(const Cx<double> *__restrict__ t1, const double *__restrict__ F,
                                                  Cx<double> *__restrict__ pw_out, int n_boxes) {
     chain_priority();
     static_assert(NPW <= 64, "k2 and k1 within a table row of 64");
@@ -910,6 +910,9 @@ __global__ __launch_bounds__(256) void k_p2pw_yx(const Cx<double> *__restrict__ 
             xi[ks][at] = odd ? x.re : x.im;
         }
     Cx<double> *dst = pw_out + (size_t)bx * NPW * COL + (size_t)k * NPW;
+    // one k2 tile at a time (unrolled, the tiles' accumulators would take the registers of a
+    // second wavefront per SIMD at p = 38)
+#pragma unroll 1
     for (int ct = 0; ct < CT; ++ct) {  // k2 tile
         acc_t yr[AT], yi[AT];
 #pragma unroll
