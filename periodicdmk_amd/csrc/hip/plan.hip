@@ -53,6 +53,11 @@ namespace {
 
 constexpr size_t kPinBytes = size_t(512) << 20;  // incoming plane-wave scratch cap
 constexpr size_t kTwoPassBytes = size_t(512) << 20;  // two-pass transform scratch cap
+// child -> parent levels with at least this many parents take one workgroup item per parent
+// (no scratch round trip; clustered N = 4e6: 12 digits 441 -> 418 ms, 9 digits 144 -> 136 ms);
+// below it one item per child keeps the GPU filled (N = 1e6: 64 parents, FP64 6 digits 3.94 ->
+// 4.16 ms with one item per parent)
+constexpr int kC2pFuseParents = 256;
 
 // f(i0, i1) over contiguous chunks of [0, n) on up to 16 host threads (construction work
 // whose chunks write disjoint data)
@@ -415,8 +420,8 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         for (size_t i = 0; i < rcnt.size(); ++i) c2p_src += rcnt[i];
         work.transform_flops += 2.0 * (double)P3 * c2p_src;
     }
-    auto make_t3 = [&](std::vector<int> &out, std::vector<int> &add, std::vector<int> &ptr, std::vector<int> &in,
-                       std::vector<int> &code) {
+    auto make_t3 = [&](const std::vector<int> &out, const std::vector<int> &add, const std::vector<int> &ptr,
+                       const std::vector<int> &in, const std::vector<int> &code) {
         Tensor3Launch L;
         L.n_items = (int)out.size();
         L.out.upload(out);
@@ -427,33 +432,43 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         work.transform_flops += 6.0 * std::pow((double)p, 4) * (double)in.size();
         return L;
     };
-    // child -> parent: every child proxy is transformed on its own into a scratch slot, then
-    // each parent adds its children's slots (in child order) to the proxy it already holds
-    int c2p_max = 0;
+    // child -> parent, per level: with at least kC2pFuseParents parents, one item per parent,
+    // its children (in child order) the inputs, added to the proxy the parent already holds
+    // (every slot's charge -> proxy group writes it); with fewer, every child is an item of its
+    // own (more workgroups) into a scratch slot, and k_sum_children adds each parent's
+    // children's slots in child order (PDMK_C2P_FUSE=<parents> moves the threshold)
+    int c2p_max = 0, fuse_min = kC2pFuseParents;
+    if (const char *e = std::getenv("PDMK_C2P_FUSE")) fuse_min = std::max(1, std::atoi(e));
     for (int l = nlevels - 1; l >= 0; --l) {
-        std::vector<int> out, add, ptr{0}, in, code, pslot, cptr{0};
+        std::vector<int> pslot, cptr{0}, in, code;
         for (int b : t.by_level[l]) {
             if (!t.proxy_from_children[b] || fold[b] >= 0) continue;
-            pslot.push_back(up_of[b]);
             for (int k = 0; k < 8; ++k) {
                 const int ch = t.first_child[b] + k;
                 if (t.pwexp[ch] && t.src_count[ch] > 0) {
-                    out.push_back((int)in.size());
-                    add.push_back(0);
                     in.push_back(up_of[ch]);
                     code.push_back(k);
-                    ptr.push_back((int)in.size());
                 }
             }
+            if ((int)in.size() == cptr.back()) continue;
+            pslot.push_back(up_of[b]);
             cptr.push_back((int)in.size());
         }
         if (pslot.empty()) continue;
         C2pLevel C;
-        C.children = make_t3(out, add, ptr, in, code);
         C.n_parents = (int)pslot.size();
-        C.parent_slot.upload(pslot);
-        C.child_ptr.upload(cptr);
-        c2p_max = std::max(c2p_max, (int)in.size());
+        C.fused = C.n_parents >= fuse_min;
+        if (C.fused) {
+            C.t3 = make_t3(pslot, std::vector<int>(pslot.size(), 1), cptr, in, code);
+        } else {
+            std::vector<int> out(in.size()), add(in.size(), 0), ptr(in.size() + 1);
+            for (size_t i = 0; i < in.size(); ++i) out[i] = (int)i;
+            for (size_t i = 0; i <= in.size(); ++i) ptr[i] = (int)i;
+            C.t3 = make_t3(out, add, ptr, in, code);
+            C.parent_slot.upload(pslot);
+            C.child_ptr.upload(cptr);
+            c2p_max = std::max(c2p_max, (int)in.size());
+        }
         c2p_levels.push_back(std::move(C));
     }
     {
@@ -1077,7 +1092,12 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
         }
     }
     for (auto &C : pl.c2p_levels) {
-        launch_tensor3<Real, P, true>(B.up, B.c2p, C.children, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, pl.t3_mfma,
+        if (C.fused) {
+            launch_tensor3<Real, P, true>(B.up, B.up, C.t3, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div,
+                                          pl.t3_mfma, nv, B.s_up, B.s_up);
+            continue;
+        }
+        launch_tensor3<Real, P, true>(B.up, B.c2p, C.t3, pl.mats_rm.template as<Real>(), s, pl.knobs.t3_div, pl.t3_mfma,
                                       nv, B.s_up, B.s_c2p);
         hipLaunchKernelGGL((k_sum_children<Real, P>), dim3(C.n_parents, (P * P * P + 255) / 256, nv), dim3(256), 0, s,
                            B.up, B.c2p, C.parent_slot.template as<int>(), C.child_ptr.template as<int>(), C.n_parents,

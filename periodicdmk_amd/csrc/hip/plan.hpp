@@ -157,9 +157,10 @@ struct Plan {
     // upward
     DevBuf up_slot, up_geom, up_rng, rng_begin, rng_count, rng_off, proxy_up;
     struct Tensor3Launch { int n_items; DevBuf out, add, ptr, in_slot, in_code; };
-    // child -> parent per level (deepest first): per-child transforms into c2p_scratch, then
+    // child -> parent per level (deepest first): fused = one item per parent adding its children
+    // to the parent's proxy; else per-child transforms into c2p_scratch, then
     // parent_slot[i] += sum of scratch slots child_ptr[i] .. child_ptr[i+1]
-    struct C2pLevel { Tensor3Launch children; int n_parents = 0; DevBuf parent_slot, child_ptr; };
+    struct C2pLevel { Tensor3Launch t3; bool fused = false; int n_parents = 0; DevBuf parent_slot, child_ptr; };
     std::vector<C2pLevel> c2p_levels;
     DevBuf c2p_scratch;
     // Outgoing plane waves of all up slots (pw_out[slot]), or, when that does not fit in the

@@ -156,3 +156,26 @@ def test_tensor3_matrix_cores_identical(pdmk, cuda, monkeypatch, digits, n, npl)
         finally:
             s.close()
     np.testing.assert_array_equal(out[None], out["0"])
+
+
+@pytest.mark.parametrize("digits,precision", [(6, "float"), (6, "double"), (9, "double"), (12, "double")])
+def test_child_to_parent_per_parent_items(pdmk, cuda, monkeypatch, digits, precision):
+    """Levels with many parents run the child -> parent transform as one item per parent (its
+    children's terms summed in one pass, then added to the parent's own proxy charges); levels
+    with few run one item per child and a separate sum.  Forcing either everywhere
+    (PDMK_C2P_FUSE=1 / a huge threshold) must give the same potentials to rounding: the two
+    differ only in the order of the final additions."""
+    cell = W.triclinic_cell()
+    pos = W.gaussian_mixture(40_000, 8, 0.03, 9, cell)
+    q = W.neutral_uniform_charges(pos.shape[0], 45)
+    out = {}
+    for mode in ("1", "1000000000"):
+        monkeypatch.setenv("PDMK_C2P_FUSE", mode)
+        s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=40, nz=4, precision=precision)
+        monkeypatch.delenv("PDMK_C2P_FUSE")
+        try:
+            out[mode] = s.evaluate(q).astype(np.float64)
+        finally:
+            s.close()
+    tol = 1e-6 if precision == "float" else 1e-13
+    assert rel(out["1"], out["1000000000"]) < tol, rel(out["1"], out["1000000000"])
