@@ -865,6 +865,54 @@ __global__ __launch_bounds__(256) void k_pw_shift(const Cx<R> *__restrict__ pw_o
     pin[(size_t)bi * nm + m] = {sr * lscale, si * lscale};
 }
 
+// The same sum in FP64 with the shift factors formed from their separable parts instead of
+// read from the 27-stencil table (27 n^2 h complex values, 13 MB at p = 28, more than an XCD's
+// L2, so every target box's shifts streamed it from memory beside the neighbours' plane
+// waves): shift0[t][m] = radial0[m] e^{i (x_c dx + x_a dz + x_b dy)} for the mode
+// m = (c h + a) n + b and stencil t = 9 (dz + 1) + 3 (dy + 1) + (dx + 1), x_j = (j - n/2) sphase;
+// ephase[j] = e^{i x_j}, taken conjugated for an offset of -1 and left out for 0.  The radial
+// factor multiplies the sum once.
+__global__ __launch_bounds__(256) void k_pw_shift_sep(const Cx<double> *__restrict__ pw_out,
+                                                      const double *__restrict__ radial0,
+                                                      const Cx<double> *__restrict__ ephase, double lscale,
+                                                      const int *__restrict__ nbr_ptr, const int *__restrict__ nbr_slot,
+                                                      const int *__restrict__ nbr_sten, int slot0, int nm, int item0,
+                                                      int n, int h, Cx<double> *__restrict__ pin, long long vs_in = 0,
+                                                      long long vs_out = 0) {
+    chain_priority();
+    pw_out += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    pin += (long long)blockIdx.z * vs_out;
+    const int bi = xcd_block(blockIdx.x, gridDim.x), it = item0 + bi;
+    const int m = blockIdx.y * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const int b = m % n, a = (m / n) % h, c = m / (n * h);
+    const Cx<double> ex = ephase[c], ez = ephase[a], ey = ephase[b];
+    double sr = 0, si = 0;
+    for (int e = nbr_ptr[it]; e < nbr_ptr[it + 1]; ++e) {
+        const int t = nbr_sten[e], dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;  // uniform
+        double fr = 1, fi = 0;
+        if (dz) {
+            fr = ez.re;
+            fi = dz > 0 ? ez.im : -ez.im;
+        }
+        if (dy) {
+            const double gi = dy > 0 ? ey.im : -ey.im, r = fr * ey.re - fi * gi;
+            fi = fma(fr, gi, fi * ey.re);
+            fr = r;
+        }
+        if (dx) {
+            const double gi = dx > 0 ? ex.im : -ex.im, r = fr * ex.re - fi * gi;
+            fi = fma(fr, gi, fi * ex.re);
+            fr = r;
+        }
+        const Cx<double> av = pw_out[(size_t)(nbr_slot[e] - slot0) * nm + m];
+        sr = fma(av.re, fr, fma(-av.im, fi, sr));
+        si = fma(av.re, fi, fma(av.im, fr, si));
+    }
+    const double rr = radial0[m] * lscale;
+    pin[(size_t)bi * nm + m] = {sr * rr, si * rr};
+}
+
 // Folded matrix table of the plane-wave transforms: F[j * 64 + k] = W(k, j).re for even j,
 // W(k, j).im for odd j (the entry's only non-zero part), 0 for k >= n; k in a row of 64.
 //

@@ -492,7 +492,20 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         for (size_t i = 0; i < s.size(); ++i) s[i] = {(Real)T.shift0[i].real(), (Real)T.shift0[i].imag()};
         poly2pw.upload(a);
         pw2poly.upload(b);
-        shift0.upload(s);
+        if (sizeof(Real) == 8) {
+            // FP64 shifts from the separable parts of the table (k_pw_shift_sep)
+            const int half = n_pw / 2, nf = (n_pw - 1) / 2;
+            const double sphase = 2 * T.beta / nf;
+            std::vector<Cx<double>> eph(n_pw);
+            for (int j = 0; j < n_pw; ++j) {
+                const std::complex<double> z = std::polar(1.0, (j - half) * sphase);
+                eph[j] = {z.real(), z.imag()};
+            }
+            shift_e.upload(eph);
+            shift_r.upload(T.radial0);
+        } else {
+            shift0.upload(s);
+        }
         // folded tables F[j * 64 + k] = the non-zero part of W(k, j) (real for even j,
         // imaginary for odd j), for the two-pass transforms
         if (n_pw <= 64) {
@@ -1178,10 +1191,18 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
         const int slot0 = pl.pw_by_level ? E.pw_lo : 0;
         for (int i0 = 0; i0 < E.n_items; i0 += pl.pin_batch) {
             const int nb = std::min(pl.pin_batch, E.n_items - i0);
-            hipLaunchKernelGGL(k_pw_shift<Real>, dim3(nb, (NM + 255) / 256, nv), dim3(256), 0, s, B.pw,
-                               pl.shift0.template as<C>(), lscale, E.nbr_ptr.template as<int>(),
-                               E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0, B.pin,
-                               B.s_pw, B.s_pin);
+            if constexpr (sizeof(Real) == 8) {
+                hipLaunchKernelGGL(k_pw_shift_sep, dim3(nb, (NM + 255) / 256, nv), dim3(256), 0, s,
+                                   (const Cx<double> *)B.pw, pl.shift_r.template as<double>(),
+                                   pl.shift_e.template as<Cx<double>>(), (double)lscale, E.nbr_ptr.template as<int>(),
+                                   E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0, NPW,
+                                   (NPW + 1) / 2, (Cx<double> *)B.pin, B.s_pw, B.s_pin);
+            } else {
+                hipLaunchKernelGGL(k_pw_shift<Real>, dim3(nb, (NM + 255) / 256, nv), dim3(256), 0, s, B.pw,
+                                   pl.shift0.template as<C>(), lscale, E.nbr_ptr.template as<int>(),
+                                   E.nbr_slot.template as<int>(), E.nbr_sten.template as<int>(), slot0, NM, i0, B.pin,
+                                   B.s_pw, B.s_pin);
+            }
             check(hipGetLastError(), "k_pw_shift");
             pw2proxy(E, i0, nb);
         }

@@ -175,7 +175,8 @@ struct Plan {
     Real *c2p_buffer() { return pw_by_level ? pw_out.template as<Real>() : c2p_scratch.template as<Real>(); }
     DevBuf mats_rm;                         // parent <-> child matrices: [transposed][minus / plus][p][p]
     // plane waves
-    DevBuf pw_out, poly2pw, pw2poly, shift0;
+    DevBuf pw_out, poly2pw, pw2poly, shift0;  // shift0: FP32 plans
+    DevBuf shift_r, shift_e;                   // FP64 plans: radial factor per mode, per-axis phase
     DevBuf poly2pw_f, pw2poly_f;  // folded tables of the two-pass transforms (64 entries a row)
     // pw_lo, pw_cnt: the range of up slots the level's shifts read
     struct EvalLevel { int level, n_items, pw_lo = 0, pw_cnt = 0; DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten; };
