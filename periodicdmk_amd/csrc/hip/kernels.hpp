@@ -246,6 +246,9 @@ template <int P> struct C2pMfmaCfg {
     static constexpr int TS = 64;                     // sources per LDS table
     static constexpr int PA = (P + 15) / 16 * 16;     // a, padded
     static constexpr int CT = PA / 16;                // column tiles
+    // LDS row strides of the tables (odd numbers of doubles: a thread writing its source's row
+    // and the 4 sources of a fragment read fall on different banks; PA and P put them on one)
+    static constexpr int SX = PA + 1, SY = P % 2 ? P : P + 1;
 };
 
 template <class R, int P>
@@ -256,9 +259,10 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
     chain_priority();
     using C = C2pMfmaCfg<P>;
     using M = Mfma16<R>;
-    constexpr int TS = C::TS, PA = C::PA, RT = C::RT, RS = C::RS, RTS = C::RTS, NW = C::NW, TPW = C::TPW, CT = C::CT;
+    constexpr int TS = C::TS, PA = C::PA, RT = C::RT, RS = C::RS, RTS = C::RTS, NW = C::NW, TPW = C::TPW, CT = C::CT,
+                  SX = C::SX, SY = C::SY;
     static_assert(P <= PA, "order above the padded column count");
-    __shared__ R Tx[TS * PA], Ty[TS * P], Tz[TS * P];
+    __shared__ R Tx[TS * SX], Ty[TS * SY], Tz[TS * SY];
     const int g = blockIdx.x / RS, part = blockIdx.x - g * RS, lane = threadIdx.x & 63;
     const int rt0 = part * RTS, rt1 = min(RT, rt0 + RTS);  // this workgroup's row tiles
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -286,7 +290,7 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
             __syncthreads();
             for (int t = threadIdx.x; t < 3 * TS; t += blockDim.x) {
                 const int s = t % TS, d = t / TS;
-                R *T = d == 0 ? Tx + s * PA : (d == 1 ? Ty + s * P : Tz + s * P);
+                R *T = d == 0 ? Tx + s * SX : (d == 1 ? Ty + s * SY : Tz + s * SY);
                 const int len = d == 0 ? PA : P;
                 if (s < ns) {
                     const P4<R> pt = src[beg + s0 + s];
@@ -311,11 +315,11 @@ __global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
                 const int s = k0 + kk;
                 R bv[CT];
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) bv[ct] = Tx[s * PA + 16 * ct + jj];
+                for (int ct = 0; ct < CT; ++ct) bv[ct] = Tx[s * SX + 16 * ct + jj];
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
                     if (rt0 + w + NW * t < rt1) {
-                        const R a = Tz[s * P + ci[t]] * Ty[s * P + bi[t]] * rv[t];
+                        const R a = Tz[s * SY + ci[t]] * Ty[s * SY + bi[t]] * rv[t];
 #pragma unroll
                         for (int ct = 0; ct < CT; ++ct) acc[t][ct] = M::mma(a, bv[ct], acc[t][ct]);
                     }
