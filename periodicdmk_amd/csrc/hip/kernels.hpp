@@ -1009,6 +1009,84 @@ __global__ __launch_bounds__(256, 2) void k_p2pw_yx(const Cx<double> *__restrict
     }
 }
 
+// Shifts of groups of kShiftGroup consecutive target boxes (siblings, mostly): the group's
+// neighbour lists merged by source slot, so each neighbour's plane waves are read once for
+// the whole group (a 2x2x2 group of boxes has 64 distinct neighbours instead of 8 x 27).
+// Entry e of group g: source slot gslot[e], and for target t of the group the stencil in
+// byte t of gsten[e] (255: not a neighbour of t).  The shift factor comes from the
+// 27-stencil table (FP32) or from the separable phases (FP64, as k_pw_shift_sep).  Targets
+// sum their neighbours in slot order.
+constexpr int kShiftGroup = 8;
+template <class R>
+__global__ __launch_bounds__(256) void k_pw_shift_grp(const Cx<R> *__restrict__ pw_out, const Cx<R> *__restrict__ shift0,
+                                                      const double *__restrict__ radial0,
+                                                      const Cx<double> *__restrict__ ephase, R lscale,
+                                                      const int *__restrict__ gptr, const int *__restrict__ gslot,
+                                                      const unsigned long long *__restrict__ gsten, int slot0, int nm,
+                                                      int item0, int n_items, int n, int h, Cx<R> *__restrict__ pin,
+                                                      long long vs_in = 0, long long vs_out = 0) {
+    chain_priority();
+    constexpr int G = kShiftGroup;
+    pw_out += (long long)blockIdx.z * vs_in;  // vector blockIdx.z of a many-vector launch
+    pin += (long long)blockIdx.z * vs_out;
+    // workgroup x = group of the batch (item0 is a multiple of G); y = mode chunk
+    const int g = item0 / G + xcd_block(blockIdx.x, gridDim.x);
+    const int m = blockIdx.y * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    Cx<R> ex{}, ey{}, ez{};
+    if constexpr (sizeof(R) == 8) {
+        const int b = m % n, a = (m / n) % h, c = m / (n * h);
+        ex = ephase[c];
+        ez = ephase[a];
+        ey = ephase[b];
+    }
+    R sr[G], si[G];
+#pragma unroll
+    for (int t = 0; t < G; ++t) sr[t] = si[t] = R(0);
+    for (int e = gptr[g]; e < gptr[g + 1]; ++e) {
+        const unsigned long long st = gsten[e];  // uniform
+        const Cx<R> av = pw_out[(size_t)(gslot[e] - slot0) * nm + m];
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+            const int sten = (int)((st >> (8 * t)) & 255);
+            if (sten == 255) continue;  // uniform
+            R fr, fi;
+            if constexpr (sizeof(R) == 8) {
+                const int dz = sten / 9 - 1, dy = (sten / 3) % 3 - 1, dx = sten % 3 - 1;
+                fr = 1;
+                fi = 0;
+                if (dz) {
+                    fr = ez.re;
+                    fi = dz > 0 ? ez.im : -ez.im;
+                }
+                if (dy) {
+                    const R gi = dy > 0 ? ey.im : -ey.im, r = fr * ey.re - fi * gi;
+                    fi = fma(fr, gi, fi * ey.re);
+                    fr = r;
+                }
+                if (dx) {
+                    const R gi = dx > 0 ? ex.im : -ex.im, r = fr * ex.re - fi * gi;
+                    fi = fma(fr, gi, fi * ex.re);
+                    fr = r;
+                }
+            } else {
+                const Cx<R> f = shift0[(size_t)sten * nm + m];
+                fr = f.re;
+                fi = f.im;
+            }
+            sr[t] = fma(av.re, fr, fma(-av.im, fi, sr[t]));
+            si[t] = fma(av.re, fi, fma(av.im, fr, si[t]));
+        }
+    }
+    R rr = lscale;
+    if constexpr (sizeof(R) == 8) rr = radial0[m] * lscale;
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+        const int it = g * G + t - item0;  // item within the batch
+        if (it < n_items) pin[(size_t)it * nm + m] = {sr[t] * rr, si[t] * rr};
+    }
+}
+
 // down[c][b][a] (+)= 2 Re sum V(k1,a) V(k2,b) h(k3) V(k3,c) pin[k1][k3][k2], V = pw2poly
 // (V(k, j) at V[k + n j]), h = 1/2 for k3 >= n/2 (planewave_to_proxy_potential).  One
 // workgroup per (box, slice of a).

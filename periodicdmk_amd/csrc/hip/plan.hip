@@ -572,6 +572,37 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         E.nbr_ptr.upload(nptr);
         E.nbr_slot.upload(nslot);
         E.nbr_sten.upload(nsten);
+        {
+            // the same lists merged per group of kShiftGroup consecutive items, by source slot
+            std::vector<int> gptr{0}, gslot;
+            std::vector<unsigned long long> gsten;
+            const int ni = (int)down.size();
+            std::vector<std::pair<int, int>> ent;  // (slot, target t * 32 + stencil)
+            for (int g0 = 0; g0 < ni; g0 += kShiftGroup) {
+                ent.clear();
+                for (int t = 0; t < kShiftGroup && g0 + t < ni; ++t)
+                    for (int e = nptr[g0 + t]; e < nptr[g0 + t + 1]; ++e) ent.push_back({nslot[e], t * 32 + nsten[e]});
+                std::stable_sort(ent.begin(), ent.end(),
+                                 [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
+                const size_t first = gslot.size();
+                for (const auto &pe : ent) {
+                    const int t = pe.second / 32, st = pe.second % 32;
+                    // the last entry of this slot, unless target t already has it (a slot twice in
+                    // one target's list: a folded neighbour)
+                    size_t k = gslot.size();
+                    if (k > first && gslot[k - 1] == pe.first && ((gsten[k - 1] >> (8 * t)) & 255) == 255) {
+                        gsten[k - 1] = (gsten[k - 1] & ~(255ull << (8 * t))) | ((unsigned long long)st << (8 * t));
+                    } else {
+                        gslot.push_back(pe.first);
+                        gsten.push_back(~(255ull << (8 * t)) | ((unsigned long long)st << (8 * t)));
+                    }
+                }
+                gptr.push_back((int)gslot.size());
+            }
+            E.grp_ptr.upload(gptr);
+            E.grp_slot.upload(gslot);
+            E.grp_sten.upload(gsten);
+        }
         work.transform_flops += 8.0 * (double)nslot.size() * NM;
         // per box: x and y steps complex against real-or-imaginary entries (two multiply-adds
         // per term), z step to a real output (one)
@@ -915,6 +946,7 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         int items = 1;
         for (auto &E : eval_levels) items = std::max(items, E.n_items);
         pin_batch = std::max(1, std::min(items, (int)(kPinBytes / ((size_t)NM * 2 * sizeof(Real)))));
+        if (pin_batch < items) pin_batch = std::max(kShiftGroup, pin_batch / kShiftGroup * kShiftGroup);
         pin.alloc((size_t)pin_batch * NM * 2 * sizeof(Real));
         // two-pass proxy <-> plane-wave transforms (FP64 at p >= 28, whose slices re-read a
         // box's input kTwoPassSlices times or more): the first contraction of a chunk of boxes
@@ -947,6 +979,12 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         pw_by_level = all_bytes + (size_t(1) << 30) > free_b;
         if (const char *e = std::getenv("PDMK_PW_LEVELS")) pw_by_level = e[0] == '1';
         if (const char *e = std::getenv("PDMK_T3_MFMA")) t3_mfma = e[0] != '0';
+        // grouped shifts at p >= 28 (N=1e6 9 digits: evaluate() 7.57 -> 7.39 ms; clustered
+        // N=4e6 132.5 -> 131.1 ms at 9 digits, 412.5 -> 403.1 ms at 12); at p = 18 one box
+        // per workgroup stays faster (6 digits: FP32 2.41 against 2.45 ms, FP64 3.87 against
+        // 3.95 ms)
+        shift_groups = p >= 28;
+        if (const char *e = std::getenv("PDMK_SHIFT_GROUPS")) shift_groups = e[0] != '0';
         if (pw_by_level) {
             pw_out.alloc(std::max(pw_slot * (size_t)lvl_cnt, c2p_bytes));
         } else {
@@ -1191,7 +1229,14 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
         const int slot0 = pl.pw_by_level ? E.pw_lo : 0;
         for (int i0 = 0; i0 < E.n_items; i0 += pl.pin_batch) {
             const int nb = std::min(pl.pin_batch, E.n_items - i0);
-            if constexpr (sizeof(Real) == 8) {
+            if (pl.shift_groups) {
+                const int ng = (i0 + nb + kShiftGroup - 1) / kShiftGroup - i0 / kShiftGroup;
+                hipLaunchKernelGGL(k_pw_shift_grp<Real>, dim3(ng, (NM + 255) / 256, nv), dim3(256), 0, s, B.pw,
+                                   pl.shift0.template as<C>(), pl.shift_r.template as<double>(),
+                                   pl.shift_e.template as<Cx<double>>(), lscale, E.grp_ptr.template as<int>(),
+                                   E.grp_slot.template as<int>(), E.grp_sten.template as<unsigned long long>(), slot0, NM,
+                                   i0, nb, NPW, (NPW + 1) / 2, B.pin, B.s_pw, B.s_pin);
+            } else if constexpr (sizeof(Real) == 8) {
                 hipLaunchKernelGGL(k_pw_shift_sep, dim3(nb, (NM + 255) / 256, nv), dim3(256), 0, s,
                                    (const Cx<double> *)B.pw, pl.shift_r.template as<double>(),
                                    pl.shift_e.template as<Cx<double>>(), (double)lscale, E.nbr_ptr.template as<int>(),

@@ -168,6 +168,7 @@ struct Plan {
     // the level's shifts); c2p_scratch then shares pw_out's memory (the upward pass is done
     // before the first plane wave is formed, on the same stream).
     bool pw_by_level = false;
+    bool shift_groups = true;  // k_pw_shift_grp (p >= 28; PDMK_SHIFT_GROUPS=0 / 1 forces either)
     bool t3_mfma = true;  // FP64 tensor transforms at p = 28 on the matrix cores (PDMK_T3_MFMA=0: vector ALUs)
     // evaluate() on the caller's stream only (the stages one after another): for callers that
     // run many trees side by side, where the trees, not the stages, fill the GPU
@@ -179,7 +180,12 @@ struct Plan {
     DevBuf shift_r, shift_e;                   // FP64 plans: radial factor per mode, per-axis phase
     DevBuf poly2pw_f, pw2poly_f;  // folded tables of the two-pass transforms (64 entries a row)
     // pw_lo, pw_cnt: the range of up slots the level's shifts read
-    struct EvalLevel { int level, n_items, pw_lo = 0, pw_cnt = 0; DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten; };
+    // per level: the items' neighbour lists (CSR of slot, stencil) and the same merged per group
+    // of kShiftGroup items (grp_ptr / grp_slot / grp_sten, byte t = stencil for item t or 255)
+    struct EvalLevel {
+        int level, n_items, pw_lo = 0, pw_cnt = 0;
+        DevBuf down, zero, nbr_ptr, nbr_slot, nbr_sten, grp_ptr, grp_slot, grp_sten;
+    };
     std::vector<EvalLevel> eval_levels;
     std::vector<Tensor3Launch> p2c_levels;  // one per level (may be empty)
     DevBuf proxy_down;

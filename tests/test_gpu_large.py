@@ -179,3 +179,27 @@ def test_child_to_parent_per_parent_items(pdmk, cuda, monkeypatch, digits, preci
             s.close()
     tol = 1e-6 if precision == "float" else 1e-13
     assert rel(out["1"], out["1000000000"]) < tol, rel(out["1"], out["1000000000"])
+
+
+@pytest.mark.parametrize("digits,precision", [(6, "float"), (9, "double"), (12, "double")])
+def test_grouped_shifts(pdmk, cuda, monkeypatch, digits, precision):
+    """The plane-wave shifts of groups of 8 consecutive target boxes (their neighbour lists
+    merged by source slot, each neighbour read once per group) against one box per workgroup
+    (PDMK_SHIFT_GROUPS=1 / 0): the same terms, summed in another order per box, on a clustered
+    triclinic tree (irregular lists, partial groups) and a cubic tree whose level-0 image boxes
+    fold (a slot that appears twice in one box's list)."""
+    cases = [(W.triclinic_cell(), W.gaussian_mixture(40_000, 8, 0.03, 12, W.triclinic_cell())),
+             (W.cubic_cell(), W.uniform_positions(30_000, 13, W.cubic_cell()))]
+    tol = 1e-6 if precision == "float" else 1e-13
+    for cell, pos in cases:
+        q = W.neutral_uniform_charges(pos.shape[0], 46)
+        out = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("PDMK_SHIFT_GROUPS", mode)
+            s = pdmk.PDMKSolver(cell, pos, n_digits=digits, n_per_leaf=40, nz=4, precision=precision)
+            monkeypatch.delenv("PDMK_SHIFT_GROUPS")
+            try:
+                out[mode] = s.evaluate(q).astype(np.float64)
+            finally:
+                s.close()
+        assert rel(out["1"], out["0"]) < tol, rel(out["1"], out["0"])
