@@ -123,11 +123,11 @@ int pdmk_gpu_stats(void* h, long long* out16);
 /// evaluate() of n_vec charge vectors on one tree: d_charges FP64 [n_vec][N], d_pot [n_vec][N]
 /// in the handle's precision (N = pdmk_gpu_n_points), enqueued on `stream` (NULL = the
 /// handle's stream).  No reference counterpart (the reference evaluates one vector per call).
-/// FP32 handles at 6 digits (the row near field) form the near field of 8 vectors per pass,
-/// each (target, source) residual once for all 8, beside the far fields, then run the proxy /
+/// FP32 handles at 6 digits (the row near field) form the near field of 16 vectors per pass,
+/// each (target, source) residual once for all 16, beside the far fields, then run the proxy /
 /// plane-wave chain per vector; other handles call evaluate() per vector.  Each vector's
 /// potentials equal pdmk_gpu_evaluate_*'s on it.  The first batched call allocates the
-/// 8-vector buffers (not capture-safe); later calls allocate nothing.  -1 on a NULL handle or
+/// 16-vector buffers (not capture-safe); later calls allocate nothing.  -1 on a NULL handle or
 /// buffer, n_vec < 0, a quasi-periodic handle or a HIP error.
 int pdmk_gpu_evaluate_many(void* h, int n_vec, const void* d_charges, void* d_pot, void* stream);
 

@@ -12,7 +12,7 @@ Many trees run side by side only on as many hardware queues as the process has
 tree's stages one after another on its own stream) eager calls reach 141 with 4 queues.
 Each graph holds every tree's calls for --per-graph of the vectors; a pass launches
 vectors / per-graph graphs back to back.  A third pass evaluates all of a tree's vectors in one
-call (pdmk_gpu_evaluate_many: the near field of 8 vectors per pass).  The graph's and the
+call (pdmk_gpu_evaluate_many: the near field of 16 vectors per pass).  The graph's and the
 many-vector potentials are checked bit for bit against the eager pass.  Prints one JSON line:
 points x vectors per second for each.
 """

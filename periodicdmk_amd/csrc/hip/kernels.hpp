@@ -2040,9 +2040,10 @@ __global__ void k_gather_many(const double *__restrict__ Q, int n_vec, int v0, i
     for (int c = 0; c < VC; ++c) qv[(size_t)i * VC + c] = v0 + c < n_vec ? (float)Q[(size_t)(v0 + c) * n + u] : 0.0f;
 }
 
-// ea[c] += q[c](lane 4 g + S) * u for the 8 charges of a row's step: DPP-broadcast FMAs in one
+// ea[c] += q[c](lane 4 g + S) * u for 8 charges of a row's step: DPP-broadcast FMAs in one
 // asm statement that starts with s_nop 1, the two wait states a DPP read needs after a VALU
-// write of its source (the compiler does not check hazards inside inline asm).
+// write of its source (the compiler does not check hazards inside inline asm).  ea and q point
+// at 8 consecutive elements of register arrays (constant offsets once inlined).
 #define PDMK_FMAC_QUAD(SEL)                                                                                    \
     asm("s_nop 1\n"                                                                                            \
         "v_fmac_f32_dpp %0, %8, %16 quad_perm:[" SEL "] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"            \
@@ -2057,7 +2058,7 @@ __global__ void k_gather_many(const double *__restrict__ Q, int n_vec, int v0, i
         : "v"(q[0]), "v"(q[1]), "v"(q[2]), "v"(q[3]), "v"(q[4]), "v"(q[5]), "v"(q[6]), "v"(q[7]), "v"(u))
 
 template <int S>
-__device__ __forceinline__ void fmac_quad8(float (&ea)[8], const float (&q)[8], float u) {
+__device__ __forceinline__ void fmac_quad8(float *ea, const float *q, float u) {
     if constexpr (S == 0) PDMK_FMAC_QUAD("0,0,0,0");
     if constexpr (S == 1) PDMK_FMAC_QUAD("1,1,1,1");
     if constexpr (S == 2) PDMK_FMAC_QUAD("2,2,2,2");
@@ -2065,18 +2066,25 @@ __device__ __forceinline__ void fmac_quad8(float (&ea)[8], const float (&q)[8], 
 }
 #undef PDMK_FMAC_QUAD
 
-// k_near_rows for VC = 8 charge vectors at once: the residual of each (target, source) pair is
+// all VC charges of a step, 8 per asm block
+template <int S, int VC>
+__device__ __forceinline__ void fmac_quads(float (&ea)[VC], const float (&q)[VC], float u) {
+#pragma unroll
+    for (int v = 0; v < VC; v += 8) fmac_quad8<S>(ea + v, q + v, u);
+}
+
+// k_near_rows for VC charge vectors at once (a multiple of 8): the residual of each (target, source) pair is
 // formed once and multiplied into VC sums.  pot_near[c * n + t] (c < vc) receives vector c's
 // near field of sorted target t; each vector's sums are those of k_near_rows on that vector,
 // bit for bit.
-template <int NC, int VC = 8>
-__global__ __launch_bounds__(1024) void k_near_rows_many(
+template <int NC, int VC>
+__global__ __launch_bounds__(256) void k_near_rows_many(
     const P4<float> *__restrict__ src, const float *__restrict__ qv, const P4<float> *__restrict__ trg,
     const int4 *__restrict__ items, const int *__restrict__ l1_ptr, const P4<float> *__restrict__ geom,
     const int *__restrict__ rs_base, const int *__restrict__ rslot, const long long *__restrict__ wave_base,
     const int *__restrict__ steps, const float *__restrict__ coeffs, float *__restrict__ pot_near, int n, int vc,
     int item0, int item1, int ne_max) {
-    static_assert(VC == 8, "the DPP FMA block takes 8 vectors");
+    static_assert(VC % 8 == 0, "the DPP FMA blocks take 8 vectors each");
     extern __shared__ float4 nf_lds[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = item0 + xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + w;
@@ -2168,10 +2176,10 @@ __global__ __launch_bounds__(1024) void k_near_rows_many(
                 u = p * rinv;
             }
             switch (s) {
-            case 0: fmac_quad8<0>(ea, qc, u); break;
-            case 1: fmac_quad8<1>(ea, qc, u); break;
-            case 2: fmac_quad8<2>(ea, qc, u); break;
-            default: fmac_quad8<3>(ea, qc, u); break;
+            case 0: fmac_quads<0>(ea, qc, u); break;
+            case 1: fmac_quads<1>(ea, qc, u); break;
+            case 2: fmac_quads<2>(ea, qc, u); break;
+            default: fmac_quads<3>(ea, qc, u); break;
             }
         }
         pc = pn;
@@ -2303,6 +2311,8 @@ __global__ void k_fill(R *__restrict__ p, R v, int n) {
 constexpr int FF_CHUNK = 8;
 constexpr int FF_TP = 16;  // points per LDS tile
 constexpr int FF_QUAD = 4;  // structure factors per scalar load in the evaluation
+constexpr int FF_MANY_S = 4;  // vectors per workgroup of a many-vector structure-factor launch
+constexpr int FF_MANY_E = 2;  // vectors per workgroup of a many-vector evaluation
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
@@ -2310,27 +2320,34 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 
 // Partial structure factors per workgroup of points:
 //   Sp[blk][m] = sum_{points of blk} q e^{-i(i th1 + j th2 + k th3)}.
-// Per point tile, q e^{-i m th} for every axis index m is tabulated in LDS (FP64 recurrence
-// from one sincos per axis).  A lane owns one chunk: per point it gathers its first mode's
-// factor from the three tables and walks the chunk by multiplying with e^{-i th1}.
+// Per point tile, e^{-i m th} for every axis index m is tabulated in LDS (FP64 recurrence
+// from one sincos per axis), with the tile's charges beside it.  A lane owns one chunk: per
+// point it gathers its first mode's factor from the three tables, walks the chunk by
+// multiplying with e^{-i th1} and adds q times each factor.  Workgroup z of a many-vector
+// launch takes vectors V z .. V z + V - 1 (< nv; vector v's charges at q + v vs_q, its partial
+// sums at Sp + v vs_sp): the factors are formed once for its V vectors, and each vector's sums
+// are those of a one-vector launch, bit for bit.
+template <int V>
 __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__ theta, const double *__restrict__ q, int n,
                                                        int ms, int mt, int mu, const int *__restrict__ chunk_ijk,
                                                        const int *__restrict__ chunk_m0, const int *__restrict__ chunk_len,
                                                        int n_chunks, int n_modes, double2 *__restrict__ Sp,
-                                                       long long vs_q = 0, long long vs_sp = 0) {
+                                                       long long vs_q = 0, long long vs_sp = 0, int nv = 1) {
     extern __shared__ double2 ff_lds[];
-    q += (long long)blockIdx.z * vs_q;  // vector blockIdx.z of a many-vector launch
-    Sp += (long long)blockIdx.z * vs_sp;
+    const int v0 = blockIdx.z * V;
     double2 *E1 = ff_lds, *E2 = E1 + FF_TP * ms, *E3 = E2 + FF_TP * mt, *Est = E3 + FF_TP * mu;
+    double *Qs = reinterpret_cast<double *>(Est + FF_TP);  // [V][FF_TP]
     const int c = blockIdx.y * blockDim.x + threadIdx.x;
     const bool active = c < n_chunks;
     const int ci = active ? chunk_ijk[3 * c] + ms / 2 : 0;
     const int cj = active ? chunk_ijk[3 * c + 1] + mt / 2 : 0;
     const int ck = active ? chunk_ijk[3 * c + 2] + mu / 2 : 0;
     const int len = active ? chunk_len[c] : 0;
-    double2 acc[FF_CHUNK];
+    double2 acc[V][FF_CHUNK];
 #pragma unroll
-    for (int r = 0; r < FF_CHUNK; ++r) acc[r] = make_double2(0, 0);
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int r = 0; r < FF_CHUNK; ++r) acc[v][r] = make_double2(0, 0);
     for (int p0 = blockIdx.x * FF_TP; p0 < n; p0 += gridDim.x * FF_TP) {
         __syncthreads();
         for (int t = threadIdx.x; t < FF_TP * 3; t += blockDim.x) {
@@ -2339,28 +2356,37 @@ __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__
             const int lo = -(cnt / 2);
             double2 *row = (ax == 0 ? E1 : (ax == 1 ? E2 : E3)) + pp * cnt;
             const double th = gp < n ? theta[3 * (size_t)gp + ax] : 0.0;
-            const double qq = gp < n ? (ax == 2 ? q[gp] : 1.0) : 0.0;
             double s1, c1, sl, cl;
             sincos(th, &s1, &c1);
             sincos(-th * lo, &sl, &cl);
             const double2 step = make_double2(c1, -s1);  // e^{-i th}
             if (ax == 0) Est[pp] = step;
-            double2 cur = make_double2(cl * qq, sl * qq);  // q e^{-i lo th}
+            double2 cur = make_double2(cl, sl);  // e^{-i lo th}
             for (int m = 0; m < cnt; ++m) {
                 row[m] = cur;
                 cur = cmul(cur, step);
             }
+        }
+        for (int t = threadIdx.x; t < FF_TP * V; t += blockDim.x) {
+            const int pp = t % FF_TP, v = t / FF_TP, gp = p0 + pp;
+            Qs[t] = gp < n && v0 + v < nv ? q[(long long)(v0 + v) * vs_q + gp] : 0.0;
         }
         __syncthreads();
         if (active) {
             const int np = min(FF_TP, n - p0);
             for (int pp = 0; pp < np; ++pp) {
                 const double2 st = Est[pp];
+                double qq[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) qq[v] = Qs[v * FF_TP + pp];
                 double2 cur = cmul(cmul(E1[pp * ms + ci], E2[pp * mt + cj]), E3[pp * mu + ck]);
 #pragma unroll
                 for (int r = 0; r < FF_CHUNK; ++r) {
-                    acc[r].x += cur.x;
-                    acc[r].y += cur.y;
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        acc[v][r].x = fma(qq[v], cur.x, acc[v][r].x);
+                        acc[v][r].y = fma(qq[v], cur.y, acc[v][r].y);
+                    }
                     cur = cmul(cur, st);
                 }
             }
@@ -2369,8 +2395,13 @@ __global__ __launch_bounds__(256) void k_ff_structure(const double *__restrict__
     if (active) {
         const int m0 = chunk_m0[c];
 #pragma unroll
-        for (int r = 0; r < FF_CHUNK; ++r)
-            if (r < len) Sp[(size_t)blockIdx.x * n_modes + m0 + r] = acc[r];
+        for (int v = 0; v < V; ++v) {
+            if (v0 + v >= nv) break;
+            double2 *out = Sp + (long long)(v0 + v) * vs_sp + (size_t)blockIdx.x * n_modes + m0;
+#pragma unroll
+            for (int r = 0; r < FF_CHUNK; ++r)
+                if (r < len) out[r] = acc[v][r];
+        }
     }
 }
 
@@ -2419,12 +2450,15 @@ __device__ __forceinline__ void ff_advance(double2 &e, int d, double2 z) {
 // (j, k, ilo, nq) holds its weighted S at S[base[L] ..], padded to nq * FF_QUAD entries; its
 // sum over i is a Horner polynomial in z1 = e^{i th1} with wave-uniform coefficients (read
 // FF_QUAD at a time), multiplied by z1^ilo z2^j z3^k, which is advanced from the previous
-// line by (small) exponent steps.
+// line by (small) exponent steps.  Workgroup z of a many-vector launch evaluates vectors
+// V z .. V z + V - 1 (< nv; S at S + v vs_s, output at far + v vs_far) with one set of
+// phases; each vector's sums are those of a one-vector launch, bit for bit.
+template <int V>
 __global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ theta, int n, const int4 *__restrict__ lines,
                                                  const int *__restrict__ base, int n_lines, const double2 *__restrict__ S,
-                                                 double *__restrict__ far, long long vs_s = 0, long long vs_far = 0) {
-    S += (long long)blockIdx.z * vs_s;  // vector blockIdx.z of a many-vector launch
-    far += (long long)blockIdx.z * vs_far;
+                                                 double *__restrict__ far, long long vs_s = 0, long long vs_far = 0,
+                                                 int nv = 1) {
+    const int v0 = blockIdx.z * V;
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     const int uc = u < n ? u : n - 1;
     double2 z1, z2, z3;
@@ -2437,31 +2471,48 @@ __global__ __launch_bounds__(256) void k_ff_eval(const double *__restrict__ thet
         sincos(theta[3 * (size_t)uc + 2], &s, &c);
         z3 = make_double2(c, s);
     }
+    // vectors past nv read vector v0's sums (in bounds) and store nothing
+    const double2 *Sv[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Sv[v] = S + (long long)(v0 + (v0 + v < nv ? v : 0)) * vs_s;
     double2 e1 = make_double2(1, 0), e2 = make_double2(1, 0), e3 = make_double2(1, 0);
     int ic = 0, jc = 0, kc = 0;
-    double acc = 0;
+    double acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0;
     for (int L = 0; L < n_lines; ++L) {
         const int4 d = lines[L];
-        const double2 *Sl = S + base[L];
+        const int bL = base[L];
         ff_advance(e1, d.z - ic, z1);
         ff_advance(e2, d.x - jc, z2);
         ff_advance(e3, d.y - kc, z3);
         ic = d.z;
         jc = d.x;
         kc = d.y;
-        double2 h = make_double2(0, 0);
+        double2 h[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) h[v] = make_double2(0, 0);
         for (int qd = d.w - 1; qd >= 0; --qd) {
-            double2 sv[FF_QUAD];
 #pragma unroll
-            for (int r = 0; r < FF_QUAD; ++r) sv[r] = Sl[FF_QUAD * qd + r];
+            for (int v = 0; v < V; ++v) {
+                double2 sv[FF_QUAD];
 #pragma unroll
-            for (int r = FF_QUAD - 1; r >= 0; --r)
-                h = make_double2(fma(h.x, z1.x, fma(-h.y, z1.y, sv[r].x)), fma(h.x, z1.y, fma(h.y, z1.x, sv[r].y)));
+                for (int r = 0; r < FF_QUAD; ++r) sv[r] = Sv[v][bL + FF_QUAD * qd + r];
+#pragma unroll
+                for (int r = FF_QUAD - 1; r >= 0; --r)
+                    h[v] = make_double2(fma(h[v].x, z1.x, fma(-h[v].y, z1.y, sv[r].x)),
+                                        fma(h[v].x, z1.y, fma(h[v].y, z1.x, sv[r].y)));
+            }
         }
         const double2 bs = cmul(cmul(e1, e2), e3);
-        acc = fma(h.x, bs.x, fma(-h.y, bs.y, acc));
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = fma(h[v].x, bs.x, fma(-h[v].y, bs.y, acc[v]));
     }
-    if (u < n) far[u] = acc;
+    if (u < n) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (v0 + v < nv) far[(long long)(v0 + v) * vs_far + u] = acc[v];
+    }
 }
 
 // ------------------------------------------------------------------ far field by NUFFT

@@ -1848,15 +1848,15 @@ void FourierSum::run_at(const double *d_q, const double *d_theta_eval, int n_eva
         check(hipGetLastError(), "k_fill far");
         return;
     }
-    const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1);
+    const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1) + sizeof(double) * FF_TP;
     dim3 grid(blocks, (n_chunks + 255) / 256);
-    hipLaunchKernelGGL(k_ff_structure, grid, dim3(256), lds, s, theta_d, d_q, n, ms, mt, mu, chunk_ijk_d, chunk_m0_d,
-                       chunk_len_d, n_chunks, n_modes, Sp_d);
+    hipLaunchKernelGGL(k_ff_structure<1>, grid, dim3(256), lds, s, theta_d, d_q, n, ms, mt, mu, chunk_ijk_d, chunk_m0_d,
+                       chunk_len_d, n_chunks, n_modes, Sp_d, 0ll, 0ll, 1);
     check(hipGetLastError(), "k_ff_structure");
     hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32), dim3(256), 0, s, Sp_d, blocks, w_d, pidx_d, n_modes, S_d);
     check(hipGetLastError(), "k_ff_reduce");
-    hipLaunchKernelGGL(k_ff_eval, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_theta_eval, n_eval, lines_d,
-                       line_base_d, n_lines, S_d, d_out);
+    hipLaunchKernelGGL(k_ff_eval<1>, dim3((n_eval + 255) / 256), dim3(256), 0, s, d_theta_eval, n_eval, lines_d,
+                       line_base_d, n_lines, S_d, d_out, 0ll, 0ll, 1);
     check(hipGetLastError(), "k_ff_eval");
 }
 
@@ -1868,21 +1868,23 @@ void FourierSum::run_many(const double *d_Q, long long ldq, int nv, double *d_ou
         return;
     }
     if (vc < 1 || vc > 65535 || nv > 65535) throw std::invalid_argument("FourierSum::run_many: vector count");
-    const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1);
+    constexpr int VS = FF_MANY_S, VE = FF_MANY_E;
+    const size_t lds = sizeof(double2) * FF_TP * (ms + mt + mu + 1) + sizeof(double) * FF_TP * VS;
     const long long spv = (long long)sp_elems(), sv = (long long)s_len;
-    // the same launches as run(), with the vectors along z: per-vector sums in the same order
+    // the sums of run() per vector, VS (VE) vectors per workgroup of the structure factors
+    // (evaluation) along z
     for (int v0 = 0; v0 < nv; v0 += vc) {
         const int c = std::min(vc, nv - v0);
-        hipLaunchKernelGGL(k_ff_structure, dim3(blocks, (n_chunks + 255) / 256, c), dim3(256), lds, s, theta_d,
-                           d_Q + v0 * ldq, n, ms, mt, mu, chunk_ijk_d, chunk_m0_d, chunk_len_d, n_chunks, n_modes,
-                           Sp_many, (long long)ldq, spv);
+        hipLaunchKernelGGL(k_ff_structure<VS>, dim3(blocks, (n_chunks + 255) / 256, (c + VS - 1) / VS), dim3(256), lds,
+                           s, theta_d, d_Q + v0 * ldq, n, ms, mt, mu, chunk_ijk_d, chunk_m0_d, chunk_len_d, n_chunks,
+                           n_modes, Sp_many, (long long)ldq, spv, c);
         check(hipGetLastError(), "k_ff_structure");
         hipLaunchKernelGGL(k_ff_reduce, dim3((n_modes + 31) / 32, 1, c), dim3(256), 0, s, Sp_many, blocks, w_d, pidx_d,
                            n_modes, S_many + v0 * sv, spv, sv);
         check(hipGetLastError(), "k_ff_reduce");
     }
-    hipLaunchKernelGGL(k_ff_eval, dim3((n + 255) / 256, 1, nv), dim3(256), 0, s, theta_d, n, lines_d, line_base_d,
-                       n_lines, S_many, d_out, sv, (long long)ldo);
+    hipLaunchKernelGGL(k_ff_eval<VE>, dim3((n + 255) / 256, 1, (nv + VE - 1) / VE), dim3(256), 0, s, theta_d, n,
+                       lines_d, line_base_d, n_lines, S_many, d_out, sv, (long long)ldo, nv);
     check(hipGetLastError(), "k_ff_eval");
 }
 

@@ -231,22 +231,24 @@ struct Plan {
     void evaluate_host(const double *q, Real *pot, double *times_ms = nullptr);
 
     // evaluate() of n_vec charge vectors (d_Q, d_P: vector-major, n per vector) on `s`.  FP32
-    // plans with row step lists take the vectors in passes of 8: the near field of the pass
-    // (each pair's residual formed once for the 8 vectors) on s_near, the far fields on s_far
-    // (the exact Fourier sum of several vectors per launch) and the proxy / plane-wave chain of
+    // plans with row step lists take the vectors in passes of 16: the near field of the pass
+    // (each pair's residual formed once for the 16 vectors) on s_near, the far fields on s_far
+    // (the exact Fourier sum of several vectors per launch, the phases of a workgroup shared by
+    // FF_MANY_S / FF_MANY_E vectors) and the proxy / plane-wave chain of
     // several vectors per launch on s, the three beside each other and beside the next pass;
     // other plans call evaluate() per vector.  Each vector's potentials equal evaluate()'s on
     // it bit for bit.  The per-vector buffers are allocated on first use (many_ensure).
     void evaluate_many(int n_vec, const double *d_Q, Real *d_P, hipStream_t s);
-    static constexpr int kManyVec = 8;  // vectors per near-field pass
+    static constexpr int kManyVec = 16;  // vectors per near-field pass
     // vectors per chain launch and per exact far-field launch (sized to a memory budget);
     // 0 until the buffers exist, -1 when they do not fit (evaluate_many then calls evaluate())
     int many_vc_chain = 0, many_vc_far = 0;
     bool many_ensure();
-    // two pass slots (b = pass & 1): charges [nsall][8] (FP32), near [8][n], far [8][n]
+    // two pass slots (b = pass & 1): charges [nsall][kManyVec] (FP32), near [kManyVec][n],
+    // far [kManyVec][n]
     DevBuf many_qv[2], many_near[2], many_far[2];
     // chain copies for many_vc_chain vectors, potentials of a pass, exact far-field partial
-    // sums for many_vc_far vectors and weighted sums for 8
+    // sums for many_vc_far vectors and weighted sums for kManyVec
     DevBuf many_up, many_c2p, many_pw, many_pin, many_down, many_pot, many_sp, many_s;
     // per slot: charges gathered, near field done, far field done, pass finished
     hipEvent_t many_ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};

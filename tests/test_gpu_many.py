@@ -1,10 +1,12 @@
 """evaluate() of many charge vectors per call (pdmk_gpu_evaluate_many) against one call per
-vector: FP32 plans at 6 digits take the vectors in passes of 8 (the near field of a pass with
+vector: FP32 plans at 6 digits take the vectors in passes of 16 (the near field of a pass with
 each residual computed once, the chain and the exact far field of several vectors per launch,
-passes overlapping); every vector's potentials equal pdmk_gpu_evaluate_float's bit for bit
-(same sums in the same order).  FP64 plans call evaluate() per vector.  Vector counts below,
-at and above a pass of 8, three passes (the pass buffers reused), and 0; a tree large enough
-for the NUFFT far field and for chain launches of fewer than 8 vectors."""
+the far field's phases shared by 4 vectors per workgroup of structure factors and 2 of the
+evaluation, passes overlapping); every vector's
+potentials equal pdmk_gpu_evaluate_float's bit for bit (same sums in the same order).  FP64
+plans call evaluate() per vector.  Vector counts below, at and above a pass of 16, groups of 4
+cut short, three passes (the pass buffers reused), and 0; a tree large enough for the NUFFT far
+field and for chain launches of fewer than 16 vectors, in two passes."""
 import numpy as np
 import pytest
 
@@ -25,7 +27,7 @@ def test_many_equals_single(pdmk, cuda, name, precision):
     cell, pos, npl = CASES[name]()
     n = pos.shape[0]
     s = pdmk.PDMKSolver(cell, pos, n_digits=6, n_per_leaf=npl, nz=4, precision=precision)
-    for nv in (1, 2, 8, 11, 20):
+    for nv in (1, 2, 11, 16, 37):
         Q = np.stack([W.neutral_uniform_charges(n, 500 + 7 * nv + v) for v in range(nv)])
         got = s.evaluate_many(Q)
         for v in range(nv):
@@ -38,8 +40,8 @@ def test_many_large_tree(pdmk, cuda):
     n = 400_000
     cell = W.cubic_cell()
     s = pdmk.PDMKSolver(cell, W.uniform_positions(n, 24, cell), n_digits=6, n_per_leaf=300, nz=4, precision="float")
-    Q = np.stack([W.neutral_uniform_charges(n, 900 + v) for v in range(9)])
+    Q = np.stack([W.neutral_uniform_charges(n, 900 + v) for v in range(18)])
     got = s.evaluate_many(Q)
-    for v in range(9):
-        np.testing.assert_array_equal(got[v], s.evaluate(Q[v]), err_msg=f"vector {v} of 9")
+    for v in range(18):
+        np.testing.assert_array_equal(got[v], s.evaluate(Q[v]), err_msg=f"vector {v} of 18")
     s.close()
