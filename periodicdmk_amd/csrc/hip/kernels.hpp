@@ -213,9 +213,11 @@ __global__ __launch_bounds__(C2pCfg<P>::threads) void k_charge2proxy(
 // is a GEMM of P^2 rows by PA columns (a padded to 16s) over the box's sources, computed with
 // v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
 // D[l / 16 + 4 r][l % 16] in accumulator element r (layout read off gfx950 hardware).  The
-// row tiles of a box are split over RS workgroups (two at p = 38, whose accumulators would
-// not fit one workgroup's registers); wave w of a workgroup owns its row tiles w, w + NW, ...
-// and every column tile.
+// row tiles of a box are split over RS workgroups; wave w of a workgroup owns its row tiles
+// w, w + NW, ... and every column tile.  Two shapes at p = 28: for trees with many boxes
+// (WIDE) two workgroups of 4 waves per box (7 row tiles per wave, 158 VGPRs: three
+// workgroups share a CU, so one's table building overlaps the others' matrix products);
+// otherwise one workgroup of 16 waves (4 row tiles per wave, 98 VGPRs).
 typedef double c2p_d4 __attribute__((ext_vector_type(4)));
 typedef float c2p_f4 __attribute__((ext_vector_type(4)));
 
@@ -237,11 +239,11 @@ template <> struct Mfma16<float> {
     }
     static __device__ __forceinline__ int row(int kk, int e) { return 4 * kk + e; }
 };
-template <int P> struct C2pMfmaCfg {
-    static constexpr int RT = (P * P + 15) / 16;      // row tiles
-    static constexpr int RS = P >= 38 ? 2 : 1;        // workgroups per box
-    static constexpr int RTS = (RT + RS - 1) / RS;    // row tiles per workgroup
-    static constexpr int NW = 8;                      // waves per workgroup (two per SIMD)
+template <int P, bool WIDE> struct C2pMfmaCfg {
+    static constexpr int RT = (P * P + 15) / 16;                  // row tiles
+    static constexpr int RS = P >= 38 || WIDE ? 2 : 1;            // workgroups per box
+    static constexpr int RTS = (RT + RS - 1) / RS;                // row tiles per workgroup
+    static constexpr int NW = WIDE ? 4 : (P >= 38 ? 8 : 16);      // waves per workgroup
     static constexpr int TPW = (RTS + NW - 1) / NW;   // row tiles per wave
     static constexpr int TS = 64;                     // sources per LDS table
     static constexpr int PA = (P + 15) / 16 * 16;     // a, padded
@@ -251,13 +253,13 @@ template <int P> struct C2pMfmaCfg {
     static constexpr int SX = PA + 1, SY = P % 2 ? P : P + 1;
 };
 
-template <class R, int P>
-__global__ __launch_bounds__(64 * C2pMfmaCfg<P>::NW) void k_charge2proxy_mfma(
+template <class R, int P, bool WIDE>
+__global__ __launch_bounds__((64 * C2pMfmaCfg<P, WIDE>::NW)) void k_charge2proxy_mfma(
     const P4<R> *__restrict__ src, const int *__restrict__ grp_slot, const R *__restrict__ grp_geom,
     const int *__restrict__ grp_rng, const int *__restrict__ rng_begin, const int *__restrict__ rng_count,
     const R *__restrict__ rng_off, R *__restrict__ proxy) {
     chain_priority();
-    using C = C2pMfmaCfg<P>;
+    using C = C2pMfmaCfg<P, WIDE>;
     using M = Mfma16<R>;
     constexpr int TS = C::TS, PA = C::PA, RT = C::RT, RS = C::RS, RTS = C::RTS, NW = C::NW, TPW = C::TPW, CT = C::CT,
                   SX = C::SX, SY = C::SY;

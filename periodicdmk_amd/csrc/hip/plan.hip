@@ -59,6 +59,11 @@ constexpr size_t kTwoPassBytes = size_t(512) << 20;  // two-pass transform scrat
 // 4.16 ms with one item per parent)
 constexpr int kC2pFuseParents = 256;
 
+// proxy slots from which the charge -> proxy matrix-core kernel (p = 28) takes its wide
+// workgroup shape: 64 boxes per CU (the N=1e6 uniform tree has about 5e3 slots, the
+// clustered N=4e6 one 1.2e5)
+constexpr int kC2pWideBoxes = 16384;
+
 // f(i0, i1) over contiguous chunks of [0, n) on up to 16 host threads (construction work
 // whose chunks write disjoint data)
 template <class F> void host_parallel(int n, F f) {
@@ -979,6 +984,11 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
         pw_by_level = all_bytes + (size_t(1) << 30) > free_b;
         if (const char *e = std::getenv("PDMK_PW_LEVELS")) pw_by_level = e[0] == '1';
         if (const char *e = std::getenv("PDMK_T3_MFMA")) t3_mfma = e[0] != '0';
+        // charge -> proxy workgroup shape at p = 28 (same sums either way): clustered N=4e6,
+        // 124,287 slots, 9-digit evaluate() 130.9 -> 126.2 ms with the wide shape; N=1e6
+        // uniform 7.73 -> 7.53 ms with the 16-wave one
+        c2p_wide = n_up >= kC2pWideBoxes;
+        if (const char *e = std::getenv("PDMK_C2P_WIDE")) c2p_wide = e[0] != '0';
         // grouped shifts at p >= 28 (N=1e6 9 digits: evaluate() 7.57 -> 7.39 ms; clustered
         // N=4e6 132.5 -> 131.1 ms at 9 digits, 412.5 -> 403.1 ms at 12); at p = 18 one box
         // per workgroup stays faster (6 digits: FP32 2.41 against 2.45 ms, FP64 3.87 against
@@ -1128,12 +1138,19 @@ void launch_transforms(Plan<Real> &pl, hipStream_t s, const ChainBufs<Real> &B) 
         // many-vector chain with it: the same sums in the same order as one vector's).
         if constexpr (std::is_same_v<Real, double> && P >= 28 && P <= 32) {
             if (nv != 1 || B.qv) throw std::logic_error("launch_transforms: one vector on the matrix-core kernels");
-            hipLaunchKernelGGL((k_charge2proxy_mfma<Real, P>), dim3(pl.n_up * C2pMfmaCfg<P>::RS),
-                               dim3(64 * C2pMfmaCfg<P>::NW), 0, s,
-                               pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
-                               pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(), pl.rng_count.template as<int>(),
-                               pl.rng_off.template as<Real>(), B.up);
-            check(hipGetLastError(), "k_charge2proxy_mfma");
+            auto c2p = [&](auto wide) {
+                constexpr bool W = decltype(wide)::value;
+                hipLaunchKernelGGL((k_charge2proxy_mfma<Real, P, W>), dim3(pl.n_up * C2pMfmaCfg<P, W>::RS),
+                                   dim3(64 * C2pMfmaCfg<P, W>::NW), 0, s, pl.src.template as<P4<Real>>(),
+                                   pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),
+                                   pl.up_rng.template as<int>(), pl.rng_begin.template as<int>(),
+                                   pl.rng_count.template as<int>(), pl.rng_off.template as<Real>(), B.up);
+                check(hipGetLastError(), "k_charge2proxy_mfma");
+            };
+            if (pl.c2p_wide)
+                c2p(std::true_type{});
+            else
+                c2p(std::false_type{});
         } else {
             hipLaunchKernelGGL((k_charge2proxy<Real, P>), dim3(pl.n_up, 1, nv), dim3(C2pCfg<P>::threads), 0, s,
                                pl.src.template as<P4<Real>>(), pl.up_slot.template as<int>(), pl.up_geom.template as<Real>(),

@@ -170,6 +170,9 @@ struct Plan {
     bool pw_by_level = false;
     bool shift_groups = true;  // k_pw_shift_grp (p >= 28; PDMK_SHIFT_GROUPS=0 / 1 forces either)
     bool t3_mfma = true;  // FP64 tensor transforms at p = 28 on the matrix cores (PDMK_T3_MFMA=0: vector ALUs)
+    // the WIDE shape of k_charge2proxy_mfma (two 4-wave workgroups per box) for trees with at
+    // least kC2pWideBoxes proxy slots, else one 16-wave workgroup (PDMK_C2P_WIDE=0 / 1 forces)
+    bool c2p_wide = false;
     // evaluate() on the caller's stream only (the stages one after another): for callers that
     // run many trees side by side, where the trees, not the stages, fill the GPU
     bool single_stream = false;

@@ -158,6 +158,26 @@ def test_tensor3_matrix_cores_identical(pdmk, cuda, monkeypatch, digits, n, npl)
     np.testing.assert_array_equal(out[None], out["0"])
 
 
+def test_charge_to_proxy_shapes_identical(pdmk, cuda, monkeypatch):
+    """FP64 at 9 digits forms the proxy charges on the matrix cores in one of two workgroup
+    shapes (two 4-wave workgroups per box for trees with many boxes, one 16-wave workgroup
+    otherwise); every output sums its sources in the same order either way, so forcing each
+    (PDMK_C2P_WIDE=1 / 0) on one clustered tree must give the same potentials bit for bit."""
+    cell = W.triclinic_cell()
+    pos = W.gaussian_mixture(60_000, 8, 0.03, 8, cell)
+    q = W.neutral_uniform_charges(pos.shape[0], 46)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("PDMK_C2P_WIDE", mode)
+        s = pdmk.PDMKSolver(cell, pos, n_digits=9, n_per_leaf=40, nz=4, precision="double")
+        monkeypatch.delenv("PDMK_C2P_WIDE")
+        try:
+            out[mode] = s.evaluate(q)
+        finally:
+            s.close()
+    np.testing.assert_array_equal(out["1"], out["0"])
+
+
 @pytest.mark.parametrize("digits,precision", [(6, "float"), (6, "double"), (9, "double"), (12, "double")])
 def test_child_to_parent_per_parent_items(pdmk, cuda, monkeypatch, digits, precision):
     """Levels with many parents run the child -> parent transform as one item per parent (its
