@@ -1837,6 +1837,10 @@ __global__ __launch_bounds__(1024) void k_near_field(const P4<R> *__restrict__ s
 // the potentials equal k_near_field's bit for bit.
 constexpr int NF_ROWS = 16;          // rows of 4 targets per wavefront
 constexpr int NF_ROW_PAD = 2 * 64;   // sentinel indices past the last block (loads run one block ahead)
+// Bit 31 of a step index flags a row's entry switch at that step (the first step of each of
+// the row's entries but its first).  Readers form byte offsets as (unsigned)index * size in
+// 32 bits, which drops the flag (source indices stay below 2^26).
+constexpr int NF_STEP_ENTRY = (int)0x80000000u;
 
 template <int S>
 __device__ __forceinline__ float quad_bcast(float v) {
@@ -1857,6 +1861,13 @@ __device__ __forceinline__ void fmac_quad1(float &ea, float q, float u) {
     if constexpr (S == 3) PDMK_FMAC_QUAD1("3,3,3,3");
 }
 #undef PDMK_FMAC_QUAD1
+
+// *(T *)((char *)base + off) with a 32-bit unsigned byte offset: a wave-uniform base and a
+// per-lane offset that the address computation needs no 64-bit arithmetic for
+template <class T, class B>
+__device__ __forceinline__ T nf_load_u32(const B *base, unsigned off) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
 
 // PASS 0: steps of row g in the k-th entry of item i -> cnt[(ie_base[i] + k) * 16 + g], and the
 // in-support pairs added to *pairs.  PASS 1: the source indices into the row blocks from
@@ -1910,7 +1921,9 @@ __global__ __launch_bounds__(256) void k_near_rows_build(
                     const unsigned long long in = __ballot(r2 < d2max && r2 > R(1e-30));
                     np += __popcll(in);
                     if ((in >> (lane & ~3)) & 0xFull) {
-                        if (PASS == 1 && (k_row & 3) == (lane & 3)) out[(size_t)(k_row >> 2) * 64] = s0 + j;
+                        // bit 31: the row's step starts an entry after its first
+                        if (PASS == 1 && (k_row & 3) == (lane & 3))
+                            out[(size_t)(k_row >> 2) * 64] = (s0 + j) | (c_row == 0 && k_row > 0 ? NF_STEP_ENTRY : 0);
                         ++k_row;
                         ++c_row;
                     }
@@ -1927,7 +1940,10 @@ __global__ __launch_bounds__(256) void k_near_rows_build(
 // (end step << 8) | ordinal of the entry in the target leaf's list1, followed by a terminator
 // (ordinal = number of entries: bs = 0, nothing in support).  Each wavefront copies its leaf's
 // entry geometry (offset, bs, 2/bs, -bs/2, bs^2) and its rows' slots into LDS (ne_max + 1
-// entries per wavefront), so an entry switch reads LDS only.
+// entries per wavefront), so an entry switch reads LDS only.  In LDS a slot is decoded ahead:
+// (end step << 16) | byte offset of the entry's geometry from the LDS base (end steps below
+// 2^16, and the workgroup's LDS below 64 KiB, both checked on the host; the terminator's end
+// is clamped to 2^16 - 1, past every step).
 template <int NC>
 __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict__ src, const P4<float> *__restrict__ trg,
                                                    const int4 *__restrict__ items, const int *__restrict__ l1_ptr,
@@ -1952,7 +1968,11 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
         geo[2 * e + 1] = make_float4(e < ne ? 2.0f / bs : 0.0f, -bs / 2.0f, 0.0f, 0.0f);
     }
     const int *rs = rslot + (size_t)rs_base[item] * NF_ROWS;
-    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) slots[k] = rs[k];
+    const unsigned geo_off = (unsigned)((const char *)geo - (const char *)nf_lds);
+    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) {
+        const int v = rs[k];
+        slots[k] = (int)(((unsigned)min(v >> 8, 0xffff) << 16) | (geo_off + 32u * (unsigned)(v & 255)));
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1967,17 +1987,18 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
     float c_next;
     asm("v_mov_b32 %0, %1" : "=v"(c_next) : "s"(c[NC - 2]));
     float tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
-    int end = 0, ns = 0;
+    int end = 0;
+    const int *slot = slots + g;  // the row's next slot
     double acc = 0;
     float ea = 0;
     auto enter = [&]() {  // the row's next non-empty entry (or the terminator)
         acc += (double)ea;
         ea = 0;
-        const int v = slots[ns * NF_ROWS + g];
-        ++ns;
-        const int o = v & 255;
-        end = v >> 8;
-        const float4 a = geo[2 * o], b = geo[2 * o + 1];
+        const unsigned v = (unsigned)*slot;
+        slot += NF_ROWS;
+        end = (int)(v >> 16);
+        const float4 *ge = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(nf_lds) + (v & 0xffffu));
+        const float4 a = ge[0], b = ge[1];
         tx = me.x + a.x;
         ty = me.y + a.y;
         tz = me.z + a.z;
@@ -1987,16 +2008,22 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
     };
     enter();
     const int K = it.w;
-    const int *sp = steps + wave_base[item] + lane;
-    int id = sp[0];
-    P4<float> pc = src[id];
-    id = sp[64];
-    for (int k = 0; k < K; k += 4) {
-        const P4<float> pn = src[id];            // next block's point (the padding keeps it in bounds)
-        id = sp[(size_t)(k + 8) * NF_ROWS];      // and the index of the block after it
+    // step indices: block b at sb + 64 b, element `lane`; the loads below take a wave-uniform
+    // base and a 32-bit lane offset (one address instruction per load)
+    const int *sb = steps + wave_base[item];
+    auto index = [&](int b) { return nf_load_u32<int>(sb + (size_t)b * 64, (unsigned)lane * 4u); };
+    auto point = [&](int id) { return nf_load_u32<P4<float>>(src, (unsigned)id * 16u); };
+    // m: ballot of the block's entry flags, bit 4 g + s = row g switches entries at step s
+    auto block = [&](const P4<float> &pc, unsigned long long m, int k) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            if (k + s >= end) enter();           // rows reach their entries' ends at different steps
+            // rows reach their entries' ends at different steps; the wave-wide test is scalar
+            // (the empty asm keeps the compiler from folding the per-lane test into it, which
+            // would issue that test at every step)
+            if ((m >> s) & 0x1111111111111111ull) {
+                asm volatile("");
+                if (k + s >= end) enter();
+            }
             float px, py, pz;
             switch (s) {
             case 0: px = quad_bcast<0>(pc.x); py = quad_bcast<0>(pc.y); pz = quad_bcast<0>(pc.z); break;
@@ -2023,8 +2050,27 @@ __global__ __launch_bounds__(1024) void k_near_rows(const P4<float> *__restrict_
             default: fmac_quad1<3>(ea, pc.w, u); break;
             }
         }
-        pc = pn;
+    };
+    // two blocks per iteration, each block's point loaded one block ahead and its index two
+    // blocks ahead (the padding keeps both in bounds); the two point registers alternate, so
+    // no copies between blocks
+    const int nb = K >> 2;
+    int id = index(0);
+    unsigned long long ma = __ballot(id < 0), mb;
+    P4<float> pa = point(id), pb;
+    id = index(1);
+    int b = 0;
+    for (; b + 2 <= nb; b += 2) {
+        mb = __ballot(id < 0);
+        pb = point(id);
+        id = index(b + 2);
+        block(pa, ma, 4 * b);
+        ma = __ballot(id < 0);
+        pa = point(id);
+        id = index(b + 3);
+        block(pb, mb, 4 * b + 4);
     }
+    if (b < nb) block(pa, ma, 4 * b);
     acc += (double)ea;
     if (active) pot_sorted[it.x + lane] += (float)acc;
 }
@@ -2145,16 +2191,18 @@ __global__ __launch_bounds__(256) void k_near_rows_many(
     const int K = it.w;
     const int *sp = steps + wave_base[item] + lane;
     int id = sp[0];
-    P4<float> pc = src[id];
+    P4<float> pc = nf_load_u32<P4<float>>(src, (unsigned)id * 16u);
     float qc[VC];
 #pragma unroll
-    for (int v = 0; v < VC; v += 4) *reinterpret_cast<float4 *>(&qc[v]) = *reinterpret_cast<const float4 *>(qv + (size_t)id * VC + v);
+    for (int v = 0; v < VC; v += 4)
+        *reinterpret_cast<float4 *>(&qc[v]) = nf_load_u32<float4>(qv + v, (unsigned)id * (unsigned)(VC * sizeof(float)));
     id = sp[64];
     for (int k = 0; k < K; k += 4) {
-        const P4<float> pn = src[id];
+        const P4<float> pn = nf_load_u32<P4<float>>(src, (unsigned)id * 16u);
         float qn[VC];
 #pragma unroll
-        for (int v = 0; v < VC; v += 4) *reinterpret_cast<float4 *>(&qn[v]) = *reinterpret_cast<const float4 *>(qv + (size_t)id * VC + v);
+        for (int v = 0; v < VC; v += 4)
+            *reinterpret_cast<float4 *>(&qn[v]) = nf_load_u32<float4>(qv + v, (unsigned)id * (unsigned)(VC * sizeof(float)));
         id = sp[(size_t)(k + 8) * NF_ROWS];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
@@ -2264,10 +2312,10 @@ __global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restri
     const int K = it.w;
     const int *sp = steps + wave_base[item] + lane;
     int id = sp[0];
-    P4<double> pc = src[id];
+    P4<double> pc = nf_load_u32<P4<double>>(src, (unsigned)id * 32u);
     id = sp[64];
     for (int k = 0; k < K; k += 4) {
-        const P4<double> pn = src[id];           // next block's point (the padding keeps it in bounds)
+        const P4<double> pn = nf_load_u32<P4<double>>(src, (unsigned)id * 32u);  // next block's point (in bounds: padding)
         id = sp[(size_t)(k + 8) * NF_ROWS];      // and the index of the block after it
         pts[2 * lane] = make_double2(pc.x, pc.y);
         pts[2 * lane + 1] = make_double2(pc.z, pc.w);

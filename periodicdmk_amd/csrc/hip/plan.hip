@@ -799,8 +799,15 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 total += (long long)itv[i].w * NF_ROWS;
             }
             n_nf_steps = total / NF_ROWS;  // steps per row summed over the items
-            // beyond int indexing, 8-bit entry ordinals or 23-bit step counts: wavefront lists
-            if (total + NF_ROW_PAD > 0x7fffffffLL || nf_ne_max > 255 || k_item_max > 0x7fff00) nf_rows = false;
+            // beyond int indexing, 8-bit entry ordinals or 23-bit step counts (16-bit for the FP32
+            // kernel's decoded slots), or source byte offsets past 32 bits (the kernels form
+            // them as index * size in 32 bits; the many-vector charges take 64 bytes per
+            // source): wavefront lists
+            const long long k_lim = std::is_same<Real, float>::value ? 0xfff0 : 0x7fff00;
+            const unsigned long long src_bytes = std::is_same<Real, float>::value ? 64 : sizeof(P4<Real>);
+            if (total + NF_ROW_PAD > 0x7fffffffLL || nf_ne_max > 255 || k_item_max > k_lim ||
+                (unsigned long long)(nsall + NF_CHUNK + 1) * src_bytes > 0xffffffffull)
+                nf_rows = false;
             // the row lists take at most a quarter of the free device memory (the plane waves
             // and the rest of the plan come after them)
             size_t mem_free = 0, mem_total = 0;
@@ -1486,7 +1493,9 @@ bool Plan<Real>::many_ensure() {
         return false;
     }
     for (int b = 0; b < 2; ++b) {
-        many_qv[b].alloc((size_t)nsall * VC * sizeof(float));
+        // charges up to the sentinel source of the row lists' padding (zero past nsall)
+        many_qv[b].alloc((size_t)(nsall + NF_CHUNK + 1) * VC * sizeof(float));
+        check(hipMemset(many_qv[b].p, 0, many_qv[b].bytes), "hipMemset");
         many_near[b].alloc((size_t)n * VC * sizeof(float));
         many_far[b].alloc((size_t)n * VC * sizeof(double));
     }
