@@ -2247,8 +2247,9 @@ __global__ __launch_bounds__(256) void k_near_rows_many(
 // points of a block reach its lanes through LDS (64-bit DPP has no quad broadcast): each lane
 // writes the point it loaded, then every lane reads its row's point of the step.  Per
 // wavefront LDS: 64 points (2 KiB), then (ne_max + 1) entries x 48 bytes of geometry and
-// (ne_max + 1) x 16 slots.  Bit-identical to k_near_field<double, NC, false> for the same
-// reason as the FP32 kernel.
+// (ne_max + 1) x 16 slots, decoded in LDS as in the FP32 kernel.  The same loop structure as
+// the FP32 kernel (two blocks per iteration, scalar entry-switch tests).  Bit-identical to
+// k_near_field<double, NC, false> for the same reason as the FP32 kernel.
 __host__ __device__ inline size_t near_rows64_lds_per_wave(int ne_max) {
     return 64 * 2 * sizeof(double2) + (size_t)(ne_max + 1) * 3 * sizeof(double2) + (size_t)(ne_max + 1) * NF_ROWS * 4;
 }
@@ -2280,7 +2281,11 @@ __global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restri
         geo[3 * e + 2] = make_double2(e < ne ? 2.0 / bs : 0.0, -bs / 2.0);
     }
     const int *rs = rslot + (size_t)rs_base[item] * NF_ROWS;
-    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) slots[k] = rs[k];
+    const unsigned geo_off = (unsigned)((const char *)geo - (const char *)nf_lds64);
+    for (int k = lane; k < (ne + 1) * NF_ROWS; k += 64) {
+        const int v = rs[k];
+        slots[k] = (int)(((unsigned)min(v >> 8, 0xffff) << 16) | (geo_off + 48u * (unsigned)(v & 255)));
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -2291,16 +2296,17 @@ __global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restri
     for (int i = 0; i < NC; ++i) c[i] = coeffs[i];
     const double ctop = c[NC - 1] + (double)vgpr_zero();  // leading coefficient in a VGPR (see k_near_field)
     double tx = 0, ty = 0, tz = 0, d2max = 0, rsc = 0, cen = 0;
-    int end = 0, ns = 0;
+    int end = 0;
+    const int *slot = slots + g;  // the row's next slot
     double acc = 0, ea = 0;
     auto enter = [&]() {  // the row's next non-empty entry (or the terminator)
         acc += ea;
         ea = 0;
-        const int v = slots[ns * NF_ROWS + g];
-        ++ns;
-        const int o = v & 255;
-        end = v >> 8;
-        const double2 a = geo[3 * o], b = geo[3 * o + 1], cc = geo[3 * o + 2];
+        const unsigned v = (unsigned)*slot;
+        slot += NF_ROWS;
+        end = (int)(v >> 16);
+        const double2 *ge = reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(nf_lds64) + (v & 0xffffu));
+        const double2 a = ge[0], b = ge[1], cc = ge[2];
         tx = me.x + a.x;
         ty = me.y + a.y;
         tz = me.z + b.x;
@@ -2310,13 +2316,11 @@ __global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restri
     };
     enter();
     const int K = it.w;
-    const int *sp = steps + wave_base[item] + lane;
-    int id = sp[0];
-    P4<double> pc = nf_load_u32<P4<double>>(src, (unsigned)id * 32u);
-    id = sp[64];
-    for (int k = 0; k < K; k += 4) {
-        const P4<double> pn = nf_load_u32<P4<double>>(src, (unsigned)id * 32u);  // next block's point (in bounds: padding)
-        id = sp[(size_t)(k + 8) * NF_ROWS];      // and the index of the block after it
+    const int *sb = steps + wave_base[item];
+    auto index = [&](int b) { return nf_load_u32<int>(sb + (size_t)b * 64, (unsigned)lane * 4u); };
+    auto point = [&](int id) { return nf_load_u32<P4<double>>(src, (unsigned)id * 32u); };
+    // m: ballot of the block's entry flags, bit 4 g + s = row g switches entries at step s
+    auto block = [&](const P4<double> &pc, unsigned long long m, int k) {
         pts[2 * lane] = make_double2(pc.x, pc.y);
         pts[2 * lane + 1] = make_double2(pc.z, pc.w);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -2324,7 +2328,11 @@ __global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restri
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            if (k + s >= end) enter();           // rows reach their entries' ends at different steps
+            // the wave-wide switch test is scalar; the empty asm keeps the per-lane test inside
+            if ((m >> s) & 0x1111111111111111ull) {
+                asm volatile("");
+                if (k + s >= end) enter();
+            }
             const double2 a = pts[2 * (4 * g + s)], b = pts[2 * (4 * g + s) + 1];
             // (source - target: the negated difference, exactly, so r^2 is the same)
             const double dx = a.x - tx, dy = a.y - ty, dz = b.x - tz;
@@ -2341,8 +2349,25 @@ __global__ __launch_bounds__(1024) void k_near_rows64(const P4<double> *__restri
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();  // every lane has read the block before it is overwritten
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        pc = pn;
+    };
+    // two blocks per iteration, points one block ahead, indices two (see k_near_rows)
+    const int nb = K >> 2;
+    int id = index(0);
+    unsigned long long ma = __ballot(id < 0), mb;
+    P4<double> pa = point(id), pb;
+    id = index(1);
+    int b = 0;
+    for (; b + 2 <= nb; b += 2) {
+        mb = __ballot(id < 0);
+        pb = point(id);
+        id = index(b + 2);
+        block(pa, ma, 4 * b);
+        ma = __ballot(id < 0);
+        pa = point(id);
+        id = index(b + 3);
+        block(pb, mb, 4 * b + 4);
     }
+    if (b < nb) block(pa, ma, 4 * b);
     acc += ea;
     if (active) pot_sorted[it.x + lane] += acc;
 }

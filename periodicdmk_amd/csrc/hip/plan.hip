@@ -799,11 +799,11 @@ Plan<Real>::Plan(const double *cell, const double *pos, int N, int n_digits_user
                 total += (long long)itv[i].w * NF_ROWS;
             }
             n_nf_steps = total / NF_ROWS;  // steps per row summed over the items
-            // beyond int indexing, 8-bit entry ordinals or 23-bit step counts (16-bit for the FP32
-            // kernel's decoded slots), or source byte offsets past 32 bits (the kernels form
-            // them as index * size in 32 bits; the many-vector charges take 64 bytes per
+            // beyond int indexing, 8-bit entry ordinals, 16-bit step counts (the kernels decode
+            // slots into 16-bit end steps), or source byte offsets past 32 bits (the kernels
+            // form them as index * size in 32 bits; the many-vector charges take 64 bytes per
             // source): wavefront lists
-            const long long k_lim = std::is_same<Real, float>::value ? 0xfff0 : 0x7fff00;
+            const long long k_lim = 0xfff0;
             const unsigned long long src_bytes = std::is_same<Real, float>::value ? 64 : sizeof(P4<Real>);
             if (total + NF_ROW_PAD > 0x7fffffffLL || nf_ne_max > 255 || k_item_max > k_lim ||
                 (unsigned long long)(nsall + NF_CHUNK + 1) * src_bytes > 0xffffffffull)
